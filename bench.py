@@ -1,0 +1,146 @@
+"""Throughput of the n-body hot path on one MI355X.
+
+Default workload (the reference's Barnes-Hut configuration used for the
+headline metric): star + disk, N = 1,048,576, theta = 0.5, dt = 1 h, FP32 pair
+forces with FP64 accumulation, FP64 octree sums and integrator.  A step is one
+iteration of the reference's loop: kick-drift, octree build, forces, kick.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "n-body-simulation_amd"))
+
+import nbody as nb  # noqa: E402
+
+METRIC = "body-steps/s at median rel. force err ≤1e-4 vs FP64 direct sum"
+FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention, reference README.md:158
+ERR_SAMPLE = 16384
+
+WORKLOADS = {
+    # name: (generator, n, seed, algorithm, theta, dt, precision)
+    "bh_disk_1m": ("disk", 1 << 20, 0x4E424F44590001, "BarnesHut", 0.5, 1.0 / 24, nb.PREC_FP32C),
+    "naive_disk_16k": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP64),
+    "naive_disk_16k_fp32": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP32C),
+    "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", 0.5, 1e-3, nb.PREC_FP32C),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gen(kind, n, seed):
+    return nb.gen_star_disk(n, seed) if kind == "disk" else nb.gen_plummer(n, seed)
+
+
+def median_rel_err(ctx, n, rng):
+    """Median relative error of the current accelerations against the FP64
+    direct sum (reference arithmetic, on the GPU) over sampled targets."""
+    ax, ay, az = ctx.accel()
+    t = np.arange(n, dtype=np.uint32) if n <= 65536 else \
+        np.sort(rng.choice(n, ERR_SAMPLE, replace=False)).astype(np.uint32)
+    rx, ry, rz = ctx.forces_direct_targets(t)
+    e = np.sqrt((ax[t] - rx) ** 2 + (ay[t] - ry) ** 2 + (az[t] - rz) ** 2) / \
+        np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    return float(np.median(e)), int(t.size)
+
+
+def cpu_baseline(d, algorithm, theta, dt):
+    """The reference's CPU path (oracle/ restatement, OpenMP on all host cores),
+    one loop iteration of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    n = d["m"].size
+    threads = orc.get_threads()
+    log(f"cpu baseline: {algorithm} n={n}, 1 loop iteration on {threads} threads ...")
+    r = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt, dt,
+                     theta=theta)
+    secs = r["loop_ms"] / 1000.0
+    return {"value": n * r["steps"] / secs, "unit": "body-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{r['steps']} loop iteration(s) (kick-drift, forces"
+                      f"{', octree build' if algorithm == 'BarnesHut' else ''}, kick) of the same "
+                      f"n={n} input, FP64, t=0 force untimed; {secs:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
+              f"--gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    kind, n, seed, algorithm, theta, dt, prec = WORKLOADS[args.workload]
+    log(f"device: {nb.device_name()}")
+    log(f"workload {args.workload}: generating {kind} n={n} ...")
+    d = gen(kind, n, seed)
+    ctx = nb.Context(n, theta=theta, precision=prec)
+    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    # t = 0 forces (outside the loop, as in the reference)
+    if algorithm == "BarnesHut":
+        ctx.bh_build()
+        ctx.bh_forces()
+    else:
+        ctx.forces_direct()
+    if args.warmup:
+        ctx.steps(algorithm, dt, args.warmup)
+    log(f"timing {args.steps} steps ...")
+    t0 = time.perf_counter()
+    ctx.steps(algorithm, dt, args.steps)  # returns after the stream is idle
+    t1 = time.perf_counter()
+    phases = ctx.phase_ms()
+    secs = t1 - t0
+    value = n * args.steps / secs
+    inter = ctx.interactions().astype(np.float64) if algorithm == "BarnesHut" else None
+    err, sample = median_rel_err(ctx, n, np.random.default_rng(1))
+    log(f"median rel err {err:.3e} over {sample} targets")
+
+    # roofline of the dominant kernel: the force kernel
+    accel_ms = phases["Acceleration Kernel Time"] / args.steps
+    if algorithm == "BarnesHut":
+        flops = FLOP_PER_INTERACTION * float(inter.sum())
+    else:
+        flops = FLOP_PER_INTERACTION * float(n) * float(n)
+    achieved = flops / (accel_ms * 1e-3) / 1e12
+    peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": None,
+            "kernel": "k_bh_group" if algorithm == "BarnesHut" else
+            ("k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"),
+            "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops}
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == nb.PREC_FP32C else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "bodies": n, "distribution": kind,
+                   "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
+                   "dt_days": dt, "median_rel_err": err, "err_sample": sample,
+                   "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
+                   "mean_interactions": None if inter is None else float(inter.mean())},
+        "roofline": roof,
+    }
+    ctx.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(d, algorithm, theta, dt)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

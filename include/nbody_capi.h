@@ -1,0 +1,179 @@
+/*
+ * nbody_capi.h -- C ABI of libnbody_hip.so, the MI355X (gfx950) n-body hot path.
+ *
+ * The reference (TimThuering/N-Body-Simulation) has no C boundary: its hot
+ * path sits behind the C++ class nBodyAlgorithm and its two subclasses
+ * (src/simulationBackend/nBodyAlgorithm.hpp:19-187, NaiveAlgorithm.hpp,
+ * BarnesHutAlgorithm.hpp), which own SYCL buffers and run every kernel with
+ * queue.submit(...).wait().  This header is the replacement seam: the
+ * re-created host classes (n-body-simulation_amd/host/) call these entry
+ * points, and so can any FFI (ctypes, cgo, JNI).  Each entry point names the
+ * reference function it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes; all pointers are HOST pointers unless the name
+ *    says _device.
+ *  - Every call is synchronous (it returns after its GPU work finished), the
+ *    same contract as the reference's .wait() after each submit.
+ *  - Every call returns NB_OK (0) or an NB_E_* code; nb_last_error() gives a
+ *    message.  Device-side failures (octree storage exhausted, bodies that
+ *    cannot be separated, zero-mass cells, traversal stack exhausted) are
+ *    reported as codes instead of the reference's silent corruption or hang.
+ *  - One host thread per context.  A context owns one HIP stream on the
+ *    device that was current when it was created.
+ */
+#ifndef NBODY_CAPI_H
+#define NBODY_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    NB_OK = 0,
+    NB_E_INVALID_ARG = 1,
+    NB_E_HIP = 2,             /* a HIP runtime call failed */
+    NB_E_NODE_OVERFLOW = 3,   /* octree needs more nodes than storage_size_param allows */
+    NB_E_STACK_OVERFLOW = 4,  /* per-body traversal stack exhausted */
+    NB_E_DEPTH = 5,           /* two bodies share all 42 octree digits (coincident bodies) */
+    NB_E_ZERO_MASS = 6,       /* a non-empty cell has zero mass (reference: endless COM loop) */
+    NB_E_NOT_BUILT = 7,       /* forces requested before an octree was built */
+    NB_E_AABB = 8,            /* bounding box has no longest edge (non-finite positions) */
+    NB_E_NO_DEVICE = 10
+};
+
+/* ---- parameters ---------------------------------------------------------- */
+enum {
+    /* Reference arithmetic: FP64 pair forces, the reference's summation order
+     * and per-body opening test. */
+    NB_PREC_FP64 = 0,
+    /* FP32 pair arithmetic with FP64 accumulation of per-tile FP32 partials;
+     * Barnes-Hut opens a cell when any body of a 64-body group needs it. */
+    NB_PREC_FP32C = 1
+};
+
+enum { NB_ALGO_NAIVE = 0, NB_ALGO_BARNES_HUT = 1 };
+
+typedef struct nb_params {
+    double G;             /* nBodyAlgorithm.hpp:55-61 (AU^3 kg^-1 d^-2 by default) */
+    double eps2;          /* configuration::epsilon2, Configuration.cpp:6 */
+    double theta;         /* configuration::barnes_hut_algorithm::theta, Configuration.cpp:10 */
+    uint32_t precision;   /* NB_PREC_* for forces (integrator, AABB, COM are always FP64) */
+    uint32_t sort_bodies; /* configuration::barnes_hut_algorithm::sortBodies, Configuration.cpp:15 */
+    uint32_t storage_size_param; /* main.cpp:122-127; node capacity = param * N (default 16) */
+    uint32_t stack_size_param;   /* main.cpp:129-134; traversal stack rule (default 16) */
+} nb_params;
+
+/* Fill *p with the reference defaults (G, eps2 = 1e-22, theta = 1.05,
+ * NB_PREC_FP64, sorting on, storage 16, stack 16; Configuration.cpp:5-33). */
+void nb_default_params(nb_params *p);
+
+/* ---- context ------------------------------------------------------------- */
+typedef struct nb_ctx nb_ctx;
+
+/* Allocates device state for n bodies (the reference allocates in the
+ * nBodyAlgorithm/BarnesHutOctree constructors, BarnesHutOctree.cpp:5-43,
+ * BarnesHutAlgorithm.cpp:8-15). */
+int nb_create(nb_ctx **out, uint32_t n, const nb_params *p);
+int nb_destroy(nb_ctx *ctx);
+int nb_set_params(nb_ctx *ctx, const nb_params *p);
+int nb_get_params(const nb_ctx *ctx, nb_params *p);
+uint32_t nb_body_count(const nb_ctx *ctx);
+const char *nb_last_error(const nb_ctx *ctx);
+const char *nb_strerror(int code);
+/* Name of the device the library runs on (info::device::name in the reference). */
+int nb_device_name(char *buf, size_t len);
+
+/* Host -> device copy of the initial state (NaiveAlgorithm.cpp:27-66). */
+int nb_set_state(nb_ctx *ctx, const double *x, const double *y, const double *z,
+                 const double *vx, const double *vy, const double *vz, const double *m);
+/* Device -> host copies (the reference's host_accessor reads,
+ * NaiveAlgorithm.cpp:172-180, :233-242).  Any pointer may be NULL. */
+int nb_get_state(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy,
+                 double *vz);
+int nb_get_accel(nb_ctx *ctx, double *ax, double *ay, double *az);
+/* Overwrite positions only (used by tests to evaluate forces on given points). */
+int nb_set_positions(nb_ctx *ctx, const double *x, const double *y, const double *z);
+
+/* ---- hot path ------------------------------------------------------------ */
+/* Direct-sum accelerations at the current positions
+ * (NaiveAlgorithm::computeAccelerations_opt_{0,1,2}, NaiveAlgorithm.cpp:262-482). */
+int nb_forces_direct(nb_ctx *ctx);
+/* FP64 direct sum for the listed targets only, in reference arithmetic; the
+ * FP64 error reference on a sample of bodies.  Outputs have nt entries. */
+int nb_forces_direct_targets(nb_ctx *ctx, uint32_t nt, const uint32_t *targets, double *ax,
+                             double *ay, double *az);
+/* Octree for the current positions: AABB, octant keys, radix sort, topology,
+ * centre of mass, body order (BarnesHutOctree::buildOctree,
+ * BarnesHutOctree.hpp:102-103; ParallelOctreeTopDownSubtrees.cpp:15-93). */
+int nb_bh_build(nb_ctx *ctx);
+/* Barnes-Hut accelerations from the last octree
+ * (BarnesHutAlgorithm::computeAccelerations, BarnesHutAlgorithm.cpp:280-401). */
+int nb_bh_forces(nb_ctx *ctx);
+/* Leapfrog part 1: v_half = v + a*dt/2; x += v_half*dt (NaiveAlgorithm.cpp:140-164). */
+int nb_kick_drift(nb_ctx *ctx, double dt);
+/* Leapfrog part 2: v = v_half + a*dt/2 (NaiveAlgorithm.cpp:206-222). */
+int nb_kick(nb_ctx *ctx, double dt);
+/* `count` iterations of the reference's loop body (NaiveAlgorithm.cpp:131-259,
+ * BarnesHutAlgorithm.cpp:149-277 without the visualisation copies):
+ * kick-drift, [octree build], forces, kick.  One host synchronisation at the
+ * end.  algorithm = NB_ALGO_*. */
+int nb_steps(nb_ctx *ctx, int algorithm, double dt, uint32_t count);
+
+/* ---- timing (TimeMeasurement keys, BarnesHutAlgorithm.cpp:81-100) --------- */
+enum {
+    NB_PHASE_AABB = 0,        /* "AABB creation" */
+    NB_PHASE_KEYS_SORT = 1,   /* keys + radix sort ("Build octree to level" + "Sort bodies for subtrees") */
+    NB_PHASE_TOPOLOGY = 2,    /* LCP, node numbering, linking ("Build subtrees") */
+    NB_PHASE_COM = 3,         /* "Compute center of mass" */
+    NB_PHASE_OCTREE = 4,      /* "Octree creation" (sum of the above + node records) */
+    NB_PHASE_ACCEL = 5,       /* "Acceleration Kernel Time" */
+    NB_PHASE_LF1 = 6,         /* "Leapfrog Part 1" */
+    NB_PHASE_LF2 = 7,         /* "Leapfrog Part 2" */
+    NB_PHASE_COUNT = 8
+};
+/* GPU time (HIP events) of each phase summed over the last nb_* call;
+ * ms must hold NB_PHASE_COUNT doubles. */
+int nb_phase_ms(const nb_ctx *ctx, double *ms);
+
+/* ---- octree inspection (parity checks) --------------------------------- */
+/* aabb = {min_x, min_y, min_z, max_x, max_y, max_z, edge} (BarnesHutOctree.hpp:22-29). */
+int nb_tree_info(nb_ctx *ctx, uint64_t *node_count, uint32_t *internal_count,
+                 uint32_t *max_depth, double aabb[7]);
+/* sortedBodiesInOrder (BarnesHutOctree.hpp:93-95): body ids in octree in-order. */
+int nb_tree_sorted(nb_ctx *ctx, uint32_t *sorted);
+/* Canonical form of the octree: depth-first pre-order from the root with
+ * children in octant-digit order 0..7, one record per node.  kind: 0 empty
+ * leaf, 1 body leaf, 2 internal; body = N unless kind == 1; mass and com_* are
+ * the reference's SUM_MASSES and centre-of-mass numerators, count its
+ * bodyCountNode.  Returns the record count (0 if cap is too small). */
+uint64_t nb_tree_canonical(nb_ctx *ctx, uint64_t cap, uint8_t *depth, uint8_t *kind,
+                           uint32_t *body, double *mass, double *com_x, double *com_y,
+                           double *com_z, uint32_t *count);
+/* Accepted interactions (cells + bodies) per body in the last nb_bh_forces,
+ * indexed by body id. */
+int nb_bh_interactions(nb_ctx *ctx, uint32_t *count);
+
+/* ---- synthetic inputs (seeded; same bits on every machine) -------------- */
+/* One star of 1 Msun at rest at the origin plus n-1 bodies on Kepler orbits:
+ * a ~ U[2.1, 3.3] AU, e ~ U[0, 0.3], i ~ U[0, 20 deg], angles ~ U[0, 2pi),
+ * m = 10^U[-20,-12] Msun; Kepler -> Cartesian as in the reference's dataset
+ * converter (dataset_converter/src/conversion.cpp:13-89).  Units kg, AU, AU/d. */
+int nb_gen_star_disk(uint32_t n, uint64_t seed, double *x, double *y, double *z, double *vx,
+                     double *vy, double *vz, double *m);
+/* Plummer sphere (Aarseth, Henon & Wielen 1974), scale radius 1, total mass 1
+ * in units where G = 1, equal masses, radius truncated at 100, centre of mass
+ * at rest at the origin.  Masses are written as (1/n)/G_ref kg so that the
+ * reference's fixed G reproduces G = 1 in these units. */
+int nb_gen_plummer(uint32_t n, uint64_t seed, double *x, double *y, double *z, double *vx,
+                   double *vy, double *vz, double *m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

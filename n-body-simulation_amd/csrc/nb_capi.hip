@@ -1,0 +1,638 @@
+// nb_capi.hip -- the extern "C" boundary of libnbody_hip.so (include/nbody_capi.h).
+//
+// A context owns the device state of one system and one HIP stream.  Each
+// entry point enqueues its kernels, synchronises once, converts the device
+// status word into an NB_E_* code and returns -- the contract of the
+// reference's queue.submit(...).wait().
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nb_internal.h"
+
+using nb::TreeBufs;
+using nb::TreeHdr;
+
+struct nb_ctx {
+    uint32_t n = 0;
+    nb_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *vhx = nullptr, *vhy = nullptr, *vhz = nullptr;
+    double *ax = nullptr, *ay = nullptr, *az = nullptr;
+    double *m = nullptr;
+    uint32_t *inter = nullptr;
+    TreeBufs tree;
+    bool tree_valid = false;
+    TreeHdr *hdr_host = nullptr;  // pinned copy of tree.hdr
+    uint32_t *exact_stack = nullptr;
+    uint32_t exact_cap = 0;
+    double *f32_scratch = nullptr;
+    uint32_t f32_splits = 0;
+    uint32_t *targets_dev = nullptr;
+    double *targets_out = nullptr;
+    uint32_t targets_cap = 0;
+    std::vector<hipEvent_t> events;
+    std::vector<std::pair<int, size_t>> marks;  // (phase, index of start event)
+    double phase[NB_PHASE_COUNT] = {0};
+    std::vector<void *> allocations;
+    std::string err;
+};
+
+namespace {
+
+int fail(nb_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define NB_HIP(ctx, call)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, NB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dev_alloc(nb_ctx *c, T **ptr, uint64_t count) {
+    void *p = nullptr;
+    const uint64_t bytes = std::max<uint64_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess)
+        return fail(c, NB_E_HIP, "hipMalloc(" + std::to_string(bytes) + " B): " + hipGetErrorString(e));
+    c->allocations.push_back(p);
+    *ptr = static_cast<T *>(p);
+    return NB_OK;
+}
+
+void dev_free(nb_ctx *c, void *p) {
+    if (!p) return;
+    auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+    if (it != c->allocations.end()) c->allocations.erase(it);
+    (void)hipFree(p);
+}
+
+// Phase timing with HIP events on the context stream.
+void mark_begin(nb_ctx *c, int phase) {
+    const size_t idx = c->marks.size() * 2;
+    while (c->events.size() < idx + 2) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        c->events.push_back(e);
+    }
+    c->marks.push_back({phase, idx});
+    (void)hipEventRecord(c->events[idx], c->stream);
+}
+
+void mark_end(nb_ctx *c) {
+    if (c->marks.empty()) return;
+    (void)hipEventRecord(c->events[c->marks.back().second + 1], c->stream);
+}
+
+void timing_reset(nb_ctx *c) {
+    c->marks.clear();
+    for (double &v : c->phase) v = 0;
+}
+
+void timing_collect(nb_ctx *c) {
+    for (auto &mk : c->marks) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->events[mk.second], c->events[mk.second + 1]) == hipSuccess)
+            c->phase[mk.first] += ms;
+    }
+    c->marks.clear();
+}
+
+int status_to_code(nb_ctx *c, uint32_t st) {
+    if (st & nb::kFlagAabb) return fail(c, NB_E_AABB, "bounding box: non-finite positions");
+    if (st & nb::kFlagDepth)
+        return fail(c, NB_E_DEPTH, "two bodies share all 42 octant digits (coincident bodies)");
+    if (st & nb::kFlagNodeOverflow)
+        return fail(c, NB_E_NODE_OVERFLOW, "octree needs more nodes than storage_size_param * N");
+    if (st & nb::kFlagZeroMass)
+        return fail(c, NB_E_ZERO_MASS, "a non-empty octree cell has zero mass");
+    if (st & nb::kFlagStack) return fail(c, NB_E_STACK_OVERFLOW, "traversal stack exhausted");
+    return NB_OK;
+}
+
+// Synchronise the stream, fetch the octree header and turn flags into a code.
+int finish(nb_ctx *c, bool read_tree_status) {
+    if (read_tree_status)
+        NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                                 c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    timing_collect(c);
+    if (read_tree_status) return status_to_code(c, c->hdr_host->status);
+    return NB_OK;
+}
+
+uint32_t exact_stack_cap(const nb_ctx *c) {
+    // the reference's stackSize (Configuration.cpp:24-33), raised to the
+    // largest depth-first stack a depth-42 octree can need (1 + 7*42)
+    const uint32_t n = c->n;
+    const uint32_t lg = (uint32_t)std::ceil(std::log2((double)std::max<uint32_t>(n, 2)));
+    uint32_t ref = c->p.stack_size_param * lg + (n < 15000 ? 500u : 0u);
+    return std::max<uint32_t>(ref, 1 + 7 * nb::kMaxDepth + 8);
+}
+
+int alloc_tree(nb_ctx *c) {
+    TreeBufs &t = c->tree;
+    const uint32_t n = c->n;
+    t.n = n;
+    const uint64_t storage = (uint64_t)std::max<uint32_t>(c->p.storage_size_param, 1) * n;
+    t.icap = (uint32_t)std::max<uint64_t>(1, storage / 8);
+    t.mcap = 1 + 8ull * t.icap;
+    int rc;
+    if ((rc = dev_alloc(c, &t.hdr, 1))) return rc;
+    if ((rc = dev_alloc(c, &t.aabb_part, 6ull * nb::kAabbBlocks))) return rc;
+    if ((rc = dev_alloc(c, &t.skey, n))) return rc;
+    if ((rc = dev_alloc(c, &t.skey_alt, n))) return rc;
+    if ((rc = dev_alloc(c, &t.klo, n))) return rc;
+    if ((rc = dev_alloc(c, &t.sidx, n))) return rc;
+    if ((rc = dev_alloc(c, &t.sidx_alt, n))) return rc;
+    if ((rc = dev_alloc(c, &t.lcp, n))) return rc;
+    if ((rc = dev_alloc(c, &t.chain, n))) return rc;
+    if ((rc = dev_alloc(c, &t.body_parent, n))) return rc;
+    if ((rc = dev_alloc(c, &t.int_depth, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_parent, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_node, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_nchild, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_arrive, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_count, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_sums, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.ref, t.mcap))) return rc;
+    if ((rc = dev_alloc(c, &t.cm, t.mcap))) return rc;
+    if ((rc = dev_alloc(c, &t.edge, t.mcap))) return rc;
+    if ((rc = dev_alloc(c, &t.ropen2, t.mcap))) return rc;
+    t.radix_blocks = nb::radix_blocks_for(n);
+    if ((rc = dev_alloc(c, &t.radix_hist, 256ull * t.radix_blocks))) return rc;
+    t.scan_part_cap = std::max(nb::scan_scratch_words(256u * t.radix_blocks),
+                               nb::scan_scratch_words(n));
+    if ((rc = dev_alloc(c, &t.scan_part, t.scan_part_cap))) return rc;
+    NB_HIP(c, hipMemsetAsync(t.hdr, 0, sizeof(TreeHdr), c->stream));
+    return NB_OK;
+}
+
+bool params_ok(const nb_params *p) {
+    return p && (p->precision == NB_PREC_FP64 || p->precision == NB_PREC_FP32C) &&
+           p->storage_size_param >= 1 && p->stack_size_param >= 1 && std::isfinite(p->G) &&
+           std::isfinite(p->eps2) && p->eps2 >= 0 && std::isfinite(p->theta) && p->theta >= 0;
+}
+
+int build_tree(nb_ctx *c) {
+    TreeBufs &t = c->tree;
+    mark_begin(c, NB_PHASE_AABB);
+    nb::launch_aabb(t, c->x, c->y, c->z, c->stream);
+    mark_end(c);
+    mark_begin(c, NB_PHASE_KEYS_SORT);
+    nb::launch_keys(t, c->x, c->y, c->z, c->stream);
+    mark_end(c);
+    mark_begin(c, NB_PHASE_TOPOLOGY);
+    nb::launch_topology(t, c->x, c->y, c->z, c->stream);
+    mark_end(c);
+    mark_begin(c, NB_PHASE_COM);
+    nb::launch_com(t, c->x, c->y, c->z, c->m, c->stream);
+    nb::launch_node_records(t, c->x, c->y, c->z, c->m, c->p.G, c->p.theta, c->stream);
+    mark_end(c);
+    return NB_OK;
+}
+
+int bh_forces_enqueue(nb_ctx *c) {
+    mark_begin(c, NB_PHASE_ACCEL);
+    if (c->p.precision == NB_PREC_FP64) {
+        const uint32_t cap = exact_stack_cap(c);
+        if (!c->exact_stack || c->exact_cap != cap) {
+            dev_free(c, c->exact_stack);
+            c->exact_stack = nullptr;
+            int rc = dev_alloc(c, &c->exact_stack, (uint64_t)cap * c->n);
+            if (rc) return rc;
+            c->exact_cap = cap;
+        }
+        nb::launch_bh_exact(c->tree, c->x, c->y, c->z, c->p.G, c->p.eps2, c->p.theta,
+                            c->p.sort_bodies, c->exact_stack, c->exact_cap, c->ax, c->ay, c->az,
+                            c->inter, c->stream);
+    } else {
+        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->p.G, c->p.eps2, c->p.theta, c->ax,
+                            c->ay, c->az, c->inter, c->stream);
+    }
+    mark_end(c);
+    return NB_OK;
+}
+
+int direct_enqueue(nb_ctx *c) {
+    mark_begin(c, NB_PHASE_ACCEL);
+    if (c->p.precision == NB_PREC_FP64) {
+        nb::launch_direct_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay, c->az,
+                              c->stream);
+    } else {
+        const uint32_t splits = nb::direct_f32c_splits(c->n);
+        if (!c->f32_scratch || c->f32_splits != splits) {
+            dev_free(c, c->f32_scratch);
+            c->f32_scratch = nullptr;
+            // 3*n*splits partials + alignment pad + n float4 (= 2 doubles each)
+            int rc = dev_alloc(c, &c->f32_scratch, 3ull * c->n * splits + 2ull * c->n + 2);
+            if (rc) return rc;
+            c->f32_splits = splits;
+        }
+        nb::launch_direct_f32c(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay,
+                               c->az, c->f32_scratch, c->f32_splits, c->stream);
+    }
+    mark_end(c);
+    return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void nb_default_params(nb_params *p) {
+    if (!p) return;
+    // nBodyAlgorithm.hpp:55-61
+    double G = 6.67428 * std::pow(10, -11);
+    const double meter_AU = 1.0 / (1.49597870691 * std::pow(10, 11));
+    const double second_Days = 1.0 / 86400;
+    G = G * (std::pow(meter_AU, 3) / std::pow(second_Days, 2));
+    p->G = G;
+    p->eps2 = std::pow(10, -22);  // Configuration.cpp:6
+    p->theta = 1.05;              // Configuration.cpp:10
+    p->precision = NB_PREC_FP64;
+    p->sort_bodies = 1;
+    p->storage_size_param = 16;
+    p->stack_size_param = 16;
+}
+
+const char *nb_strerror(int code) {
+    switch (code) {
+        case NB_OK: return "ok";
+        case NB_E_INVALID_ARG: return "invalid argument";
+        case NB_E_HIP: return "HIP runtime error";
+        case NB_E_NODE_OVERFLOW: return "octree storage exhausted";
+        case NB_E_STACK_OVERFLOW: return "traversal stack exhausted";
+        case NB_E_DEPTH: return "bodies not separable within 42 octree levels";
+        case NB_E_ZERO_MASS: return "zero-mass octree cell";
+        case NB_E_NOT_BUILT: return "octree not built for the current positions";
+        case NB_E_AABB: return "bounding box undefined (non-finite positions)";
+        case NB_E_NO_DEVICE: return "no HIP device";
+        default: return "unknown error";
+    }
+}
+
+const char *nb_last_error(const nb_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int nb_device_name(char *buf, size_t len) {
+    if (!buf || len == 0) return NB_E_INVALID_ARG;
+    int dev = 0, count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return NB_E_NO_DEVICE;
+    if (hipGetDevice(&dev) != hipSuccess) return NB_E_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return NB_E_HIP;
+    std::snprintf(buf, len, "%s (%s)", prop.name, prop.gcnArchName);
+    return NB_OK;
+}
+
+int nb_create(nb_ctx **out, uint32_t n, const nb_params *p) {
+    if (!out || n == 0 || !params_ok(p)) return NB_E_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return NB_E_NO_DEVICE;
+    nb_ctx *c = new nb_ctx();
+    c->n = n;
+    c->p = *p;
+    int rc = NB_OK;
+    auto bail = [&](int code) {
+        nb_destroy(c);
+        return code;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(NB_E_HIP);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(NB_E_HIP);
+    double **state[] = {&c->x, &c->y, &c->z, &c->vx, &c->vy, &c->vz, &c->vhx,
+                        &c->vhy, &c->vhz, &c->ax, &c->ay, &c->az, &c->m};
+    for (double **s : state)
+        if ((rc = dev_alloc(c, s, n))) return bail(rc);
+    if ((rc = dev_alloc(c, &c->inter, n))) return bail(rc);
+    if (hipHostMalloc((void **)&c->hdr_host, sizeof(TreeHdr), hipHostMallocDefault) != hipSuccess)
+        return bail(NB_E_HIP);
+    std::memset(c->hdr_host, 0, sizeof(TreeHdr));
+    if ((rc = alloc_tree(c))) return bail(rc);
+    for (double **s : state)
+        if (hipMemsetAsync(*s, 0, (size_t)n * sizeof(double), c->stream) != hipSuccess)
+            return bail(NB_E_HIP);
+    if (hipMemsetAsync(c->inter, 0, (size_t)n * sizeof(uint32_t), c->stream) != hipSuccess)
+        return bail(NB_E_HIP);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(NB_E_HIP);
+    *out = c;
+    return NB_OK;
+}
+
+int nb_destroy(nb_ctx *c) {
+    if (!c) return NB_E_INVALID_ARG;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void *p : c->allocations) (void)hipFree(p);
+    for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
+    if (c->hdr_host) (void)hipHostFree(c->hdr_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NB_OK;
+}
+
+int nb_set_params(nb_ctx *c, const nb_params *p) {
+    if (!c || !params_ok(p)) return fail(c, NB_E_INVALID_ARG, "invalid parameters");
+    const bool storage_changed = p->storage_size_param != c->p.storage_size_param;
+    c->p = *p;
+    if (storage_changed) {
+        // reallocate octree buffers for the new capacity
+        TreeBufs &t = c->tree;
+        void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
+                        t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
+                        t.int_nchild, t.int_arrive, t.int_count, t.int_sums, t.ref, t.cm,
+                        t.edge, t.ropen2, t.radix_hist, t.scan_part};
+        (void)hipStreamSynchronize(c->stream);
+        for (void *b : bufs) dev_free(c, b);
+        c->tree = TreeBufs();
+        c->tree_valid = false;
+        int rc = alloc_tree(c);
+        if (rc) return rc;
+        NB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    c->tree_valid = false;  // opening radii depend on theta
+    return NB_OK;
+}
+
+int nb_get_params(const nb_ctx *c, nb_params *p) {
+    if (!c || !p) return NB_E_INVALID_ARG;
+    *p = c->p;
+    return NB_OK;
+}
+
+uint32_t nb_body_count(const nb_ctx *c) { return c ? c->n : 0; }
+
+int nb_set_state(nb_ctx *c, const double *x, const double *y, const double *z, const double *vx,
+                 const double *vy, const double *vz, const double *m) {
+    if (!c || !x || !y || !z || !vx || !vy || !vz || !m)
+        return fail(c, NB_E_INVALID_ARG, "nb_set_state: null pointer");
+    const size_t bytes = (size_t)c->n * sizeof(double);
+    const double *src[] = {x, y, z, vx, vy, vz, m};
+    double *dst[] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m};
+    for (int k = 0; k < 7; ++k)
+        NB_HIP(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyHostToDevice, c->stream));
+    NB_HIP(c, hipMemsetAsync(c->ax, 0, bytes, c->stream));
+    NB_HIP(c, hipMemsetAsync(c->ay, 0, bytes, c->stream));
+    NB_HIP(c, hipMemsetAsync(c->az, 0, bytes, c->stream));
+    c->tree_valid = false;
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_set_positions(nb_ctx *c, const double *x, const double *y, const double *z) {
+    if (!c || !x || !y || !z) return fail(c, NB_E_INVALID_ARG, "nb_set_positions: null pointer");
+    const size_t bytes = (size_t)c->n * sizeof(double);
+    NB_HIP(c, hipMemcpyAsync(c->x, x, bytes, hipMemcpyHostToDevice, c->stream));
+    NB_HIP(c, hipMemcpyAsync(c->y, y, bytes, hipMemcpyHostToDevice, c->stream));
+    NB_HIP(c, hipMemcpyAsync(c->z, z, bytes, hipMemcpyHostToDevice, c->stream));
+    c->tree_valid = false;
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_get_state(nb_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
+                 double *vz) {
+    if (!c) return NB_E_INVALID_ARG;
+    const size_t bytes = (size_t)c->n * sizeof(double);
+    double *dst[] = {x, y, z, vx, vy, vz};
+    const double *src[] = {c->x, c->y, c->z, c->vx, c->vy, c->vz};
+    for (int k = 0; k < 6; ++k)
+        if (dst[k]) NB_HIP(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_get_accel(nb_ctx *c, double *ax, double *ay, double *az) {
+    if (!c) return NB_E_INVALID_ARG;
+    const size_t bytes = (size_t)c->n * sizeof(double);
+    double *dst[] = {ax, ay, az};
+    const double *src[] = {c->ax, c->ay, c->az};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k]) NB_HIP(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_forces_direct(nb_ctx *c) {
+    if (!c) return NB_E_INVALID_ARG;
+    timing_reset(c);
+    int rc = direct_enqueue(c);
+    if (rc) return rc;
+    return finish(c, false);
+}
+
+int nb_forces_direct_targets(nb_ctx *c, uint32_t nt, const uint32_t *targets, double *ax,
+                             double *ay, double *az) {
+    if (!c || (nt > 0 && (!targets || !ax || !ay || !az)))
+        return fail(c, NB_E_INVALID_ARG, "nb_forces_direct_targets: bad arguments");
+    if (nt == 0) return NB_OK;
+    for (uint32_t t = 0; t < nt; ++t)
+        if (targets[t] >= c->n) return fail(c, NB_E_INVALID_ARG, "target index out of range");
+    if (nt > c->targets_cap) {
+        dev_free(c, c->targets_dev);
+        dev_free(c, c->targets_out);
+        c->targets_dev = nullptr;
+        c->targets_out = nullptr;
+        int rc;
+        if ((rc = dev_alloc(c, &c->targets_dev, nt))) return rc;
+        if ((rc = dev_alloc(c, &c->targets_out, 3ull * nt))) return rc;
+        c->targets_cap = nt;
+    }
+    timing_reset(c);
+    NB_HIP(c, hipMemcpyAsync(c->targets_dev, targets, (size_t)nt * 4, hipMemcpyHostToDevice, c->stream));
+    mark_begin(c, NB_PHASE_ACCEL);
+    nb::launch_direct_targets_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, nt,
+                                  c->targets_dev, c->targets_out, c->targets_out + nt,
+                                  c->targets_out + 2ull * nt, c->stream);
+    mark_end(c);
+    double *dst[] = {ax, ay, az};
+    for (int k = 0; k < 3; ++k)
+        NB_HIP(c, hipMemcpyAsync(dst[k], c->targets_out + (uint64_t)k * nt, (size_t)nt * 8,
+                                 hipMemcpyDeviceToHost, c->stream));
+    return finish(c, false);
+}
+
+int nb_bh_build(nb_ctx *c) {
+    if (!c) return NB_E_INVALID_ARG;
+    timing_reset(c);
+    mark_begin(c, NB_PHASE_OCTREE);
+    build_tree(c);
+    mark_end(c);
+    int rc = finish(c, true);
+    c->tree_valid = rc == NB_OK;
+    return rc;
+}
+
+int nb_bh_forces(nb_ctx *c) {
+    if (!c) return NB_E_INVALID_ARG;
+    if (!c->tree_valid) return fail(c, NB_E_NOT_BUILT, "nb_bh_forces: build the octree first");
+    timing_reset(c);
+    int rc = bh_forces_enqueue(c);
+    if (rc) return rc;
+    return finish(c, true);
+}
+
+int nb_kick_drift(nb_ctx *c, double dt) {
+    if (!c) return NB_E_INVALID_ARG;
+    timing_reset(c);
+    mark_begin(c, NB_PHASE_LF1);
+    nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx, c->vhy,
+                          c->vhz, c->x, c->y, c->z, c->stream);
+    mark_end(c);
+    c->tree_valid = false;
+    return finish(c, false);
+}
+
+int nb_kick(nb_ctx *c, double dt) {
+    if (!c) return NB_E_INVALID_ARG;
+    timing_reset(c);
+    mark_begin(c, NB_PHASE_LF2);
+    nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy, c->vz,
+                    c->stream);
+    mark_end(c);
+    return finish(c, false);
+}
+
+int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
+    if (!c || (algorithm != NB_ALGO_NAIVE && algorithm != NB_ALGO_BARNES_HUT))
+        return fail(c, NB_E_INVALID_ARG, "nb_steps: bad algorithm");
+    timing_reset(c);
+    for (uint32_t s = 0; s < count; ++s) {
+        mark_begin(c, NB_PHASE_LF1);
+        nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx, c->vhy,
+                              c->vhz, c->x, c->y, c->z, c->stream);
+        mark_end(c);
+        int rc;
+        if (algorithm == NB_ALGO_BARNES_HUT) {
+            mark_begin(c, NB_PHASE_OCTREE);
+            build_tree(c);
+            mark_end(c);
+            rc = bh_forces_enqueue(c);
+        } else {
+            rc = direct_enqueue(c);
+        }
+        if (rc) return rc;
+        mark_begin(c, NB_PHASE_LF2);
+        nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy,
+                        c->vz, c->stream);
+        mark_end(c);
+    }
+    const bool bh = algorithm == NB_ALGO_BARNES_HUT && count > 0;
+    int rc = finish(c, bh);
+    c->tree_valid = false;  // positions moved after the last build
+    return rc;
+}
+
+int nb_phase_ms(const nb_ctx *c, double *ms) {
+    if (!c || !ms) return NB_E_INVALID_ARG;
+    for (int k = 0; k < NB_PHASE_COUNT; ++k) ms[k] = c->phase[k];
+    return NB_OK;
+}
+
+int nb_tree_info(nb_ctx *c, uint64_t *node_count, uint32_t *internal_count, uint32_t *max_depth,
+                 double aabb[7]) {
+    if (!c) return NB_E_INVALID_ARG;
+    NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                             c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    const TreeHdr &h = *c->hdr_host;
+    const uint32_t I = std::min(h.internal_count, c->tree.icap);
+    if (node_count) *node_count = 1 + 8ull * I;
+    if (internal_count) *internal_count = h.internal_count;
+    if (max_depth) *max_depth = h.max_depth;
+    if (aabb) {
+        const double v[7] = {h.minx, h.miny, h.minz, h.maxx, h.maxy, h.maxz, h.edge};
+        for (int k = 0; k < 7; ++k) aabb[k] = v[k];
+    }
+    return NB_OK;
+}
+
+int nb_tree_sorted(nb_ctx *c, uint32_t *sorted) {
+    if (!c || !sorted) return NB_E_INVALID_ARG;
+    NB_HIP(c, hipMemcpyAsync(sorted, c->tree.sidx, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_bh_interactions(nb_ctx *c, uint32_t *count) {
+    if (!c || !count) return NB_E_INVALID_ARG;
+    NB_HIP(c, hipMemcpyAsync(count, c->inter, (size_t)c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+uint64_t nb_tree_canonical(nb_ctx *c, uint64_t cap, uint8_t *depth, uint8_t *kind,
+                           uint32_t *body, double *mass, double *com_x, double *com_y,
+                           double *com_z, uint32_t *count) {
+    if (!c || !depth || !kind || !body || !mass || !com_x || !com_y || !com_z || !count) return 0;
+    if (hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return 0;
+    const uint32_t I = std::min(c->hdr_host->internal_count, c->tree.icap);
+    const uint64_t nodes = 1 + 8ull * I;
+    if (cap < nodes) return 0;
+    const uint32_t n = c->n;
+    std::vector<int32_t> ref(nodes);
+    std::vector<nb::InternalSums> sums(I);
+    std::vector<uint32_t> cnt(I);
+    std::vector<double> hx(n), hy(n), hz(n), hm(n);
+    bool ok = hipMemcpy(ref.data(), c->tree.ref, nodes * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (I) {
+        ok = ok && hipMemcpy(sums.data(), c->tree.int_sums, (size_t)I * sizeof(nb::InternalSums),
+                             hipMemcpyDeviceToHost) == hipSuccess;
+        ok = ok && hipMemcpy(cnt.data(), c->tree.int_count, (size_t)I * 4, hipMemcpyDeviceToHost) ==
+                       hipSuccess;
+    }
+    ok = ok && hipMemcpy(hx.data(), c->x, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(hy.data(), c->y, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(hz.data(), c->z, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(hm.data(), c->m, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return 0;
+    std::vector<std::pair<uint64_t, uint32_t>> stack{{0, 0}};
+    uint64_t k = 0;
+    while (!stack.empty()) {
+        const auto [node, d] = stack.back();
+        stack.pop_back();
+        if (k >= cap) return 0;
+        const int32_t r = ref[node];
+        depth[k] = (uint8_t)d;
+        if (r >= 0) {
+            kind[k] = 2;
+            body[k] = n;
+            mass[k] = sums[r].m;
+            com_x[k] = sums[r].x;
+            com_y[k] = sums[r].y;
+            com_z[k] = sums[r].z;
+            count[k] = cnt[r];
+            for (int dg = 7; dg >= 0; --dg) stack.push_back({1ull + 8ull * (uint64_t)r + dg, d + 1});
+        } else if (r != nb::kRefEmpty) {
+            const uint32_t b = nb::body_of_ref(r);
+            kind[k] = 1;
+            body[k] = b;
+            mass[k] = hm[b];
+            com_x[k] = hx[b] * hm[b];
+            com_y[k] = hy[b] * hm[b];
+            com_z[k] = hz[b] * hm[b];
+            count[k] = 1;
+        } else {
+            kind[k] = 0;
+            body[k] = n;
+            mass[k] = com_x[k] = com_y[k] = com_z[k] = 0.0;
+            count[k] = 0;
+        }
+        ++k;
+    }
+    return k;
+}
+
+}  // extern "C"

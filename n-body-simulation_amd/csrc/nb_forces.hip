@@ -1,0 +1,468 @@
+// nb_forces.hip -- direct-sum and Barnes-Hut force kernels for gfx950.
+//
+// FP64 kernels reproduce the reference's arithmetic operation for operation
+// (this file is compiled with -ffp-contract=off, and sycl::rsqrt is evaluated
+// as 1.0/sqrt(x), two correctly rounded IEEE operations):
+//   k_direct_f64   NaiveAlgorithm.cpp:299-352 (sum over j in index order)
+//   k_bh_exact     BarnesHutAlgorithm.cpp:319-393 (per-body stack traversal,
+//                  same push order, same opening test)
+// FP32 kernels do the pair arithmetic in FP32 (fmaf, v_rsq_f32) and fold the
+// FP32 partial of every tile into FP64 accumulators:
+//   k_direct_f32c  LDS-staged source tiles, two targets per lane, split j
+//   k_bh_group     one wavefront per 64 consecutive (octree-ordered) bodies;
+//                  a cell is opened if any body of the group needs it; lanes
+//                  test 8 sibling sets at a time and accepted cells/bodies are
+//                  batched in LDS and evaluated 64 at a time by every lane.
+#include <algorithm>
+#include <cmath>
+
+#include "nb_internal.h"
+
+namespace nb {
+namespace {
+
+__device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status, f); }
+
+// ---------------------------------------------------------------------------
+// FP64 direct sum, reference order.  One lane per body; sources staged in LDS
+// 64 at a time; the j loop is sequential per body as in the reference.
+constexpr int kDirectF64Tile = 64;
+
+__device__ inline void direct_f64_body(uint32_t n, const double *__restrict__ x,
+                                       const double *__restrict__ y,
+                                       const double *__restrict__ z,
+                                       const double *__restrict__ m, double eps2, bool valid,
+                                       double px, double py, double pz, double *acc) {
+    __shared__ double sx[kDirectF64Tile], sy[kDirectF64Tile], sz[kDirectF64Tile],
+        sm[kDirectF64Tile];
+    double acc_x = 0, acc_y = 0, acc_z = 0;
+    for (uint32_t k = 0; k < n; k += kDirectF64Tile) {
+        const uint32_t j = k + threadIdx.x;
+        if (j < n) {
+            sx[threadIdx.x] = x[j];
+            sy[threadIdx.x] = y[j];
+            sz[threadIdx.x] = z[j];
+            sm[threadIdx.x] = m[j];
+        }
+        __syncthreads();
+        const uint32_t cnt = min((uint32_t)kDirectF64Tile, n - k);
+        if (valid) {
+            for (uint32_t jj = 0; jj < cnt; ++jj) {
+                const double r_x = sx[jj] - px;
+                const double r_y = sy[jj] - py;
+                const double r_z = sz[jj] - pz;
+                double denominator = (r_x * r_x) + (r_y * r_y) + (r_z * r_z) + eps2;
+                denominator = denominator * denominator * denominator;
+                denominator = 1.0 / sqrt(denominator);
+                acc_x += sm[jj] * (r_x * denominator);
+                acc_y += sm[jj] * (r_y * denominator);
+                acc_z += sm[jj] * (r_z * denominator);
+            }
+        }
+        __syncthreads();
+    }
+    acc[0] = acc_x;
+    acc[1] = acc_y;
+    acc[2] = acc_z;
+}
+
+__global__ void __launch_bounds__(kDirectF64Tile) k_direct_f64(
+    uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
+    double *ax, double *ay, double *az) {
+    const uint32_t i = blockIdx.x * kDirectF64Tile + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t ii = valid ? i : 0;
+    double acc[3];
+    direct_f64_body(n, x, y, z, m, eps2, valid, x[ii], y[ii], z[ii], acc);
+    if (valid) {
+        ax[i] = acc[0] * G;
+        ay[i] = acc[1] * G;
+        az[i] = acc[2] * G;
+    }
+}
+
+__global__ void __launch_bounds__(kDirectF64Tile) k_direct_targets_f64(
+    uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
+    uint32_t nt, const uint32_t *__restrict__ targets, double *ax, double *ay, double *az) {
+    const uint32_t t = blockIdx.x * kDirectF64Tile + threadIdx.x;
+    const bool valid = t < nt;
+    const uint32_t i = valid ? targets[t] : 0;
+    double acc[3];
+    direct_f64_body(n, x, y, z, m, eps2, valid, x[i], y[i], z[i], acc);
+    if (valid) {
+        ax[t] = acc[0] * G;
+        ay[t] = acc[1] * G;
+        az[t] = acc[2] * G;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 direct sum with FP64 tile accumulation.
+constexpr int kF32Threads = 256;
+constexpr int kF32Tile = 256;  // sources per LDS tile
+constexpr int kF32PerLane = 2; // targets per lane (each LDS read feeds 2 pairs)
+
+__global__ void __launch_bounds__(kF32Threads) k_pack_f32(uint32_t n,
+                                                          const double *__restrict__ x,
+                                                          const double *__restrict__ y,
+                                                          const double *__restrict__ z,
+                                                          const double *__restrict__ m, double G,
+                                                          float4 *pos) {
+    const uint32_t i = blockIdx.x * kF32Threads + threadIdx.x;
+    if (i < n) pos[i] = make_float4((float)x[i], (float)y[i], (float)z[i], (float)(G * m[i]));
+}
+
+__device__ inline void pair_f32(const float4 s, float px, float py, float pz, float eps2,
+                                float &fx, float &fy, float &fz) {
+    const float dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float f = (s.w * inv) * (inv * inv);
+    fx = fmaf(f, dx, fx);
+    fy = fmaf(f, dy, fy);
+    fz = fmaf(f, dz, fz);
+}
+
+// grid.x: blocks of kF32Threads*kF32PerLane targets; grid.y: source splits.
+__global__ void __launch_bounds__(kF32Threads) k_direct_f32c(uint32_t n,
+                                                             const float4 *__restrict__ pos,
+                                                             float eps2, uint32_t chunk,
+                                                             double *__restrict__ partial) {
+    __shared__ float4 tile[kF32Tile];
+    float px[kF32PerLane], py[kF32PerLane], pz[kF32PerLane];
+    double ax[kF32PerLane], ay[kF32PerLane], az[kF32PerLane];
+    uint32_t ti[kF32PerLane];
+#pragma unroll
+    for (int q = 0; q < kF32PerLane; ++q) {
+        ti[q] = blockIdx.x * (kF32Threads * kF32PerLane) + q * kF32Threads + threadIdx.x;
+        const float4 p = pos[ti[q] < n ? ti[q] : n - 1];
+        px[q] = p.x;
+        py[q] = p.y;
+        pz[q] = p.z;
+        ax[q] = ay[q] = az[q] = 0.0;
+    }
+    const uint32_t j0 = blockIdx.y * chunk;
+    const uint32_t j1 = min(n, j0 + chunk);
+    for (uint32_t k = j0; k < j1; k += kF32Tile) {
+        const uint32_t j = k + threadIdx.x;
+        tile[threadIdx.x] = j < j1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        __syncthreads();
+        float fx[kF32PerLane], fy[kF32PerLane], fz[kF32PerLane];
+#pragma unroll
+        for (int q = 0; q < kF32PerLane; ++q) fx[q] = fy[q] = fz[q] = 0.f;
+#pragma unroll 8
+        for (int jj = 0; jj < kF32Tile; ++jj) {
+            const float4 s = tile[jj];
+#pragma unroll
+            for (int q = 0; q < kF32PerLane; ++q) pair_f32(s, px[q], py[q], pz[q], eps2, fx[q], fy[q], fz[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < kF32PerLane; ++q) {
+            ax[q] += (double)fx[q];
+            ay[q] += (double)fy[q];
+            az[q] += (double)fz[q];
+        }
+        __syncthreads();
+    }
+    const uint64_t base = (uint64_t)blockIdx.y * 3ull * n;
+#pragma unroll
+    for (int q = 0; q < kF32PerLane; ++q) {
+        if (ti[q] < n) {
+            partial[base + ti[q]] = ax[q];
+            partial[base + n + ti[q]] = ay[q];
+            partial[base + 2ull * n + ti[q]] = az[q];
+        }
+    }
+}
+
+// Deterministic reduction of the split partials in split order.
+__global__ void __launch_bounds__(kF32Threads) k_direct_f32c_reduce(
+    uint32_t n, uint32_t splits, const double *__restrict__ partial, double *ax, double *ay,
+    double *az) {
+    const uint32_t i = blockIdx.x * kF32Threads + threadIdx.x;
+    if (i >= n) return;
+    double sx = 0, sy = 0, sz = 0;
+    for (uint32_t s = 0; s < splits; ++s) {
+        const uint64_t base = (uint64_t)s * 3ull * n;
+        sx += partial[base + i];
+        sy += partial[base + n + i];
+        sz += partial[base + 2ull * n + i];
+    }
+    ax[i] = sx;
+    ay[i] = sy;
+    az[i] = sz;
+}
+
+// ---------------------------------------------------------------------------
+// Reference-semantics Barnes-Hut traversal (FP64).  Per body a depth-first
+// walk with its own stack (here in global memory, interleaved across bodies
+// so that lanes at the same stack depth touch adjacent words); children are
+// pushed in digit order 5,7,4,6,1,3,0,2 as in BarnesHutAlgorithm.cpp:370-385.
+constexpr int kExactThreads = 256;
+
+__global__ void __launch_bounds__(kExactThreads) k_bh_exact(
+    uint32_t n, const uint32_t *__restrict__ order, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double4 *__restrict__ cm,
+    const double *__restrict__ edge, const int32_t *__restrict__ ref, double G, double eps2,
+    double theta, uint32_t *stack, uint32_t cap, uint32_t icap, double *ax, double *ay,
+    double *az, uint32_t *inter, TreeHdr *h) {
+    const uint32_t id = blockIdx.x * kExactThreads + threadIdx.x;
+    if (id >= n) return;
+    const uint32_t i = order ? order[id] : id;
+    const int32_t self_ref = ref_of_body(i);
+    const double pos_x_i = x[i], pos_y_i = y[i], pos_z_i = z[i];
+    uint32_t *stk = stack + id;
+    const uint64_t stride = n;
+    double acc_x = 0, acc_y = 0, acc_z = 0;
+    uint32_t cnt = 0;
+    uint32_t top = 1;
+    stk[0] = 0;  // root
+    // every node is visited at most once per body in a correct octree
+    uint64_t budget = 1ull + 8ull * (uint64_t)icap;
+    while (top > 0) {
+        if (budget-- == 0) {
+            raise_flag(h, kFlagStack);
+            break;
+        }
+        --top;
+        const uint32_t node = stk[(uint64_t)top * stride];
+        const double4 c = cm[node];
+        const int32_t r = ref[node];
+        if (c.w != 0 && r != self_ref) {
+            const double d_x = c.x - pos_x_i;
+            const double d_y = c.y - pos_y_i;
+            const double d_z = c.z - pos_z_i;
+            const double d = 1.0 / sqrt(d_x * d_x + d_y * d_y + d_z * d_z);
+            const double currentTheta = edge[node] * d;
+            if ((currentTheta < theta) || r < 0) {
+                double denominator = (d_x * d_x) + (d_y * d_y) + (d_z * d_z) + eps2;
+                denominator = denominator * denominator * denominator;
+                denominator = 1.0 / sqrt(denominator);
+                acc_x += c.w * (d_x * denominator);
+                acc_y += c.w * (d_y * denominator);
+                acc_z += c.w * (d_z * denominator);
+                ++cnt;
+            } else {
+                if (top + 8 > cap) {
+                    raise_flag(h, kFlagStack);
+                    break;
+                }
+                const uint32_t base = 1u + 8u * (uint32_t)r;
+                stk[(uint64_t)(top++) * stride] = base + 5;
+                stk[(uint64_t)(top++) * stride] = base + 7;
+                stk[(uint64_t)(top++) * stride] = base + 4;
+                stk[(uint64_t)(top++) * stride] = base + 6;
+                stk[(uint64_t)(top++) * stride] = base + 1;
+                stk[(uint64_t)(top++) * stride] = base + 3;
+                stk[(uint64_t)(top++) * stride] = base + 0;
+                stk[(uint64_t)(top++) * stride] = base + 2;
+            }
+        }
+    }
+    ax[i] = acc_x * G;
+    ay[i] = acc_y * G;
+    az[i] = acc_z * G;
+    if (inter) inter[i] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Group traversal (FP32 pair maths, FP64 accumulation).
+constexpr int kGroup = 64;          // bodies per wavefront
+constexpr int kStackCap = 2048;     // internal cells waiting to be opened
+constexpr int kListCap = 2 * kGroup;
+
+__device__ inline uint64_t lanes_below() {
+    const uint32_t lane = threadIdx.x & 63u;
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+__global__ void __launch_bounds__(kGroup) k_bh_group(
+    uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double4 *__restrict__ cm,
+    const float *__restrict__ ropen2, const int32_t *__restrict__ ref, double G, float eps2,
+    uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter, TreeHdr *h) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ float4 lst[kListCap];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t gid = blockIdx.x * kGroup + lane;
+    const bool valid = gid < n;
+    const uint32_t bi = sidx[valid ? gid : n - 1];
+    const double px = x[bi], py = y[bi], pz = z[bi];
+
+    // group bounding box (FP64, exact min/max), centre and half extent
+    double mnx = px, mny = py, mnz = pz, mxx = px, mxy = py, mxz = pz;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
+        mny = fmin(mny, __shfl_xor(mny, off, 64));
+        mnz = fmin(mnz, __shfl_xor(mnz, off, 64));
+        mxx = fmax(mxx, __shfl_xor(mxx, off, 64));
+        mxy = fmax(mxy, __shfl_xor(mxy, off, 64));
+        mxz = fmax(mxz, __shfl_xor(mxz, off, 64));
+    }
+    const double cx = 0.5 * (mnx + mxx), cy = 0.5 * (mny + mxy), cz = 0.5 * (mnz + mxz);
+    // half extents rounded up so that the FP32 box always contains the bodies
+    const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f;
+    const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f;
+    const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f;
+    const float rx = (float)(px - cx), ry = (float)(py - cy), rz = (float)(pz - cz);
+
+    double acc_x = 0, acc_y = 0, acc_z = 0;
+    uint32_t n_inter = 0;
+    uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
+    const uint64_t below = lanes_below();
+
+    auto flush = [&](uint32_t cnt) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (uint32_t e = 0; e < cnt; ++e) pair_f32(lst[e], rx, ry, rz, eps2, fx, fy, fz);
+        acc_x += (double)fx;
+        acc_y += (double)fy;
+        acc_z += (double)fz;
+        n_inter += cnt;
+    };
+
+    // classify one node for the whole group: 0 skip, 1 list, 2 open
+    auto classify = [&](uint32_t node, bool act, float4 &entry, int32_t &r) -> int {
+        r = act ? ref[node] : kRefEmpty;
+        if (r == kRefEmpty) return 0;
+        const double4 c = cm[node];
+        const float dx = (float)(c.x - cx), dy = (float)(c.y - cy), dz = (float)(c.z - cz);
+        entry = make_float4(dx, dy, dz, (float)(G * c.w));
+        if (r < 0) return 1;  // body leaf: exact pair (self pair contributes 0)
+        const float ex = fmaxf(fabsf(dx) - hx, 0.f);
+        const float ey = fmaxf(fabsf(dy) - hy, 0.f);
+        const float ez = fmaxf(fabsf(dz) - hz, 0.f);
+        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+        return d2 > ropen2[node] ? 1 : 2;
+    };
+
+    {
+        float4 e;
+        int32_t r;
+        const int cls = classify(0, true, e, r);  // every lane evaluates the root alike
+        if (cls == 1) {
+            if (lane == 0) lst[0] = e;
+            ln = 1;
+        } else if (cls == 2) {
+            if (lane == 0) stk[0] = (uint32_t)r;
+            sp = 1;
+        }
+        __syncthreads();
+    }
+    bool overflow = false;
+    // A correct octree opens every internal cell at most once per group, so
+    // icap iterations bound the walk; the cap turns a corrupt tree into an
+    // error instead of an endless loop.
+    uint32_t budget = icap + 16;
+    while (sp > 0) {
+        if (budget-- == 0) {
+            overflow = true;
+            break;
+        }
+        const uint32_t take = sp < 8u ? sp : 8u;
+        sp -= take;
+        const uint32_t slot = lane >> 3;
+        const bool act = slot < take;
+        const uint32_t k = act ? stk[sp + slot] : 0u;
+        __syncthreads();
+        const uint32_t node = 1u + 8u * k + (lane & 7u);
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+        int32_t r;
+        const int cls = classify(node, act, e, r);
+        const uint64_t bl = __ballot(cls == 1);
+        const uint64_t bo = __ballot(cls == 2);
+        if (cls == 1) lst[ln + __popcll(bl & below)] = e;
+        if (cls == 2) {
+            const uint32_t p = sp + __popcll(bo & below);
+            if (p < kStackCap) stk[p] = (uint32_t)r;
+        }
+        ln += __popcll(bl);
+        sp += __popcll(bo);
+        if (sp > kStackCap) {
+            overflow = true;
+            break;
+        }
+        __syncthreads();
+        if (ln >= kGroup) {
+            flush(kGroup);
+            __syncthreads();
+            const uint32_t rest = ln - kGroup;
+            if (lane < rest) lst[lane] = lst[kGroup + lane];
+            ln = rest;
+            __syncthreads();
+        }
+    }
+    if (overflow) raise_flag(h, kFlagStack);
+    if (ln > 0) flush(ln);
+    if (valid) {
+        ax[bi] = acc_x;
+        ay[bi] = acc_y;
+        az[bi] = acc_z;
+        if (inter) inter[bi] = n_inter;
+    }
+}
+
+}  // namespace
+
+void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
+                       const double *m, double G, double eps2, double *ax, double *ay,
+                       double *az, hipStream_t s) {
+    hipLaunchKernelGGL(k_direct_f64, dim3(ceil_div(n, kDirectF64Tile)), dim3(kDirectF64Tile), 0,
+                       s, n, x, y, z, m, G, eps2, ax, ay, az);
+}
+
+void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, const double *z,
+                               const double *m, double G, double eps2, uint32_t nt,
+                               const uint32_t *targets, double *ax, double *ay, double *az,
+                               hipStream_t s) {
+    if (nt == 0) return;
+    hipLaunchKernelGGL(k_direct_targets_f64, dim3(ceil_div(nt, kDirectF64Tile)),
+                       dim3(kDirectF64Tile), 0, s, n, x, y, z, m, G, eps2, nt, targets, ax, ay,
+                       az);
+}
+
+uint32_t direct_f32c_splits(uint32_t n) {
+    const uint32_t blocks = ceil_div(n, kF32Threads * kF32PerLane);
+    uint32_t splits = ceil_div(2048, blocks);
+    const uint32_t max_splits = std::max<uint32_t>(1, n / kF32Tile);
+    return std::max<uint32_t>(1, std::min(splits, max_splits));
+}
+
+void launch_direct_f32c(uint32_t n, const double *x, const double *y, const double *z,
+                        const double *m, double G, double eps2, double *ax, double *ay,
+                        double *az, double *partial, uint32_t splits, hipStream_t s) {
+    // partial holds 3*n*splits doubles, then (16-byte aligned) n float4 positions
+    float4 *pos = reinterpret_cast<float4 *>(partial + ((3ull * n * splits + 1) & ~1ull));
+    hipLaunchKernelGGL(k_pack_f32, dim3(ceil_div(n, kF32Threads)), dim3(kF32Threads), 0, s, n, x,
+                       y, z, m, G, pos);
+    const uint32_t chunk = ceil_div(ceil_div(n, splits), kF32Tile) * kF32Tile;
+    const dim3 grid(ceil_div(n, kF32Threads * kF32PerLane), splits);
+    hipLaunchKernelGGL(k_direct_f32c, grid, dim3(kF32Threads), 0, s, n, (const float4 *)pos,
+                       (float)eps2, chunk, partial);
+    hipLaunchKernelGGL(k_direct_f32c_reduce, dim3(ceil_div(n, kF32Threads)), dim3(kF32Threads), 0,
+                       s, n, splits, (const double *)partial, ax, ay, az);
+}
+
+void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const double *z,
+                     double G, double eps2, double theta, int sort_bodies, uint32_t *stack,
+                     uint32_t stack_cap, double *ax, double *ay, double *az,
+                     uint32_t *interactions, hipStream_t s) {
+    hipLaunchKernelGGL(k_bh_exact, dim3(ceil_div(t.n, kExactThreads)), dim3(kExactThreads), 0, s,
+                       t.n, sort_bodies ? (const uint32_t *)t.sidx : nullptr, x, y, z,
+                       (const double4 *)t.cm, (const double *)t.edge, (const int32_t *)t.ref, G,
+                       eps2, theta, stack, stack_cap, t.icap, ax, ay, az, interactions, t.hdr);
+}
+
+void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
+                     double G, double eps2, double theta, double *ax, double *ay, double *az,
+                     uint32_t *interactions, hipStream_t s) {
+    (void)theta;  // folded into the node records' opening radii
+    hipLaunchKernelGGL(k_bh_group, dim3(ceil_div(t.n, kGroup)), dim3(kGroup), 0, s, t.n,
+                       (const uint32_t *)t.sidx, x, y, z, (const double4 *)t.cm,
+                       (const float *)t.ropen2, (const int32_t *)t.ref, G, (float)eps2, t.icap,
+                       ax, ay, az, interactions, t.hdr);
+}
+
+}  // namespace nb

@@ -1,0 +1,63 @@
+// nb_integrate.hip -- FP64 kick-drift-kick leapfrog.
+//
+// Same operations as NaiveAlgorithm.cpp:154-163 and :217-221 (identical in
+// BarnesHutAlgorithm.cpp:171-181, :234-238); compiled with -ffp-contract=off
+// so `v + a*(dt/2)` stays a rounded multiply followed by a rounded add.
+// HBM-bound: 120 bytes of algorithmic traffic per body-step (read x, v, a;
+// write x, v) across the two kernels.
+#include "nb_internal.h"
+
+namespace nb {
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void __launch_bounds__(kThreads) k_kick_drift(
+    uint32_t n, double delta_t, const double *__restrict__ ax, const double *__restrict__ ay,
+    const double *__restrict__ az, const double *__restrict__ vx, const double *__restrict__ vy,
+    const double *__restrict__ vz, double *__restrict__ vhx, double *__restrict__ vhy,
+    double *__restrict__ vhz, double *__restrict__ x, double *__restrict__ y,
+    double *__restrict__ z) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const double hx = vx[i] + ax[i] * (delta_t / 2.0);
+    const double hy = vy[i] + ay[i] * (delta_t / 2.0);
+    const double hz = vz[i] + az[i] * (delta_t / 2.0);
+    vhx[i] = hx;
+    vhy[i] = hy;
+    vhz[i] = hz;
+    x[i] = x[i] + hx * delta_t;
+    y[i] = y[i] + hy * delta_t;
+    z[i] = z[i] + hz * delta_t;
+}
+
+__global__ void __launch_bounds__(kThreads) k_kick(
+    uint32_t n, double delta_t, const double *__restrict__ ax, const double *__restrict__ ay,
+    const double *__restrict__ az, const double *__restrict__ vhx,
+    const double *__restrict__ vhy, const double *__restrict__ vhz, double *__restrict__ vx,
+    double *__restrict__ vy, double *__restrict__ vz) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vx[i] = vhx[i] + ax[i] * (delta_t / 2.0);
+    vy[i] = vhy[i] + ay[i] * (delta_t / 2.0);
+    vz[i] = vhz[i] + az[i] * (delta_t / 2.0);
+}
+
+}  // namespace
+
+void launch_kick_drift(uint32_t n, double dt, const double *ax, const double *ay,
+                       const double *az, const double *vx, const double *vy, const double *vz,
+                       double *vhx, double *vhy, double *vhz, double *x, double *y, double *z,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_kick_drift, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt, ax,
+                       ay, az, vx, vy, vz, vhx, vhy, vhz, x, y, z);
+}
+
+void launch_kick(uint32_t n, double dt, const double *ax, const double *ay, const double *az,
+                 const double *vhx, const double *vhy, const double *vhz, double *vx,
+                 double *vy, double *vz, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt, ax, ay,
+                       az, vhx, vhy, vhz, vx, vy, vz);
+}
+
+}  // namespace nb

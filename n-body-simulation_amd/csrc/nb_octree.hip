@@ -1,0 +1,534 @@
+// nb_octree.hip -- Barnes-Hut octree build for gfx950.
+//
+// The reference builds its octree by lock-based top-down insertion
+// (ParallelOctreeTopDownSubtrees.cpp:95-813): one work-group inserts every
+// body down to level 7, then one work-group per subtree finishes the job.
+// The resulting tree is the unique octree in which a cell is split iff it
+// holds two or more bodies; only the node ids depend on the insertion race.
+// This file builds that same tree without locks:
+//
+//  1. AABB (FP64 min/max including the origin) and cubify, on the device.
+//  2. Per body, the first 21 octant digits with the reference's own FP64
+//     classification chain: mid = cellMin + edge/2, digit = 4[y>mid_y] +
+//     2[x>mid_x] + [z<mid_z], child min moved to mid on the x/y bits and on a
+//     cleared z bit.  Digits are packed most significant first into a 63-bit
+//     key, so key order is the reference's in-order body order.
+//  3. Stable radix sort of (key, body).
+//  4. Runs of bodies sharing all 21 digits get digits 21..41 and are sorted
+//     inside the run (rare: bodies closer than edge/2^21).
+//  5. lcp[i] = depth of the deepest cell holding sorted bodies i and i+1.
+//     The internal cells are exactly the pairs (i, depth l) with
+//     lcp[i-1] < l <= lcp[i]: each cell is numbered by the first adjacent pair
+//     of its body range.  An exclusive scan of the chain lengths numbers the
+//     internal cells k in depth-first pre-order; internal cell k stores its 8
+//     children at node ids 1 + 8k + digit.
+//  6. Parents come from a galloping search for the first body of the parent
+//     cell; children count themselves into their parent.
+//  7. Centre of mass bottom-up: the last child to arrive at a cell sums the 8
+//     children in digit order 0..7 from 0.0 -- the reference's summation
+//     (BarnesHutOctree.cpp:299-311), so every sum is bit-identical.
+//  8. Node records for traversal (COM = numerator / mass as the reference
+//     divides per visit, edge = rootEdge * 2^-depth, opening radius^2).
+#include <algorithm>
+#include <cmath>
+
+#include "nb_internal.h"
+
+namespace nb {
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ inline double sycl_min(double a, double b) { return b < a ? b : a; }
+__device__ inline double sycl_max(double a, double b) { return a < b ? b : a; }
+
+__device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status, f); }
+
+// ---- 1. AABB ---------------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_build_begin(TreeHdr *h) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        h->internal_count = 0;
+        h->max_depth = 0;
+        h->status = 0;
+        h->deep_runs = 0;
+    }
+}
+
+// Block partial extrema.  Like the reference's partials (BarnesHutOctree.cpp:
+// 58-72, value-initialised to 0.0) every partial starts at the origin.
+__global__ void __launch_bounds__(kThreads) k_aabb_partial(const double *__restrict__ x,
+                                                           const double *__restrict__ y,
+                                                           const double *__restrict__ z,
+                                                           uint32_t n, double *part,
+                                                           TreeHdr *h) {
+    double mn[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
+    bool bad = false;
+    for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kThreads) {
+        const double p[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            bad |= !isfinite(p[c]);
+            mn[c] = sycl_min(mn[c], p[c]);
+            mx[c] = sycl_max(mx[c], p[c]);
+        }
+    }
+    if (bad) raise_flag(h, kFlagAabb);
+    // min/max are exact, so any reduction order gives the reference's result
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = sycl_min(mn[c], __shfl_xor(mn[c], off, 64));
+            mx[c] = sycl_max(mx[c], __shfl_xor(mx[c], off, 64));
+        }
+    }
+    __shared__ double sh[kThreads / 64][6];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            sh[w][c] = mn[c];
+            sh[w][3 + c] = mx[c];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double v = sh[0][threadIdx.x];
+        for (int k = 1; k < kThreads / 64; ++k)
+            v = threadIdx.x < 3 ? sycl_min(v, sh[k][threadIdx.x]) : sycl_max(v, sh[k][threadIdx.x]);
+        part[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+// Final reduction + cubify (BarnesHutOctree.cpp:161-190) on one thread per
+// component, then one thread writes the cube.
+__global__ void __launch_bounds__(64) k_aabb_final(const double *part, uint32_t np, TreeHdr *h) {
+    __shared__ double ext[6];
+    if (threadIdx.x < 6) {
+        const bool is_min = threadIdx.x < 3;
+        double v = part[(uint64_t)threadIdx.x * np];
+        for (uint32_t k = 1; k < np; ++k) {
+            const double p = part[(uint64_t)threadIdx.x * np + k];
+            v = is_min ? (p < v ? p : v) : (p > v ? p : v);
+        }
+        ext[threadIdx.x] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double min_x = ext[0], min_y = ext[1], min_z = ext[2];
+    double max_x = ext[3], max_y = ext[4], max_z = ext[5];
+    const double x_length = fabs(max_x - min_x);
+    const double y_length = fabs(max_y - min_y);
+    const double z_length = fabs(max_z - min_z);
+    // std::max(a, b) == (a < b) ? b : a
+    const double yz = y_length < z_length ? z_length : y_length;
+    const double maxEdgeLength = x_length < yz ? yz : x_length;
+    if (maxEdgeLength == x_length) {
+        min_z = min_z - ((maxEdgeLength - z_length) / 2);
+        min_y = min_y - ((maxEdgeLength - y_length) / 2);
+        max_z = max_z + ((maxEdgeLength - z_length) / 2);
+        max_y = max_y + ((maxEdgeLength - y_length) / 2);
+    } else if (maxEdgeLength == y_length) {
+        min_x = min_x - ((maxEdgeLength - x_length) / 2);
+        min_z = min_z - ((maxEdgeLength - z_length) / 2);
+        max_x = max_x + ((maxEdgeLength - x_length) / 2);
+        max_z = max_z + ((maxEdgeLength - z_length) / 2);
+    } else if (maxEdgeLength == z_length) {
+        min_x = min_x - ((maxEdgeLength - x_length) / 2);
+        min_y = min_y - ((maxEdgeLength - y_length) / 2);
+        max_x = max_x + ((maxEdgeLength - x_length) / 2);
+        max_y = max_y + ((maxEdgeLength - y_length) / 2);
+    } else {
+        raise_flag(h, kFlagAabb);
+    }
+    h->minx = min_x;
+    h->miny = min_y;
+    h->minz = min_z;
+    h->maxx = max_x;
+    h->maxy = max_y;
+    h->maxz = max_z;
+    h->edge = maxEdgeLength;
+}
+
+// ---- 2. keys ---------------------------------------------------------------
+struct Cell {
+    double mx, my, mz, e;
+};
+
+// One level of the reference's classification chain; returns the digit and
+// moves the cell to the child (Subtrees.cpp:256-315, :348-358).
+__device__ inline uint32_t descend(Cell &c, double px, double py, double pz) {
+    const double h = c.e / 2.0;
+    const double midx = c.mx + h, midy = c.my + h, midz = c.mz + h;
+    const bool up = py > midy;
+    const bool right = px > midx;
+    const bool back = pz < midz;
+    if (up) c.my = midy;
+    if (right) c.mx = midx;
+    if (!back) c.mz = midz;
+    c.e = h;
+    return (uint32_t)up * 4u + (uint32_t)right * 2u + (uint32_t)back;
+}
+
+__device__ inline uint64_t key_from(Cell &c, double px, double py, double pz) {
+    uint64_t key = 0;
+#pragma unroll
+    for (int l = 0; l < kKeyLevels; ++l) key = (key << 3) | descend(c, px, py, pz);
+    return key;
+}
+
+__global__ void __launch_bounds__(kThreads) k_keys(const double *__restrict__ x,
+                                                   const double *__restrict__ y,
+                                                   const double *__restrict__ z, uint32_t n,
+                                                   const TreeHdr *__restrict__ h,
+                                                   uint64_t *skey, uint32_t *sidx) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    Cell c{h->minx, h->miny, h->minz, h->edge};
+    skey[i] = key_from(c, x[i], y[i], z[i]);
+    sidx[i] = i;
+}
+
+// ---- 4./5. LCP and deep runs --------------------------------------------------
+__device__ inline int lcp_depth(uint64_t a, uint64_t b) {
+    // number of equal leading digits of two 63-bit keys (bit 63 is always 0)
+    return (__clzll((long long)(a ^ b)) - 1) / 3;
+}
+
+__global__ void __launch_bounds__(kThreads) k_lcp(const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ z, uint32_t n,
+                                                  TreeHdr *h, uint64_t *skey, uint32_t *sidx,
+                                                  uint64_t *klo, int32_t *lcp) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i + 1 >= n) return;
+    const uint64_t k0 = skey[i], k1 = skey[i + 1];
+    if (k0 != k1) {
+        lcp[i] = lcp_depth(k0, k1);
+        return;
+    }
+    if (i > 0 && skey[i - 1] == k0) return;  // not the first pair of the run
+    // run [a, b] of bodies sharing all 21 digits: extend their keys by 21
+    // more digits from the common level-21 cell and sort the run by them.
+    const uint32_t a = i;
+    uint32_t b = i + 1;
+    while (b + 1 < n && skey[b + 1] == k0) ++b;
+    atomicAdd(&h->deep_runs, 1u);
+    Cell c0{h->minx, h->miny, h->minz, h->edge};
+    {
+        const uint32_t q = sidx[a];
+        key_from(c0, x[q], y[q], z[q]);  // walk to the shared level-21 cell
+    }
+    for (uint32_t p = a; p <= b; ++p) {
+        Cell c = c0;
+        const uint32_t q = sidx[p];
+        klo[p] = key_from(c, x[q], y[q], z[q]);
+    }
+    // insertion sort by the second key word; stable, so equal keys keep body order
+    for (uint32_t p = a + 1; p <= b; ++p) {
+        const uint64_t kv = klo[p];
+        const uint32_t iv = sidx[p];
+        uint32_t q = p;
+        while (q > a && klo[q - 1] > kv) {
+            klo[q] = klo[q - 1];
+            sidx[q] = sidx[q - 1];
+            --q;
+        }
+        klo[q] = kv;
+        sidx[q] = iv;
+    }
+    for (uint32_t p = a; p < b; ++p) {
+        const uint64_t d = klo[p] ^ klo[p + 1];
+        if (d == 0) {
+            raise_flag(h, kFlagDepth);  // not separated within kMaxDepth digits
+            lcp[p] = kMaxDepth - 1;
+        } else {
+            lcp[p] = kKeyLevels + lcp_depth(klo[p], klo[p + 1]);
+        }
+    }
+}
+
+// chain[i] = number of internal cells owned by pair i = max(0, lcp[i] - lcp[i-1])
+__global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ lcp, uint32_t n,
+                                                    uint32_t *chain) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i + 1 >= n) return;
+    const int cp = i > 0 ? lcp[i - 1] : -1;
+    const int cn = lcp[i];
+    chain[i] = cn > cp ? (uint32_t)(cn - cp) : 0u;
+}
+
+// ---- 6. topology --------------------------------------------------------------
+struct KeyView {
+    const uint64_t *skey;
+    const uint64_t *klo;
+    __device__ inline uint32_t digit(uint32_t p, int l) const {
+        return l < kKeyLevels ? (uint32_t)(skey[p] >> (3 * (kKeyLevels - 1 - l))) & 7u
+                              : (uint32_t)(klo[p] >> (3 * (2 * kKeyLevels - 1 - l))) & 7u;
+    }
+    // do sorted bodies p and q share their first v digits?
+    __device__ inline bool same(uint32_t p, uint32_t q, int v) const {
+        if (v <= 0) return true;
+        if (v <= kKeyLevels) return ((skey[p] ^ skey[q]) >> (63 - 3 * v)) == 0;
+        return skey[p] == skey[q] && ((klo[p] ^ klo[q]) >> (63 - 3 * (v - kKeyLevels))) == 0;
+    }
+    // first sorted position of the depth-v cell that holds sorted body i
+    __device__ inline uint32_t cell_start(uint32_t i, int v) const {
+        if (v <= 0 || i == 0) return 0;
+        int64_t in = i;       // shares the prefix
+        int64_t out = -1;     // does not (or -1)
+        uint32_t step = 1;
+        while (true) {
+            const int64_t q = (int64_t)i - step;
+            if (q < 0) break;
+            if (!same((uint32_t)q, i, v)) {
+                out = q;
+                break;
+            }
+            in = q;
+            step <<= 1;
+        }
+        while (in - out > 1) {
+            const int64_t mid = out + (in - out) / 2;
+            if (same((uint32_t)mid, i, v)) in = mid; else out = mid;
+        }
+        return (uint32_t)in;
+    }
+};
+
+__global__ void __launch_bounds__(kThreads) k_fill(TreeHdr *h, uint32_t icap, int32_t *ref,
+                                                   uint32_t *nchild, uint32_t *arrive) {
+    const uint32_t I = min(h->internal_count, icap);
+    const uint64_t nodes = 1 + 8ull * I;
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride)
+        ref[j] = kRefEmpty;
+    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < I; j += stride) {
+        nchild[j] = 0;
+        arrive[j] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, TreeHdr *h,
+                                                   KeyView kv, const uint32_t *__restrict__ sidx,
+                                                   const int32_t *__restrict__ lcp,
+                                                   const uint32_t *__restrict__ off,
+                                                   int32_t *ref, uint8_t *int_depth,
+                                                   uint32_t *int_parent, uint32_t *int_node,
+                                                   uint32_t *nchild, uint32_t *body_parent) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (n == 1) {  // the root is a leaf holding the only body
+        ref[0] = ref_of_body(sidx[0]);
+        body_parent[0] = 0xffffffffu;
+        atomicMax(&h->max_depth, 0u);
+        return;
+    }
+    const int cp = i > 0 ? lcp[i - 1] : -1;
+    const int cn = i + 1 < n ? lcp[i] : -1;
+    auto owner_index = [&](uint32_t a, int v) -> uint32_t {
+        const int ca = a > 0 ? lcp[a - 1] : -1;
+        return off[a] + (uint32_t)(v - ca - 1);
+    };
+    // internal cells owned by pair i: depths cp+1 .. cn
+    for (int l = cp + 1; l <= cn; ++l) {
+        const uint32_t k = off[i] + (uint32_t)(l - cp - 1);
+        if (k >= icap) {
+            raise_flag(h, kFlagNodeOverflow);
+            break;
+        }
+        int_depth[k] = (uint8_t)l;
+        if (l == 0) {
+            int_node[k] = 0;
+            int_parent[k] = 0xffffffffu;
+            ref[0] = (int32_t)k;
+            continue;
+        }
+        const uint32_t pk = l > cp + 1 ? k - 1 : owner_index(kv.cell_start(i, cp), cp);
+        if (pk >= icap) {
+            raise_flag(h, kFlagNodeOverflow);
+            break;
+        }
+        const uint32_t node = 1u + 8u * pk + kv.digit(i, l - 1);
+        int_node[k] = node;
+        int_parent[k] = pk;
+        ref[node] = (int32_t)k;
+        atomicAdd(&nchild[pk], 1u);
+    }
+    // the body's leaf: one level below the deepest cell it shares with a neighbour
+    const int D = cp > cn ? cp : cn;
+    const uint32_t pk = owner_index(kv.cell_start(i, D), D);
+    if (pk >= icap) {
+        raise_flag(h, kFlagNodeOverflow);
+        body_parent[i] = 0xffffffffu;
+        return;
+    }
+    const uint32_t leaf = 1u + 8u * pk + kv.digit(i, D);
+    ref[leaf] = ref_of_body(sidx[i]);
+    body_parent[i] = pk;
+    atomicAdd(&nchild[pk], 1u);
+    atomicMax(&h->max_depth, (uint32_t)(D + 1));
+}
+
+// ---- 7. centre of mass --------------------------------------------------------
+__device__ inline uint32_t arrive_acq_rel(uint32_t *p) {
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void __launch_bounds__(kThreads) k_com(uint32_t n, uint32_t icap, TreeHdr *h,
+                                                  const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ z,
+                                                  const double *__restrict__ m,
+                                                  const uint32_t *__restrict__ sidx,
+                                                  const uint32_t *__restrict__ body_parent,
+                                                  const int32_t *__restrict__ ref,
+                                                  const uint32_t *__restrict__ int_parent,
+                                                  const uint32_t *__restrict__ nchild,
+                                                  uint32_t *arrive, InternalSums *sums,
+                                                  uint32_t *counts) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
+    uint32_t pk = body_parent[i];
+    if (pk == 0xffffffffu) return;  // single body (root leaf) or overflow
+    // a leaf has at most kMaxDepth ancestors; the bound only guards against a
+    // corrupt parent chain
+    for (int level = 0; level <= kMaxDepth; ++level) {
+        const uint32_t seen = arrive_acq_rel(&arrive[pk]);
+        if (seen + 1 != nchild[pk]) return;  // a sibling will finish this cell
+        double sx = 0, sy = 0, sz = 0, sm = 0;
+        uint32_t cnt = 0;
+        const uint64_t base = 1ull + 8ull * pk;
+        for (int d = 0; d < 8; ++d) {
+            const int32_t r = ref[base + d];
+            if (r >= 0) {
+                const InternalSums s = sums[r];
+                sx += s.x;
+                sy += s.y;
+                sz += s.z;
+                sm += s.m;
+                cnt += counts[r];
+            } else if (r != kRefEmpty) {
+                const uint32_t b = body_of_ref(r);
+                const double mb = m[b];
+                sx += x[b] * mb;
+                sy += y[b] * mb;
+                sz += z[b] * mb;
+                sm += mb;
+                cnt += 1;
+            }
+        }
+        sums[pk] = InternalSums{sx, sy, sz, sm};
+        counts[pk] = cnt;
+        if (sm == 0) raise_flag(h, kFlagZeroMass);
+        if (pk == 0) return;
+        pk = int_parent[pk];
+        if (pk >= icap) {
+            raise_flag(h, kFlagNodeOverflow);
+            return;
+        }
+    }
+    raise_flag(h, kFlagNodeOverflow);
+}
+
+// ---- 8. node records -----------------------------------------------------------
+__global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict__ h,
+                                                      uint32_t icap,
+                                                      const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      const double *__restrict__ m,
+                                                      const int32_t *__restrict__ ref,
+                                                      const InternalSums *__restrict__ sums,
+                                                      const uint8_t *__restrict__ int_depth,
+                                                      double theta, double4 *cm, double *edge,
+                                                      float *ropen2) {
+    const uint32_t I = min(h->internal_count, icap);
+    const uint64_t nodes = 1 + 8ull * I;
+    const double root_edge = h->edge;
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride) {
+        const int32_t r = ref[j];
+        double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
+        double e = 0.0;
+        float ro2 = 0.0f;
+        if (r >= 0) {
+            const InternalSums s = sums[r];
+            c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
+            e = ldexp(root_edge, -(int)int_depth[r]);
+            const double ro = e / theta;
+            ro2 = (float)(ro * ro);
+        } else if (r != kRefEmpty) {
+            const uint32_t b = body_of_ref(r);
+            const double mb = m[b];
+            c = make_double4((x[b] * mb) / mb, (y[b] * mb) / mb, (z[b] * mb) / mb, mb);
+        }
+        cm[j] = c;
+        edge[j] = e;
+        ropen2[j] = ro2;
+    }
+}
+
+}  // namespace
+
+void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_build_begin, dim3(1), dim3(kThreads), 0, s, t.hdr);
+    const uint32_t nb = std::min<uint32_t>(kAabbBlocks, ceil_div(t.n, 4 * kThreads));
+    hipLaunchKernelGGL(k_aabb_partial, dim3(nb), dim3(kThreads), 0, s, x, y, z, t.n, t.aabb_part,
+                       t.hdr);
+    hipLaunchKernelGGL(k_aabb_final, dim3(1), dim3(64), 0, s, (const double *)t.aabb_part, nb,
+                       t.hdr);
+}
+
+void launch_keys(const TreeBufs &t, const double *x, const double *y, const double *z,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, t.n,
+                       (const TreeHdr *)t.hdr, t.skey, t.sidx);
+    radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels, t.radix_hist,
+                       t.radix_blocks, t.scan_part, t.scan_part_cap, s);
+}
+
+void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,
+                     hipStream_t s) {
+    const uint32_t n = t.n;
+    const dim3 grid(ceil_div(n, kThreads));
+    if (n > 1) {
+        hipLaunchKernelGGL(k_lcp, grid, dim3(kThreads), 0, s, x, y, z, n, t.hdr, t.skey, t.sidx,
+                           t.klo, t.lcp);
+        hipLaunchKernelGGL(k_chain, grid, dim3(kThreads), 0, s, (const int32_t *)t.lcp, n,
+                           t.chain);
+        scan_exclusive_u32(t.chain, t.chain, n - 1, &t.hdr->internal_count, t.scan_part,
+                           t.scan_part_cap, s);
+    }
+    const uint32_t fill_blocks = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(8ull * n, kThreads)));
+    hipLaunchKernelGGL(k_fill, dim3(fill_blocks), dim3(kThreads), 0, s, t.hdr, t.icap, t.ref,
+                       t.int_nchild, t.int_arrive);
+    KeyView kv{t.skey, t.klo};
+    hipLaunchKernelGGL(k_link, grid, dim3(kThreads), 0, s, n, t.icap, t.hdr, kv,
+                       (const uint32_t *)t.sidx, (const int32_t *)t.lcp,
+                       (const uint32_t *)t.chain, t.ref, t.int_depth, t.int_parent, t.int_node,
+                       t.int_nchild, t.body_parent);
+}
+
+void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
+                const double *m, hipStream_t s) {
+    hipLaunchKernelGGL(k_com, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, t.n, t.icap, t.hdr, x,
+                       y, z, m, (const uint32_t *)t.sidx, (const uint32_t *)t.body_parent,
+                       (const int32_t *)t.ref, (const uint32_t *)t.int_parent,
+                       (const uint32_t *)t.int_nchild, t.int_arrive, t.int_sums, t.int_count);
+}
+
+void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
+                         const double *m, double G, double theta, hipStream_t s) {
+    (void)G;
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(8192, ceil_div(8ull * t.n, kThreads)));
+    hipLaunchKernelGGL(k_records, dim3(blocks), dim3(kThreads), 0, s, (const TreeHdr *)t.hdr,
+                       t.icap, x, y, z, m, (const int32_t *)t.ref,
+                       (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, theta,
+                       t.cm, t.edge, t.ropen2);
+}
+
+}  // namespace nb

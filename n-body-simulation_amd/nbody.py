@@ -1,0 +1,286 @@
+"""ctypes binding of libnbody_hip.so (include/nbody_capi.h) for tests and bench.
+
+The library is the product; this module only marshals numpy arrays across the
+C ABI.  It never falls back to another implementation: if the shared library
+is missing or no HIP device is present, the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnbody_hip.so")
+
+NB_OK = 0
+NB_E_INVALID_ARG = 1
+NB_E_HIP = 2
+NB_E_NODE_OVERFLOW = 3
+NB_E_STACK_OVERFLOW = 4
+NB_E_DEPTH = 5
+NB_E_ZERO_MASS = 6
+NB_E_NOT_BUILT = 7
+NB_E_AABB = 8
+NB_E_NO_DEVICE = 10
+
+PREC_FP64 = 0
+PREC_FP32C = 1
+ALGO_NAIVE = 0
+ALGO_BARNES_HUT = 1
+
+PHASES = ["AABB creation", "keys and sort", "topology", "Compute center of mass",
+          "Octree creation", "Acceleration Kernel Time", "Leapfrog Part 1", "Leapfrog Part 2"]
+
+# every symbol include/nbody_capi.h declares
+EXPORTS = [
+    "nb_default_params", "nb_create", "nb_destroy", "nb_set_params", "nb_get_params",
+    "nb_body_count", "nb_last_error", "nb_strerror", "nb_device_name", "nb_set_state",
+    "nb_get_state", "nb_get_accel", "nb_set_positions", "nb_forces_direct",
+    "nb_forces_direct_targets", "nb_bh_build", "nb_bh_forces", "nb_kick_drift", "nb_kick",
+    "nb_steps", "nb_phase_ms", "nb_tree_info", "nb_tree_sorted", "nb_tree_canonical",
+    "nb_bh_interactions", "nb_gen_star_disk", "nb_gen_plummer",
+]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("G", ctypes.c_double), ("eps2", ctypes.c_double), ("theta", ctypes.c_double),
+                ("precision", ctypes.c_uint32), ("sort_bodies", ctypes.c_uint32),
+                ("storage_size_param", ctypes.c_uint32), ("stack_size_param", ctypes.c_uint32)]
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, code, what, detail=""):
+        super().__init__(f"{what}: code {code} ({detail})")
+        self.code = code
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the shared library and declare the C signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    sig = {
+        "nb_default_params": (None, [ctypes.POINTER(Params)]),
+        "nb_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_uint32, ctypes.POINTER(Params)]),
+        "nb_destroy": (ctypes.c_int, [vp]),
+        "nb_set_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),
+        "nb_get_params": (ctypes.c_int, [vp, ctypes.POINTER(Params)]),
+        "nb_body_count": (ctypes.c_uint32, [vp]),
+        "nb_last_error": (ctypes.c_char_p, [vp]),
+        "nb_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+        "nb_device_name": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
+        "nb_set_state": (ctypes.c_int, [vp] + [_dp] * 7),
+        "nb_get_state": (ctypes.c_int, [vp] + [_dp] * 6),
+        "nb_get_accel": (ctypes.c_int, [vp] + [_dp] * 3),
+        "nb_set_positions": (ctypes.c_int, [vp] + [_dp] * 3),
+        "nb_forces_direct": (ctypes.c_int, [vp]),
+        "nb_forces_direct_targets": (ctypes.c_int, [vp, ctypes.c_uint32, _u32p] + [_dp] * 3),
+        "nb_bh_build": (ctypes.c_int, [vp]),
+        "nb_bh_forces": (ctypes.c_int, [vp]),
+        "nb_kick_drift": (ctypes.c_int, [vp, ctypes.c_double]),
+        "nb_kick": (ctypes.c_int, [vp, ctypes.c_double]),
+        "nb_steps": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_uint32]),
+        "nb_phase_ms": (ctypes.c_int, [vp, _dp]),
+        "nb_tree_info": (ctypes.c_int, [vp, _u64p, _u32p, _u32p, _dp]),
+        "nb_tree_sorted": (ctypes.c_int, [vp, _u32p]),
+        "nb_tree_canonical": (ctypes.c_uint64, [vp, ctypes.c_uint64, _u8p, _u8p, _u32p]
+                              + [_dp] * 4 + [_u32p]),
+        "nb_bh_interactions": (ctypes.c_int, [vp, _u32p]),
+        "nb_gen_star_disk": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
+        "nb_gen_plummer": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def default_params(**kw):
+    p = Params()
+    load().nb_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def strerror(code):
+    return load().nb_strerror(code).decode()
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    rc = load().nb_device_name(buf, 256)
+    if rc:
+        raise NBodyError(rc, "nb_device_name", strerror(rc))
+    return buf.value.decode()
+
+
+def gen_star_disk(n, seed):
+    out = [np.empty(n) for _ in range(7)]
+    rc = load().nb_gen_star_disk(n, seed, *map(_d, out))
+    if rc:
+        raise NBodyError(rc, "nb_gen_star_disk")
+    return dict(zip(["x", "y", "z", "vx", "vy", "vz", "m"], out))
+
+
+def gen_plummer(n, seed):
+    out = [np.empty(n) for _ in range(7)]
+    rc = load().nb_gen_plummer(n, seed, *map(_d, out))
+    if rc:
+        raise NBodyError(rc, "nb_gen_plummer")
+    return dict(zip(["x", "y", "z", "vx", "vy", "vz", "m"], out))
+
+
+class Context:
+    """One system of n bodies on the current HIP device."""
+
+    def __init__(self, n, params=None, **kw):
+        self.lib = load()
+        self.n = int(n)
+        p = params if params is not None else default_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        h = ctypes.c_void_p()
+        rc = self.lib.nb_create(ctypes.byref(h), self.n, ctypes.byref(p))
+        if rc:
+            raise NBodyError(rc, "nb_create", strerror(rc))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise NBodyError(rc, what, self.lib.nb_last_error(self.h).decode())
+
+    @property
+    def params(self):
+        p = Params()
+        self._check(self.lib.nb_get_params(self.h, ctypes.byref(p)), "nb_get_params")
+        return p
+
+    def set_params(self, **kw):
+        p = self.params
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self._check(self.lib.nb_set_params(self.h, ctypes.byref(p)), "nb_set_params")
+
+    def set_state(self, x, y, z, vx, vy, vz, m):
+        arrs = [_f64(a) for a in (x, y, z, vx, vy, vz, m)]
+        assert all(a.size == self.n for a in arrs)
+        self._keep = arrs
+        self._check(self.lib.nb_set_state(self.h, *map(_d, arrs)), "nb_set_state")
+
+    def set_positions(self, x, y, z):
+        arrs = [_f64(a) for a in (x, y, z)]
+        self._check(self.lib.nb_set_positions(self.h, *map(_d, arrs)), "nb_set_positions")
+
+    def state(self):
+        out = [np.empty(self.n) for _ in range(6)]
+        self._check(self.lib.nb_get_state(self.h, *map(_d, out)), "nb_get_state")
+        return dict(zip(["x", "y", "z", "vx", "vy", "vz"], out))
+
+    def accel(self):
+        out = [np.empty(self.n) for _ in range(3)]
+        self._check(self.lib.nb_get_accel(self.h, *map(_d, out)), "nb_get_accel")
+        return tuple(out)
+
+    def forces_direct(self):
+        self._check(self.lib.nb_forces_direct(self.h), "nb_forces_direct")
+        return self.accel()
+
+    def forces_direct_targets(self, targets):
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        out = [np.empty(t.size) for _ in range(3)]
+        self._check(self.lib.nb_forces_direct_targets(self.h, t.size, t.ctypes.data_as(_u32p),
+                                                      *map(_d, out)), "nb_forces_direct_targets")
+        return tuple(out)
+
+    def bh_build(self):
+        self._check(self.lib.nb_bh_build(self.h), "nb_bh_build")
+
+    def bh_forces(self):
+        self._check(self.lib.nb_bh_forces(self.h), "nb_bh_forces")
+        return self.accel()
+
+    def kick_drift(self, dt):
+        self._check(self.lib.nb_kick_drift(self.h, dt), "nb_kick_drift")
+
+    def kick(self, dt):
+        self._check(self.lib.nb_kick(self.h, dt), "nb_kick")
+
+    def steps(self, algorithm, dt, count):
+        algo = {"naive": ALGO_NAIVE, "BarnesHut": ALGO_BARNES_HUT}.get(algorithm, algorithm)
+        self._check(self.lib.nb_steps(self.h, algo, dt, count), "nb_steps")
+
+    def phase_ms(self):
+        out = np.zeros(len(PHASES))
+        self._check(self.lib.nb_phase_ms(self.h, _d(out)), "nb_phase_ms")
+        return dict(zip(PHASES, out))
+
+    def tree_info(self):
+        nodes = ctypes.c_uint64()
+        internal = ctypes.c_uint32()
+        depth = ctypes.c_uint32()
+        aabb = np.zeros(7)
+        self._check(self.lib.nb_tree_info(self.h, ctypes.byref(nodes), ctypes.byref(internal),
+                                          ctypes.byref(depth), _d(aabb)), "nb_tree_info")
+        return dict(node_count=nodes.value, internal_count=internal.value,
+                    max_depth=depth.value, aabb=aabb)
+
+    def tree_sorted(self):
+        out = np.empty(self.n, np.uint32)
+        self._check(self.lib.nb_tree_sorted(self.h, out.ctypes.data_as(_u32p)), "nb_tree_sorted")
+        return out
+
+    def interactions(self):
+        out = np.empty(self.n, np.uint32)
+        self._check(self.lib.nb_bh_interactions(self.h, out.ctypes.data_as(_u32p)),
+                    "nb_bh_interactions")
+        return out
+
+    def tree_canonical(self):
+        cap = self.tree_info()["node_count"]
+        out = dict(depth=np.zeros(cap, np.uint8), kind=np.zeros(cap, np.uint8),
+                   body=np.zeros(cap, np.uint32), mass=np.zeros(cap), com_x=np.zeros(cap),
+                   com_y=np.zeros(cap), com_z=np.zeros(cap), count=np.zeros(cap, np.uint32))
+        k = self.lib.nb_tree_canonical(self.h, cap, out["depth"].ctypes.data_as(_u8p),
+                                       out["kind"].ctypes.data_as(_u8p),
+                                       out["body"].ctypes.data_as(_u32p), _d(out["mass"]),
+                                       _d(out["com_x"]), _d(out["com_y"]), _d(out["com_z"]),
+                                       out["count"].ctypes.data_as(_u32p))
+        if k != cap:
+            raise NBodyError(-1, "nb_tree_canonical", f"returned {k}, expected {cap}")
+        return out

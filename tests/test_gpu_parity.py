@@ -1,0 +1,287 @@
+"""GPU parity: libnbody_hip.so (through its C ABI) against the CPU oracle.
+
+Bars: FP64 paths bit-exact (forces, integrator, octree sums); topology and
+body order bit-exact; FP32 paths within stated relative tolerances against
+the FP64 direct sum.
+"""
+import numpy as np
+import pytest
+
+from helpers import assert_canonical_equal, clustered, rel_err, uniform_cube
+
+pytestmark = pytest.mark.gpu
+
+
+def star_disk(nb, n, seed=0x4E424F44590000):
+    return nb.gen_star_disk(n, seed)
+
+
+def plummer(nb, n, seed=0x4E424F44590002):
+    return nb.gen_plummer(n, seed)
+
+
+def make(gpu, kind, n, seed=1):
+    if kind == "cube":
+        return uniform_cube(n, seed)
+    if kind == "cluster":
+        return clustered(n, seed)
+    if kind == "disk":
+        return star_disk(gpu, n)
+    if kind == "plummer":
+        return plummer(gpu, n)
+    raise ValueError(kind)
+
+
+def ctx_for(gpu, d, **kw):
+    c = gpu.Context(len(d["x"]), **kw)
+    c.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    return c
+
+
+def bits(a):
+    return np.asarray(a, dtype=np.float64).view(np.int64)
+
+
+# ---- direct sum ---------------------------------------------------------------------
+@pytest.mark.parametrize("kind,n", [("cube", 1), ("cube", 2), ("cube", 65), ("cube", 1000),
+                                    ("disk", 4099), ("cluster", 3000)])
+def test_direct_f64_bitexact(gpu, oracle, kind, n):
+    d = make(gpu, kind, n)
+    with ctx_for(gpu, d) as c:
+        ax, ay, az = c.forces_direct()
+    rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"])
+    for a, r in ((ax, rx), (ay, ry), (az, rz)):
+        np.testing.assert_array_equal(bits(a), bits(r))
+
+
+def test_direct_targets_f64(gpu, oracle):
+    d = make(gpu, "disk", 5000)
+    t = np.array([0, 1, 4999, 2500, 17, 17], dtype=np.uint32)
+    with ctx_for(gpu, d) as c:
+        ax, ay, az = c.forces_direct_targets(t)
+    rx, ry, rz = oracle.naive_accel_targets(d["x"], d["y"], d["z"], d["m"], t)
+    for a, r in ((ax, rx), (ay, ry), (az, rz)):
+        np.testing.assert_array_equal(bits(a), bits(r))
+
+
+@pytest.mark.parametrize("kind,n,med,worst", [("disk", 16384, 1e-6, 1e-4),
+                                              ("plummer", 20000, 1e-5, 1e-3),
+                                              ("cube", 777, 1e-5, 1e-3)])
+def test_direct_f32c_accuracy(gpu, oracle, kind, n, med, worst):
+    d = make(gpu, kind, n)
+    with ctx_for(gpu, d, precision=gpu.PREC_FP32C) as c:
+        ax, ay, az = c.forces_direct()
+    rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"])
+    e = rel_err(ax, ay, az, rx, ry, rz)
+    assert np.median(e) < med, np.median(e)
+    assert np.quantile(e, 0.999) < worst, np.quantile(e, 0.999)
+
+
+# ---- integrator -----------------------------------------------------------------------
+def test_leapfrog_bitexact(gpu, oracle):
+    d = make(gpu, "cube", 3001, 5)
+    dt = 1.0 / 24
+    with ctx_for(gpu, d) as c:
+        c.forces_direct()
+        a = c.accel()
+        c.kick_drift(dt)
+        s1 = c.state()
+        c.forces_direct()
+        a2 = c.accel()
+        c.kick(dt)
+        s2 = c.state()
+    x = [d["x"].copy(), d["y"].copy(), d["z"].copy()]
+    vh = oracle.kick_drift(dt, list(a), [d["vx"], d["vy"], d["vz"]], x)
+    for k, ref in zip(("x", "y", "z"), x):
+        np.testing.assert_array_equal(bits(s1[k]), bits(ref))
+    v = oracle.kick(dt, list(a2), vh)
+    for k, ref in zip(("vx", "vy", "vz"), v):
+        np.testing.assert_array_equal(bits(s2[k]), bits(ref))
+
+
+# ---- octree -----------------------------------------------------------------------------
+def test_tree_reference_fixture(gpu, oracle):
+    # tests/BarnesHutTest.cpp:11-33 and :75-127
+    d = dict(x=np.array([0.0, 0.0, 2.0]), y=np.array([1.0, 0.0, 0.0]), z=np.array([0.0, 2.0, 0.0]),
+             vx=np.zeros(3), vy=np.zeros(3), vz=np.zeros(3), m=np.array([10.0, 10.0, 10.0]))
+    with ctx_for(gpu, d) as c:
+        c.bh_build()
+        info = c.tree_info()
+        assert list(info["aabb"]) == [0, -0.5, 0, 2, 1.5, 2, 2]
+        assert info["node_count"] == 9
+        assert list(c.tree_sorted()) == [1, 2, 0]
+        can = c.tree_canonical()
+    assert can["mass"][0] == 30.0
+    # body0 in digit 5, body2 in digit 3, body1 in digit 0
+    kids = {int(can["body"][i]): i - 1 for i in range(1, 9) if can["kind"][i] == 1}
+    assert kids == {0: 5, 2: 3, 1: 0}
+    t = oracle.Tree(3, 48)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    assert_canonical_equal(can, t.canonical(), "fixture")
+
+
+@pytest.mark.parametrize("kind,n", [("cube", 1), ("cube", 2), ("cube", 9), ("cube", 1000),
+                                    ("cluster", 5000), ("disk", 20000), ("plummer", 20000),
+                                    ("cube", 70001)])
+def test_tree_parity(gpu, oracle, kind, n):
+    d = make(gpu, kind, n, seed=n)
+    with ctx_for(gpu, d) as c:
+        c.bh_build()
+        info = c.tree_info()
+        sorted_gpu = c.tree_sorted()
+        can = c.tree_canonical()
+    t = oracle.Tree(n)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    np.testing.assert_array_equal(info["aabb"], t.aabb)
+    assert info["node_count"] == t.node_count
+    np.testing.assert_array_equal(sorted_gpu, t.export()["sorted"])
+    assert_canonical_equal(can, t.canonical(), f"{kind} n={n}")
+
+
+def test_tree_deep_runs(gpu, oracle):
+    # pairs of bodies closer than edge/2^21 force the 42-digit path
+    rng = np.random.default_rng(3)
+    n = 600
+    base = rng.uniform(-1, 1, (n // 2, 3))
+    off = rng.normal(0, 1e-9, (n // 2, 3))
+    p = np.concatenate([base, base + off])
+    d = dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+             vy=np.zeros(n), vz=np.zeros(n), m=rng.uniform(1, 2, n))
+    with ctx_for(gpu, d) as c:
+        c.bh_build()
+        info = c.tree_info()
+        can = c.tree_canonical()
+        srt = c.tree_sorted()
+    assert info["max_depth"] > 21
+    t = oracle.Tree(n)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    np.testing.assert_array_equal(srt, t.export()["sorted"])
+    assert_canonical_equal(can, t.canonical(), "deep runs")
+
+
+# ---- Barnes-Hut forces -------------------------------------------------------------------
+@pytest.mark.parametrize("kind,n,theta", [("cube", 2000, 0.5), ("cluster", 4000, 1.05),
+                                          ("disk", 20000, 0.5), ("plummer", 20000, 0.7)])
+def test_bh_exact_bitexact(gpu, oracle, kind, n, theta):
+    d = make(gpu, kind, n, seed=7)
+    with ctx_for(gpu, d, theta=theta) as c:
+        c.bh_build()
+        ax, ay, az = c.bh_forces()
+        inter = c.interactions()
+    t = oracle.Tree(n)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    rx, ry, rz, rint = t.bh_accel(theta)
+    np.testing.assert_array_equal(inter, rint)
+    for a, r in ((ax, rx), (ay, ry), (az, rz)):
+        np.testing.assert_array_equal(bits(a), bits(r))
+
+
+@pytest.mark.parametrize("kind,n,theta", [("disk", 30000, 0.5), ("plummer", 30000, 0.5),
+                                          ("plummer", 30000, 1.0), ("cluster", 8000, 0.7)])
+def test_bh_group_accuracy(gpu, oracle, kind, n, theta):
+    d = make(gpu, kind, n, seed=9)
+    with ctx_for(gpu, d, theta=theta, precision=gpu.PREC_FP32C) as c:
+        c.bh_build()
+        ax, ay, az = c.bh_forces()
+    rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"])
+    e = rel_err(ax, ay, az, rx, ry, rz)
+    t = oracle.Tree(n)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    bx, by, bz, _ = t.bh_accel(theta)
+    eref = rel_err(bx, by, bz, rx, ry, rz)
+    # opening a cell when any body of the group needs it is at least as
+    # accurate as the per-body test; FP32 pair maths adds ~1e-6
+    assert np.median(e) <= np.median(eref) + 2e-6, (np.median(e), np.median(eref))
+    if theta <= 0.5:
+        assert np.median(e) < 1e-4
+
+
+# ---- full loop ------------------------------------------------------------------------------
+@pytest.mark.parametrize("algo", ["naive", "BarnesHut"])
+def test_simulation_loop_bitexact(gpu, oracle, algo):
+    n = 3000
+    d = make(gpu, "disk", n)
+    dt, steps = 1.0 / 24, 6
+    theta = 0.5
+    with ctx_for(gpu, d, theta=theta) as c:
+        if algo == "naive":
+            c.forces_direct()
+        else:
+            c.bh_build()
+            c.bh_forces()
+        c.steps(algo, dt, steps)
+        s = c.state()
+        a = c.accel()
+    r = oracle.simulate(algo, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt,
+                        steps * dt, theta=theta)
+    assert r["steps"] == steps
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(bits(s[k]), bits(r[k]), err_msg=k)
+    for got, k in zip(a, ("ax", "ay", "az")):
+        np.testing.assert_array_equal(bits(got), bits(r[k]), err_msg=k)
+
+
+# ---- error handling ----------------------------------------------------------------------------
+def test_coincident_bodies_reported(gpu):
+    d = uniform_cube(100, 2)
+    d["x"][5], d["y"][5], d["z"][5] = d["x"][9], d["y"][9], d["z"][9]
+    with ctx_for(gpu, d) as c:
+        with pytest.raises(gpu.NBodyError) as e:
+            c.bh_build()
+        assert e.value.code == gpu.NB_E_DEPTH
+
+
+def test_zero_mass_reported(gpu):
+    d = uniform_cube(100, 3)
+    d["m"][4] = 0.0
+    with ctx_for(gpu, d) as c:
+        with pytest.raises(gpu.NBodyError) as e:
+            c.bh_build()
+        assert e.value.code == gpu.NB_E_ZERO_MASS
+
+
+def test_nan_position_reported(gpu):
+    d = uniform_cube(100, 4)
+    d["y"][50] = np.nan
+    with ctx_for(gpu, d) as c:
+        with pytest.raises(gpu.NBodyError) as e:
+            c.bh_build()
+        assert e.value.code == gpu.NB_E_AABB
+
+
+def test_forces_before_build(gpu):
+    d = uniform_cube(100, 5)
+    with ctx_for(gpu, d) as c:
+        with pytest.raises(gpu.NBodyError) as e:
+            c.bh_forces()
+        assert e.value.code == gpu.NB_E_NOT_BUILT
+
+
+def test_storage_overflow_reported(gpu):
+    d = clustered(4000, 6)
+    with ctx_for(gpu, d, storage_size_param=1) as c:
+        with pytest.raises(gpu.NBodyError) as e:
+            c.bh_build()
+        assert e.value.code == gpu.NB_E_NODE_OVERFLOW
+
+
+# ---- full-size properties ------------------------------------------------------------------------
+def test_million_body_tree_properties(gpu):
+    n = 1 << 20
+    d = make(gpu, "disk", n)
+    with ctx_for(gpu, d, theta=0.5, precision=gpu.PREC_FP32C) as c:
+        c.bh_build()
+        info = c.tree_info()
+        srt = c.tree_sorted()
+        assert np.array_equal(np.sort(srt), np.arange(n, dtype=np.uint32))
+        assert info["node_count"] == 1 + 8 * info["internal_count"]
+        can = c.tree_canonical()
+        assert can["count"][0] == n
+        assert (can["kind"] == 1).sum() == n
+        np.testing.assert_array_equal(can["body"][can["kind"] == 1], srt)
+        ax, ay, az = c.bh_forces()
+        rng = np.random.default_rng(0)
+        t = np.sort(rng.choice(n, 16384, replace=False)).astype(np.uint32)
+        rx, ry, rz = c.forces_direct_targets(t)
+    e = rel_err(ax[t], ay[t], az[t], rx, ry, rz)
+    assert np.median(e) < 1e-4
