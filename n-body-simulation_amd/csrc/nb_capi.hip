@@ -77,7 +77,9 @@ void dev_free(nb_ctx *c, void *p) {
     (void)hipFree(p);
 }
 
-// Phase timing with HIP events on the context stream.
+// Phase timing with HIP events on the context stream.  Phases may nest
+// (the octree phases inside "Octree creation"); an end closes the most
+// recent open mark of its phase.
 void mark_begin(nb_ctx *c, int phase) {
     const size_t idx = c->marks.size() * 2;
     while (c->events.size() < idx + 2) {
@@ -89,9 +91,13 @@ void mark_begin(nb_ctx *c, int phase) {
     (void)hipEventRecord(c->events[idx], c->stream);
 }
 
-void mark_end(nb_ctx *c) {
-    if (c->marks.empty()) return;
-    (void)hipEventRecord(c->events[c->marks.back().second + 1], c->stream);
+void mark_end(nb_ctx *c, int phase) {
+    for (size_t k = c->marks.size(); k-- > 0;) {
+        if (c->marks[k].first == phase) {
+            (void)hipEventRecord(c->events[c->marks[k].second + 1], c->stream);
+            return;
+        }
+    }
 }
 
 void timing_reset(nb_ctx *c) {
@@ -147,6 +153,9 @@ int alloc_tree(nb_ctx *c) {
     const uint64_t storage = (uint64_t)std::max<uint32_t>(c->p.storage_size_param, 1) * n;
     t.icap = (uint32_t)std::max<uint64_t>(1, storage / 8);
     t.mcap = 1 + 8ull * t.icap;
+    // the group traversal packs an internal-cell index into 26 bits
+    if (t.icap >= (1u << 26))
+        return fail(c, NB_E_INVALID_ARG, "storage_size_param * N exceeds 2^29 octree nodes");
     int rc;
     if ((rc = dev_alloc(c, &t.hdr, 1))) return rc;
     if ((rc = dev_alloc(c, &t.aabb_part, 6ull * nb::kAabbBlocks))) return rc;
@@ -165,10 +174,17 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.int_arrive, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_count, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_sums, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_local, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_first, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.groups, n))) return rc;
+    // a cell crossing a COM block boundary holds that boundary's two bodies:
+    // at most kMaxDepth such cells per boundary
+    t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
+    if ((rc = dev_alloc(c, &t.spanning, t.spanning_cap))) return rc;
     if ((rc = dev_alloc(c, &t.ref, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.cm, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.edge, t.mcap))) return rc;
-    if ((rc = dev_alloc(c, &t.ropen2, t.mcap))) return rc;
+    if ((rc = dev_alloc(c, &t.rec, t.mcap))) return rc;
     t.radix_blocks = nb::radix_blocks_for(n);
     if ((rc = dev_alloc(c, &t.radix_hist, 256ull * t.radix_blocks))) return rc;
     t.scan_part_cap = std::max(nb::scan_scratch_words(256u * t.radix_blocks),
@@ -188,17 +204,18 @@ int build_tree(nb_ctx *c) {
     TreeBufs &t = c->tree;
     mark_begin(c, NB_PHASE_AABB);
     nb::launch_aabb(t, c->x, c->y, c->z, c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_AABB);
     mark_begin(c, NB_PHASE_KEYS_SORT);
     nb::launch_keys(t, c->x, c->y, c->z, c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_KEYS_SORT);
     mark_begin(c, NB_PHASE_TOPOLOGY);
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
     nb::launch_com(t, c->x, c->y, c->z, c->m, c->stream);
-    nb::launch_node_records(t, c->x, c->y, c->z, c->m, c->p.G, c->p.theta, c->stream);
-    mark_end(c);
+    const bool fp64 = c->p.precision == NB_PREC_FP64;
+    nb::launch_node_records(t, c->x, c->y, c->z, c->m, c->p.G, fp64, !fp64, c->stream);
+    mark_end(c, NB_PHASE_COM);
     return NB_OK;
 }
 
@@ -217,10 +234,10 @@ int bh_forces_enqueue(nb_ctx *c) {
                             c->p.sort_bodies, c->exact_stack, c->exact_cap, c->ax, c->ay, c->az,
                             c->inter, c->stream);
     } else {
-        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->p.G, c->p.eps2, c->p.theta, c->ax,
-                            c->ay, c->az, c->inter, c->stream);
+        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->p.eps2, c->p.theta, c->ax, c->ay,
+                            c->az, c->inter, c->stream);
     }
-    mark_end(c);
+    mark_end(c, NB_PHASE_ACCEL);
     return NB_OK;
 }
 
@@ -242,7 +259,7 @@ int direct_enqueue(nb_ctx *c) {
         nb::launch_direct_f32c(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay,
                                c->az, c->f32_scratch, c->f32_splits, c->stream);
     }
-    mark_end(c);
+    mark_end(c, NB_PHASE_ACCEL);
     return NB_OK;
 }
 
@@ -349,8 +366,9 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         TreeBufs &t = c->tree;
         void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
-                        t.int_nchild, t.int_arrive, t.int_count, t.int_sums, t.ref, t.cm,
-                        t.edge, t.ropen2, t.radix_hist, t.scan_part};
+                        t.int_nchild, t.int_arrive, t.int_count, t.int_sums, t.int_local,
+                        t.int_first, t.groups, t.spanning, t.ref, t.cm, t.edge, t.rec,
+                        t.radix_hist, t.scan_part};
         (void)hipStreamSynchronize(c->stream);
         for (void *b : bufs) dev_free(c, b);
         c->tree = TreeBufs();
@@ -453,7 +471,7 @@ int nb_forces_direct_targets(nb_ctx *c, uint32_t nt, const uint32_t *targets, do
     nb::launch_direct_targets_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, nt,
                                   c->targets_dev, c->targets_out, c->targets_out + nt,
                                   c->targets_out + 2ull * nt, c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_ACCEL);
     double *dst[] = {ax, ay, az};
     for (int k = 0; k < 3; ++k)
         NB_HIP(c, hipMemcpyAsync(dst[k], c->targets_out + (uint64_t)k * nt, (size_t)nt * 8,
@@ -466,7 +484,7 @@ int nb_bh_build(nb_ctx *c) {
     timing_reset(c);
     mark_begin(c, NB_PHASE_OCTREE);
     build_tree(c);
-    mark_end(c);
+    mark_end(c, NB_PHASE_OCTREE);
     int rc = finish(c, true);
     c->tree_valid = rc == NB_OK;
     return rc;
@@ -487,7 +505,7 @@ int nb_kick_drift(nb_ctx *c, double dt) {
     mark_begin(c, NB_PHASE_LF1);
     nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx, c->vhy,
                           c->vhz, c->x, c->y, c->z, c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_LF1);
     c->tree_valid = false;
     return finish(c, false);
 }
@@ -498,7 +516,7 @@ int nb_kick(nb_ctx *c, double dt) {
     mark_begin(c, NB_PHASE_LF2);
     nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy, c->vz,
                     c->stream);
-    mark_end(c);
+    mark_end(c, NB_PHASE_LF2);
     return finish(c, false);
 }
 
@@ -510,12 +528,12 @@ int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
         mark_begin(c, NB_PHASE_LF1);
         nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx, c->vhy,
                               c->vhz, c->x, c->y, c->z, c->stream);
-        mark_end(c);
+        mark_end(c, NB_PHASE_LF1);
         int rc;
         if (algorithm == NB_ALGO_BARNES_HUT) {
             mark_begin(c, NB_PHASE_OCTREE);
             build_tree(c);
-            mark_end(c);
+            mark_end(c, NB_PHASE_OCTREE);
             rc = bh_forces_enqueue(c);
         } else {
             rc = direct_enqueue(c);
@@ -524,7 +542,7 @@ int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
         mark_begin(c, NB_PHASE_LF2);
         nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy,
                         c->vz, c->stream);
-        mark_end(c);
+        mark_end(c, NB_PHASE_LF2);
     }
     const bool bh = algorithm == NB_ALGO_BARNES_HUT && count > 0;
     int rc = finish(c, bh);

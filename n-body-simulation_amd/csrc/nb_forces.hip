@@ -9,10 +9,10 @@
 // FP32 kernels do the pair arithmetic in FP32 (fmaf, v_rsq_f32) and fold the
 // FP32 partial of every tile into FP64 accumulators:
 //   k_direct_f32c  LDS-staged source tiles, two targets per lane, split j
-//   k_bh_group     one wavefront per 64 consecutive (octree-ordered) bodies;
+//   k_bh_group     one wavefront per group (an octree cell of <= 64 bodies);
 //                  a cell is opened if any body of the group needs it; lanes
 //                  test 8 sibling sets at a time and accepted cells/bodies are
-//                  batched in LDS and evaluated 64 at a time by every lane.
+//                  batched in LDS and evaluated 64 at a time.
 #include <algorithm>
 #include <cmath>
 
@@ -269,9 +269,25 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 
 // ---------------------------------------------------------------------------
 // Group traversal (FP32 pair maths, FP64 accumulation).
-constexpr int kGroup = 64;          // bodies per wavefront
-constexpr int kStackCap = 2048;     // internal cells waiting to be opened
+//
+// One wavefront walks the octree for one group: an octree cell with at most
+// 64 bodies (a contiguous run of the octree order).  A cell is accepted when
+// its centre of mass is farther than edge/theta from the group's bounding box
+// -- so from every body of the group: the reference's opening test
+// (BarnesHutAlgorithm.cpp:355-359) taken over the group; bodies in leaves are
+// always taken exactly.  Each iteration pops up to 8 cells whose children
+// must be examined, and lane l examines child (l & 7) of cell l >> 3: one
+// 32-byte record load per lane.  Accepted cells and bodies are appended to an
+// LDS list; every 64 entries the list is evaluated against the group's bodies.
+// A group of s bodies uses 64/s2 lanes per body (s2 = s rounded up to a power
+// of two), each taking every (64/s2)-th list entry, so all 64 lanes stay busy.
+// Positions are relative to the group centre; the records' hi/lo split keeps
+// the FP32 differences exact to the last FP32 bit.  Waves take groups from an
+// atomic work queue.
+constexpr int kGroup = 64;          // lanes per wavefront = bodies per group at most
+constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 6 | depth)
 constexpr int kListCap = 2 * kGroup;
+constexpr int kGroupBlocks = 8192;  // persistent waves
 
 __device__ inline uint64_t lanes_below() {
     const uint32_t lane = threadIdx.x & 63u;
@@ -280,128 +296,175 @@ __device__ inline uint64_t lanes_below() {
 
 __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double4 *__restrict__ cm,
-    const float *__restrict__ ropen2, const int32_t *__restrict__ ref, double G, float eps2,
-    uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter, TreeHdr *h) {
+    const double *__restrict__ y, const double *__restrict__ z,
+    const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
+    float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
+    TreeHdr *h) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
     const uint32_t lane = threadIdx.x;
-    const uint32_t gid = blockIdx.x * kGroup + lane;
-    const bool valid = gid < n;
-    const uint32_t bi = sidx[valid ? gid : n - 1];
-    const double px = x[bi], py = y[bi], pz = z[bi];
-
-    // group bounding box (FP64, exact min/max), centre and half extent
-    double mnx = px, mny = py, mnz = pz, mxx = px, mxy = py, mxz = pz;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
-        mny = fmin(mny, __shfl_xor(mny, off, 64));
-        mnz = fmin(mnz, __shfl_xor(mnz, off, 64));
-        mxx = fmax(mxx, __shfl_xor(mxx, off, 64));
-        mxy = fmax(mxy, __shfl_xor(mxy, off, 64));
-        mxz = fmax(mxz, __shfl_xor(mxz, off, 64));
-    }
-    const double cx = 0.5 * (mnx + mxx), cy = 0.5 * (mny + mxy), cz = 0.5 * (mnz + mxz);
-    // half extents rounded up so that the FP32 box always contains the bodies
-    const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f;
-    const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f;
-    const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f;
-    const float rx = (float)(px - cx), ry = (float)(py - cy), rz = (float)(pz - cz);
-
-    double acc_x = 0, acc_y = 0, acc_z = 0;
-    uint32_t n_inter = 0;
-    uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
     const uint64_t below = lanes_below();
-
-    auto flush = [&](uint32_t cnt) {
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (uint32_t e = 0; e < cnt; ++e) pair_f32(lst[e], rx, ry, rz, eps2, fx, fy, fz);
-        acc_x += (double)fx;
-        acc_y += (double)fy;
-        acc_z += (double)fz;
-        n_inter += cnt;
-    };
-
-    // classify one node for the whole group: 0 skip, 1 list, 2 open
-    auto classify = [&](uint32_t node, bool act, float4 &entry, int32_t &r) -> int {
-        r = act ? ref[node] : kRefEmpty;
-        if (r == kRefEmpty) return 0;
-        const double4 c = cm[node];
-        const float dx = (float)(c.x - cx), dy = (float)(c.y - cy), dz = (float)(c.z - cz);
-        entry = make_float4(dx, dy, dz, (float)(G * c.w));
-        if (r < 0) return 1;  // body leaf: exact pair (self pair contributes 0)
-        const float ex = fmaxf(fabsf(dx) - hx, 0.f);
-        const float ey = fmaxf(fabsf(dy) - hy, 0.f);
-        const float ez = fmaxf(fabsf(dz) - hz, 0.f);
-        const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-        return d2 > ropen2[node] ? 1 : 2;
-    };
-
-    {
-        float4 e;
-        int32_t r;
-        const int cls = classify(0, true, e, r);  // every lane evaluates the root alike
-        if (cls == 1) {
-            if (lane == 0) lst[0] = e;
-            ln = 1;
-        } else if (cls == 2) {
-            if (lane == 0) stk[0] = (uint32_t)r;
-            sp = 1;
-        }
-        __syncthreads();
-    }
+    const uint32_t ngroups = min(h->group_count, n);
+    // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
+    const double ro_root = h->edge / theta;
+    const float ro2_root = (float)(ro_root * ro_root);
+    uint32_t peak = 0;
     bool overflow = false;
-    // A correct octree opens every internal cell at most once per group, so
-    // icap iterations bound the walk; the cap turns a corrupt tree into an
-    // error instead of an endless loop.
-    uint32_t budget = icap + 16;
-    while (sp > 0) {
-        if (budget-- == 0) {
-            overflow = true;
+
+    while (!overflow) {
+        uint32_t slot = 0;
+        if (lane == 0)
+            slot = __hip_atomic_fetch_add(&h->group_next, 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = __shfl(slot, 0, 64);
+        if (g >= ngroups) break;
+        const uint2 grp = groups[g];
+        if (grp.y == 0 || grp.y > (uint32_t)kGroup || grp.x >= n || grp.y > n - grp.x) {
+            overflow = true;  // corrupt group table: report instead of reading out of bounds
             break;
         }
-        const uint32_t take = sp < 8u ? sp : 8u;
-        sp -= take;
-        const uint32_t slot = lane >> 3;
-        const bool act = slot < take;
-        const uint32_t k = act ? stk[sp + slot] : 0u;
-        __syncthreads();
-        const uint32_t node = 1u + 8u * k + (lane & 7u);
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-        int32_t r;
-        const int cls = classify(node, act, e, r);
-        const uint64_t bl = __ballot(cls == 1);
-        const uint64_t bo = __ballot(cls == 2);
-        if (cls == 1) lst[ln + __popcll(bl & below)] = e;
-        if (cls == 2) {
-            const uint32_t p = sp + __popcll(bo & below);
-            if (p < kStackCap) stk[p] = (uint32_t)r;
+        const uint32_t s = grp.y;
+        const uint32_t s2 = s <= 1 ? 1u : 1u << (32 - __clz(s - 1));
+        const uint32_t sub = lane & (s2 - 1);  // which body of the group
+        const uint32_t q = lane / s2;          // which share of the list
+        const uint32_t lpb = kGroup / s2;      // lanes per body
+        const bool valid = sub < s;
+        const uint32_t bi = sidx[grp.x + (valid ? sub : 0)];
+        const double px = x[bi], py = y[bi], pz = z[bi];
+
+        // bounding box (FP64, exact min/max), centre and half extent
+        double mnx = px, mny = py, mnz = pz, mxx = px, mxy = py, mxz = pz;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
+            mny = fmin(mny, __shfl_xor(mny, off, 64));
+            mnz = fmin(mnz, __shfl_xor(mnz, off, 64));
+            mxx = fmax(mxx, __shfl_xor(mxx, off, 64));
+            mxy = fmax(mxy, __shfl_xor(mxy, off, 64));
+            mxz = fmax(mxz, __shfl_xor(mxz, off, 64));
         }
-        ln += __popcll(bl);
-        sp += __popcll(bo);
-        if (sp > kStackCap) {
-            overflow = true;
-            break;
+        const double cx = 0.5 * (mnx + mxx), cy = 0.5 * (mny + mxy), cz = 0.5 * (mnz + mxz);
+        // half extents rounded up so that the FP32 box always contains the bodies
+        const float hx = (float)(0.5 * (mxx - mnx)) * 1.000001f;
+        const float hy = (float)(0.5 * (mxy - mny)) * 1.000001f;
+        const float hz = (float)(0.5 * (mxz - mnz)) * 1.000001f;
+        const float gxh = (float)cx, gyh = (float)cy, gzh = (float)cz;
+        const float gxl = (float)(cx - (double)gxh), gyl = (float)(cy - (double)gyh),
+                    gzl = (float)(cz - (double)gzh);
+        const float rx = (float)(px - cx), ry = (float)(py - cy), rz = (float)(pz - cz);
+
+        double acc_x = 0, acc_y = 0, acc_z = 0;
+        uint32_t n_inter = 0;
+        uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
+
+        auto flush = [&](uint32_t cnt) {
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            for (uint32_t e = q; e < cnt; e += lpb) pair_f32(lst[e], rx, ry, rz, eps2, fx, fy, fz);
+            acc_x += (double)fx;
+            acc_y += (double)fy;
+            acc_z += (double)fz;
+            n_inter += cnt;
+        };
+
+        // 0 skip, 1 list, 2 open
+        auto classify = [&](const NodeF32 &r, int depth, float4 &entry) -> int {
+            if (r.ref == kRefEmpty) return 0;
+            const float dx = (r.x - gxh) + (r.lx - gxl);
+            const float dy = (r.y - gyh) + (r.ly - gyl);
+            const float dz = (r.z - gzh) + (r.lz - gzl);
+            entry = make_float4(dx, dy, dz, r.gm);
+            if (r.ref < 0) return 1;  // body leaf: exact pair (the self pair adds 0)
+            const float ex = fmaxf(fabsf(dx) - hx, 0.f);
+            const float ey = fmaxf(fabsf(dy) - hy, 0.f);
+            const float ez = fmaxf(fabsf(dz) - hz, 0.f);
+            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+            return d2 > ldexpf(ro2_root, -2 * depth) ? 1 : 2;
+        };
+
+        {
+            float4 e;
+            const NodeF32 root = rec[0];
+            const int cls = classify(root, 0, e);  // every lane evaluates the root alike
+            if (cls == 1) {
+                if (lane == 0) lst[0] = e;
+                ln = 1;
+            } else if (cls == 2) {
+                if (lane == 0) stk[0] = ((uint32_t)root.ref << 6);
+                sp = 1;
+            }
+            __syncthreads();
         }
+
+        // pop up to 8 cells and load the record of this lane's child
+        NodeF32 r;
+        int depth = 0;
+        auto pop_and_load = [&]() -> uint32_t {
+            const uint32_t take = sp < 8u ? sp : 8u;
+            sp -= take;
+            const bool act = (lane >> 3) < take;
+            const uint32_t entry = act ? stk[sp + (lane >> 3)] : 0u;
+            depth = (int)(entry & 63u) + 1;
+            if (act) r = rec[1ull + 8ull * (entry >> 6) + (lane & 7u)];
+            else r.ref = kRefEmpty;
+            __syncthreads();
+            return take;
+        };
+
+        // A correct octree opens every internal cell at most once per group,
+        // so icap iterations bound the walk; the cap turns a corrupt tree into
+        // an error instead of an endless loop.
+        uint32_t budget = icap + 16;
+        uint32_t take = sp > 0 ? pop_and_load() : 0u;
+        while (take > 0) {
+            if (budget-- == 0) {
+                overflow = true;
+                break;
+            }
+            float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+            const int cls = classify(r, depth, e);
+            const uint64_t bl = __ballot(cls == 1);
+            const uint64_t bo = __ballot(cls == 2);
+            if (cls == 1) lst[ln + __popcll(bl & below)] = e;
+            if (cls == 2) {
+                const uint32_t p = sp + __popcll(bo & below);
+                if (p < kStackCap) stk[p] = ((uint32_t)r.ref << 6) | (uint32_t)depth;
+            }
+            ln += __popcll(bl);
+            sp += __popcll(bo);
+            peak = sp > peak ? sp : peak;
+            if (sp > kStackCap) {
+                overflow = true;
+                break;
+            }
+            __syncthreads();
+            // issue the next loads; the list evaluation below hides their latency
+            take = sp > 0 ? pop_and_load() : 0u;
+            if (ln >= kGroup) {
+                flush(kGroup);
+                __syncthreads();
+                const uint32_t rest = ln - kGroup;
+                if (lane < rest) lst[lane] = lst[kGroup + lane];
+                ln = rest;
+                __syncthreads();
+            }
+        }
+        if (ln > 0) flush(ln);
         __syncthreads();
-        if (ln >= kGroup) {
-            flush(kGroup);
-            __syncthreads();
-            const uint32_t rest = ln - kGroup;
-            if (lane < rest) lst[lane] = lst[kGroup + lane];
-            ln = rest;
-            __syncthreads();
+        // add up the list shares of each body (fixed order: deterministic)
+        for (uint32_t off = s2; off < (uint32_t)kGroup; off <<= 1) {
+            acc_x += __shfl_xor(acc_x, off, 64);
+            acc_y += __shfl_xor(acc_y, off, 64);
+            acc_z += __shfl_xor(acc_z, off, 64);
+        }
+        if (valid && q == 0 && !overflow) {
+            ax[bi] = acc_x;
+            ay[bi] = acc_y;
+            az[bi] = acc_z;
+            if (inter) inter[bi] = n_inter;
         }
     }
     if (overflow) raise_flag(h, kFlagStack);
-    if (ln > 0) flush(ln);
-    if (valid) {
-        ax[bi] = acc_x;
-        ay[bi] = acc_y;
-        az[bi] = acc_z;
-        if (inter) inter[bi] = n_inter;
-    }
+    if (lane == 0) atomicMax(&h->stack_peak, peak);
 }
 
 }  // namespace
@@ -456,13 +519,13 @@ void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const 
 }
 
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     double G, double eps2, double theta, double *ax, double *ay, double *az,
+                     double eps2, double theta, double *ax, double *ay, double *az,
                      uint32_t *interactions, hipStream_t s) {
-    (void)theta;  // folded into the node records' opening radii
-    hipLaunchKernelGGL(k_bh_group, dim3(ceil_div(t.n, kGroup)), dim3(kGroup), 0, s, t.n,
-                       (const uint32_t *)t.sidx, x, y, z, (const double4 *)t.cm,
-                       (const float *)t.ropen2, (const int32_t *)t.ref, G, (float)eps2, t.icap,
-                       ax, ay, az, interactions, t.hdr);
+    const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
+    hipLaunchKernelGGL(k_bh_group, dim3(blocks), dim3(kGroup), 0, s, t.n,
+                       (const uint32_t *)t.sidx, x, y, z, (const NodeF32 *)t.rec,
+                       (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                       interactions, t.hdr);
 }
 
 }  // namespace nb

@@ -43,11 +43,33 @@ struct TreeHdr {
     uint32_t max_depth;                                // deepest leaf
     uint32_t status;                                   // kFlag* bits
     uint32_t deep_runs;                                // runs of bodies sharing 21 digits
+    uint32_t spanning_count;                           // cells crossing a COM block boundary
+    uint32_t stack_peak;                               // deepest group-traversal stack seen
+    uint32_t group_count;                              // traversal groups
+    uint32_t group_next;                               // work-queue cursor of the traversal
 };
+
+// Bodies per traversal group: a group is an octree cell holding at most
+// kGroupMax bodies whose parent holds more (or a lone body under such a parent).
+constexpr uint32_t kGroupMax = 64;
 
 struct InternalSums {
     double x, y, z, m;  // centre-of-mass numerators and mass
 };
+
+// Traversal record of one node (32 bytes, children of a cell are adjacent):
+// FP32 centre of mass split into a high part and the FP32 rounding of the
+// remainder (hi + lo carries ~48 bits), G*m in FP32, and the node reference.
+struct NodeF32 {
+    float x, y, z, gm;
+    int32_t ref;
+    float lx, ly, lz;
+};
+
+// Bodies per block of the first centre-of-mass pass: cells whose body range
+// stays inside one block are summed there with workgroup-scope atomics; the
+// few cells crossing a block boundary are finished by one workgroup.
+constexpr uint32_t kComBlock = 4096;
 
 // Device buffers of one octree.
 struct TreeBufs {
@@ -65,10 +87,15 @@ struct TreeBufs {
     uint32_t *int_parent = nullptr, *int_node = nullptr, *int_nchild = nullptr,
              *int_arrive = nullptr, *int_count = nullptr;
     InternalSums *int_sums = nullptr;
+    uint8_t *int_local = nullptr;   // 1 if the cell's bodies lie in one kComBlock block
+    uint32_t *int_first = nullptr;  // first sorted position of the cell's bodies
+    uint2 *groups = nullptr;        // traversal groups {first sorted position, body count}
+    uint32_t *spanning = nullptr;   // cells crossing a block boundary
+    uint32_t spanning_cap = 0;
     int32_t *ref = nullptr;
-    double4 *cm = nullptr;
+    double4 *cm = nullptr;          // FP64 records (reference-semantics traversal)
     double *edge = nullptr;
-    float *ropen2 = nullptr;
+    NodeF32 *rec = nullptr;         // FP32 records (group traversal)
     // scan / sort scratch
     uint32_t *scan_part = nullptr;
     uint32_t *radix_hist = nullptr;
@@ -98,8 +125,10 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
                      hipStream_t s);
 void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
                 const double *m, hipStream_t s);
+// f64: FP64 records (cm, edge) for the reference-semantics traversal;
+// f32: packed FP32 records and traversal groups for the group traversal.
 void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
-                         const double *m, double G, double theta, hipStream_t s);
+                         const double *m, double G, bool f64, bool f32, hipStream_t s);
 
 // nb_forces.hip
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
@@ -118,7 +147,7 @@ void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const 
                      uint32_t stack_cap, double *ax, double *ay, double *az,
                      uint32_t *interactions, hipStream_t s);
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     double G, double eps2, double theta, double *ax, double *ay, double *az,
+                     double eps2, double theta, double *ax, double *ay, double *az,
                      uint32_t *interactions, hipStream_t s);
 
 // nb_integrate.hip

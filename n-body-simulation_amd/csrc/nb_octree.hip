@@ -26,9 +26,14 @@
 //     cell; children count themselves into their parent.
 //  7. Centre of mass bottom-up: the last child to arrive at a cell sums the 8
 //     children in digit order 0..7 from 0.0 -- the reference's summation
-//     (BarnesHutOctree.cpp:299-311), so every sum is bit-identical.
+//     (BarnesHutOctree.cpp:299-311), so every sum is bit-identical.  Cells
+//     inside a block of 4096 sorted bodies are finished by that block's
+//     workgroup (workgroup-scope atomics: agent-scope ones would write back
+//     and invalidate the L2 of an XCD at every arrival); the few cells that
+//     cross block boundaries are finished level by level by one workgroup.
 //  8. Node records for traversal (COM = numerator / mass as the reference
-//     divides per visit, edge = rootEdge * 2^-depth, opening radius^2).
+//     divides per visit, edge = rootEdge * 2^-depth; an FP32 hi/lo record for
+//     the group traversal).
 #include <algorithm>
 #include <cmath>
 
@@ -51,6 +56,10 @@ __global__ void __launch_bounds__(kThreads) k_build_begin(TreeHdr *h) {
         h->max_depth = 0;
         h->status = 0;
         h->deep_runs = 0;
+        h->spanning_count = 0;
+        h->stack_peak = 0;
+        h->group_count = 0;
+        h->group_next = 0;
     }
 }
 
@@ -103,16 +112,35 @@ __global__ void __launch_bounds__(kThreads) k_aabb_partial(const double *__restr
 
 // Final reduction + cubify (BarnesHutOctree.cpp:161-190) on one thread per
 // component, then one thread writes the cube.
-__global__ void __launch_bounds__(64) k_aabb_final(const double *part, uint32_t np, TreeHdr *h) {
+__global__ void __launch_bounds__(kThreads) k_aabb_final(const double *part, uint32_t np,
+                                                         TreeHdr *h) {
+    __shared__ double sh[kThreads / 64][6];
     __shared__ double ext[6];
-    if (threadIdx.x < 6) {
-        const bool is_min = threadIdx.x < 3;
-        double v = part[(uint64_t)threadIdx.x * np];
-        for (uint32_t k = 1; k < np; ++k) {
-            const double p = part[(uint64_t)threadIdx.x * np + k];
-            v = is_min ? (p < v ? p : v) : (p > v ? p : v);
+    // every partial already includes the origin, so 0.0 is a neutral start
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = threadIdx.x; k < np; k += kThreads) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = sycl_min(v[c], part[(uint64_t)c * np + k]);
+            v[3 + c] = sycl_max(v[3 + c], part[(uint64_t)(3 + c) * np + k]);
         }
-        ext[threadIdx.x] = v;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = sycl_min(v[c], __shfl_xor(v[c], off, 64));
+            v[3 + c] = sycl_max(v[3 + c], __shfl_xor(v[3 + c], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 6; ++c) sh[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double r = sh[0][threadIdx.x];
+        for (int w = 1; w < kThreads / 64; ++w)
+            r = threadIdx.x < 3 ? sycl_min(r, sh[w][threadIdx.x]) : sycl_max(r, sh[w][threadIdx.x]);
+        ext[threadIdx.x] = r;
     }
     __syncthreads();
     if (threadIdx.x != 0) return;
@@ -316,7 +344,9 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
                                                    const uint32_t *__restrict__ off,
                                                    int32_t *ref, uint8_t *int_depth,
                                                    uint32_t *int_parent, uint32_t *int_node,
-                                                   uint32_t *nchild, uint32_t *body_parent) {
+                                                   uint32_t *nchild, uint32_t *body_parent,
+                                                   uint8_t *int_local, uint32_t *spanning,
+                                                   uint32_t spanning_cap, uint32_t *int_first) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     if (n == 1) {  // the root is a leaf holding the only body
@@ -339,6 +369,17 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
             break;
         }
         int_depth[k] = (uint8_t)l;
+        int_first[k] = i;
+        // The cell's bodies start at i; it crosses into the next COM block iff
+        // that block's first body shares its l-digit prefix with body i.
+        const uint32_t next_block = (i / kComBlock + 1) * kComBlock;
+        const bool local = !(next_block < n && kv.same(next_block, i, l));
+        int_local[k] = local ? 1 : 0;
+        if (!local) {
+            const uint32_t slot = atomicAdd(&h->spanning_count, 1u);
+            if (slot < spanning_cap) spanning[slot] = k;
+            else raise_flag(h, kFlagNodeOverflow);
+        }
         if (l == 0) {
             int_node[k] = 0;
             int_parent[k] = 0xffffffffu;
@@ -372,65 +413,134 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
 }
 
 // ---- 7. centre of mass --------------------------------------------------------
-__device__ inline uint32_t arrive_acq_rel(uint32_t *p) {
-    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__global__ void __launch_bounds__(kThreads) k_com(uint32_t n, uint32_t icap, TreeHdr *h,
-                                                  const double *__restrict__ x,
-                                                  const double *__restrict__ y,
-                                                  const double *__restrict__ z,
-                                                  const double *__restrict__ m,
-                                                  const uint32_t *__restrict__ sidx,
-                                                  const uint32_t *__restrict__ body_parent,
-                                                  const int32_t *__restrict__ ref,
-                                                  const uint32_t *__restrict__ int_parent,
-                                                  const uint32_t *__restrict__ nchild,
-                                                  uint32_t *arrive, InternalSums *sums,
-                                                  uint32_t *counts) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
-    uint32_t pk = body_parent[i];
-    if (pk == 0xffffffffu) return;  // single body (root leaf) or overflow
-    // a leaf has at most kMaxDepth ancestors; the bound only guards against a
-    // corrupt parent chain
-    for (int level = 0; level <= kMaxDepth; ++level) {
-        const uint32_t seen = arrive_acq_rel(&arrive[pk]);
-        if (seen + 1 != nchild[pk]) return;  // a sibling will finish this cell
-        double sx = 0, sy = 0, sz = 0, sm = 0;
-        uint32_t cnt = 0;
-        const uint64_t base = 1ull + 8ull * pk;
-        for (int d = 0; d < 8; ++d) {
-            const int32_t r = ref[base + d];
-            if (r >= 0) {
-                const InternalSums s = sums[r];
-                sx += s.x;
-                sy += s.y;
-                sz += s.z;
-                sm += s.m;
-                cnt += counts[r];
-            } else if (r != kRefEmpty) {
-                const uint32_t b = body_of_ref(r);
-                const double mb = m[b];
-                sx += x[b] * mb;
-                sy += y[b] * mb;
-                sz += z[b] * mb;
-                sm += mb;
-                cnt += 1;
-            }
-        }
-        sums[pk] = InternalSums{sx, sy, sz, sm};
-        counts[pk] = cnt;
-        if (sm == 0) raise_flag(h, kFlagZeroMass);
-        if (pk == 0) return;
-        pk = int_parent[pk];
-        if (pk >= icap) {
-            raise_flag(h, kFlagNodeOverflow);
-            return;
+// Sum of the 8 children of internal cell pk in digit order 0..7 from 0.0
+// (BarnesHutOctree.cpp:299-311).  Empty children add +0.0, which changes no
+// running sum that starts at +0.0, so they are skipped.
+__device__ inline void sum_children(uint32_t pk, const double *__restrict__ x,
+                                    const double *__restrict__ y, const double *__restrict__ z,
+                                    const double *__restrict__ m,
+                                    const int32_t *__restrict__ ref, InternalSums *sums,
+                                    uint32_t *counts, TreeHdr *h) {
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    uint32_t cnt = 0;
+    const uint64_t base = 1ull + 8ull * pk;
+    for (int d = 0; d < 8; ++d) {
+        const int32_t r = ref[base + d];
+        if (r >= 0) {
+            const InternalSums s = sums[r];
+            sx += s.x;
+            sy += s.y;
+            sz += s.z;
+            sm += s.m;
+            cnt += counts[r];
+        } else if (r != kRefEmpty) {
+            const uint32_t b = body_of_ref(r);
+            const double mb = m[b];
+            sx += x[b] * mb;
+            sy += y[b] * mb;
+            sz += z[b] * mb;
+            sm += mb;
+            cnt += 1;
         }
     }
-    raise_flag(h, kFlagNodeOverflow);
+    sums[pk] = InternalSums{sx, sy, sz, sm};
+    counts[pk] = cnt;
+    if (sm == 0) raise_flag(h, kFlagZeroMass);
+}
+
+__device__ inline uint32_t arrive_workgroup(uint32_t *p) {
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Pass 1: one workgroup per kComBlock consecutive (sorted) bodies.  A thread
+// starts at each body's leaf and climbs while the parent cell lies inside the
+// block; the last child to arrive at a cell sums it.  All arrivals at such a
+// cell come from this workgroup, so workgroup-scope atomics suffice (no L2
+// write-back or invalidation across the chip's XCDs).
+constexpr int kComThreads = 1024;
+
+__global__ void __launch_bounds__(kComThreads) k_com_local(
+    uint32_t n, uint32_t icap, TreeHdr *h, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ body_parent,
+    const int32_t *__restrict__ ref, const uint32_t *__restrict__ int_parent,
+    const uint32_t *__restrict__ nchild, const uint8_t *__restrict__ int_local,
+    uint32_t *arrive, InternalSums *sums, uint32_t *counts) {
+    const uint32_t s = blockIdx.x * kComBlock;
+    const uint32_t e = min(n, s + kComBlock);
+    for (uint32_t i = s + threadIdx.x; i < e; i += kComThreads) {
+        if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
+        uint32_t pk = body_parent[i];
+        if (pk == 0xffffffffu) continue;  // single body (root leaf) or overflow
+        // a leaf has at most kMaxDepth ancestors; the bound guards a corrupt chain
+        for (int level = 0; level <= kMaxDepth; ++level) {
+            if (!int_local[pk]) break;  // crosses a block boundary: pass 2
+            const uint32_t seen = arrive_workgroup(&arrive[pk]);
+            if (seen + 1 != nchild[pk]) break;  // a sibling finishes this cell
+            sum_children(pk, x, y, z, m, ref, sums, counts, h);
+            if (pk == 0) break;
+            pk = int_parent[pk];
+            if (pk >= icap) {
+                raise_flag(h, kFlagNodeOverflow);
+                break;
+            }
+        }
+    }
+}
+
+// Pass 2: one workgroup finishes the cells that cross block boundaries,
+// deepest level first, with a barrier between levels.
+__global__ void __launch_bounds__(kComThreads) k_com_spanning(
+    uint32_t spanning_cap, TreeHdr *h, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const int32_t *__restrict__ ref, const uint8_t *__restrict__ int_depth,
+    const uint32_t *__restrict__ spanning, InternalSums *sums, uint32_t *counts) {
+    const uint32_t M = min(h->spanning_count, spanning_cap);
+    if (M == 0) return;
+    __shared__ uint32_t deepest;
+    if (threadIdx.x == 0) deepest = 0;
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < M; j += kComThreads)
+        atomicMax(&deepest, (uint32_t)int_depth[spanning[j]]);
+    __syncthreads();
+    for (int level = (int)deepest; level >= 0; --level) {
+        for (uint32_t j = threadIdx.x; j < M; j += kComThreads) {
+            const uint32_t k = spanning[j];
+            if (int_depth[k] == level) sum_children(k, x, y, z, m, ref, sums, counts, h);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- traversal groups -------------------------------------------------------------
+// A group is a cell with at most kGroupMax bodies whose parent holds more, or
+// a lone body whose parent holds more.  Groups partition the bodies into
+// spatially compact runs of the octree order.
+__global__ void __launch_bounds__(kThreads) k_groups(uint32_t n, uint32_t icap, TreeHdr *h,
+                                                     const uint32_t *__restrict__ counts,
+                                                     const uint32_t *__restrict__ int_parent,
+                                                     const uint32_t *__restrict__ int_first,
+                                                     const uint32_t *__restrict__ body_parent,
+                                                     uint2 *groups) {
+    const uint32_t I = min(h->internal_count, icap);
+    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+    const uint64_t t0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    for (uint64_t k = t0; k < I; k += stride) {
+        const uint32_t c = counts[k];
+        const uint32_t pc = k == 0 ? 0xffffffffu : counts[int_parent[k]];
+        if (c <= kGroupMax && pc > kGroupMax) {
+            const uint32_t slot = atomicAdd(&h->group_count, 1u);
+            if (slot < n) groups[slot] = make_uint2(int_first[k], c);
+        }
+    }
+    for (uint64_t i = t0; i < n; i += stride) {
+        const uint32_t pk = body_parent[i];
+        const bool lone = pk == 0xffffffffu ? (n == 1) : counts[pk] > kGroupMax;
+        if (lone) {
+            const uint32_t slot = atomicAdd(&h->group_count, 1u);
+            if (slot < n) groups[slot] = make_uint2((uint32_t)i, 1u);
+        }
+    }
 }
 
 // ---- 8. node records -----------------------------------------------------------
@@ -443,8 +553,8 @@ __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict_
                                                       const int32_t *__restrict__ ref,
                                                       const InternalSums *__restrict__ sums,
                                                       const uint8_t *__restrict__ int_depth,
-                                                      double theta, double4 *cm, double *edge,
-                                                      float *ropen2) {
+                                                      double G, int write_f64, int write_f32,
+                                                      double4 *cm, double *edge, NodeF32 *rec) {
     const uint32_t I = min(h->internal_count, icap);
     const uint64_t nodes = 1 + 8ull * I;
     const double root_edge = h->edge;
@@ -453,21 +563,33 @@ __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict_
         const int32_t r = ref[j];
         double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
         double e = 0.0;
-        float ro2 = 0.0f;
         if (r >= 0) {
+            // the reference divides numerator by mass at every visit
+            // (BarnesHutAlgorithm.cpp:351-353); the quotient is the same
             const InternalSums s = sums[r];
             c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
             e = ldexp(root_edge, -(int)int_depth[r]);
-            const double ro = e / theta;
-            ro2 = (float)(ro * ro);
         } else if (r != kRefEmpty) {
             const uint32_t b = body_of_ref(r);
             const double mb = m[b];
             c = make_double4((x[b] * mb) / mb, (y[b] * mb) / mb, (z[b] * mb) / mb, mb);
         }
-        cm[j] = c;
-        edge[j] = e;
-        ropen2[j] = ro2;
+        if (write_f64) {
+            cm[j] = c;
+            edge[j] = e;
+        }
+        if (write_f32) {
+            NodeF32 o;
+            o.x = (float)c.x;
+            o.y = (float)c.y;
+            o.z = (float)c.z;
+            o.gm = (float)(G * c.w);
+            o.ref = r;
+            o.lx = (float)(c.x - (double)o.x);
+            o.ly = (float)(c.y - (double)o.y);
+            o.lz = (float)(c.z - (double)o.z);
+            rec[j] = o;
+        }
     }
 }
 
@@ -479,7 +601,7 @@ void launch_aabb(const TreeBufs &t, const double *x, const double *y, const doub
     const uint32_t nb = std::min<uint32_t>(kAabbBlocks, ceil_div(t.n, 4 * kThreads));
     hipLaunchKernelGGL(k_aabb_partial, dim3(nb), dim3(kThreads), 0, s, x, y, z, t.n, t.aabb_part,
                        t.hdr);
-    hipLaunchKernelGGL(k_aabb_final, dim3(1), dim3(64), 0, s, (const double *)t.aabb_part, nb,
+    hipLaunchKernelGGL(k_aabb_final, dim3(1), dim3(kThreads), 0, s, (const double *)t.aabb_part, nb,
                        t.hdr);
 }
 
@@ -510,25 +632,36 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
     hipLaunchKernelGGL(k_link, grid, dim3(kThreads), 0, s, n, t.icap, t.hdr, kv,
                        (const uint32_t *)t.sidx, (const int32_t *)t.lcp,
                        (const uint32_t *)t.chain, t.ref, t.int_depth, t.int_parent, t.int_node,
-                       t.int_nchild, t.body_parent);
+                       t.int_nchild, t.body_parent, t.int_local, t.spanning, t.spanning_cap,
+                       t.int_first);
 }
 
 void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
                 const double *m, hipStream_t s) {
-    hipLaunchKernelGGL(k_com, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, t.n, t.icap, t.hdr, x,
-                       y, z, m, (const uint32_t *)t.sidx, (const uint32_t *)t.body_parent,
-                       (const int32_t *)t.ref, (const uint32_t *)t.int_parent,
-                       (const uint32_t *)t.int_nchild, t.int_arrive, t.int_sums, t.int_count);
+    hipLaunchKernelGGL(k_com_local, dim3(ceil_div(t.n, kComBlock)), dim3(kComThreads), 0, s, t.n,
+                       t.icap, t.hdr, x, y, z, m, (const uint32_t *)t.sidx,
+                       (const uint32_t *)t.body_parent, (const int32_t *)t.ref,
+                       (const uint32_t *)t.int_parent, (const uint32_t *)t.int_nchild,
+                       (const uint8_t *)t.int_local, t.int_arrive, t.int_sums, t.int_count);
+    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, t.spanning_cap, t.hdr, x,
+                       y, z, m, (const int32_t *)t.ref, (const uint8_t *)t.int_depth,
+                       (const uint32_t *)t.spanning, t.int_sums, t.int_count);
 }
 
 void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
-                         const double *m, double G, double theta, hipStream_t s) {
-    (void)G;
+                         const double *m, double G, bool f64, bool f32, hipStream_t s) {
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(8192, ceil_div(8ull * t.n, kThreads)));
     hipLaunchKernelGGL(k_records, dim3(blocks), dim3(kThreads), 0, s, (const TreeHdr *)t.hdr,
                        t.icap, x, y, z, m, (const int32_t *)t.ref,
-                       (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, theta,
-                       t.cm, t.edge, t.ropen2);
+                       (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, G,
+                       (int)f64, (int)f32, t.cm, t.edge, t.rec);
+    if (f32) {
+        const uint32_t gb = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(t.n, kThreads)));
+        hipLaunchKernelGGL(k_groups, dim3(gb), dim3(kThreads), 0, s, t.n, t.icap, t.hdr,
+                           (const uint32_t *)t.int_count, (const uint32_t *)t.int_parent,
+                           (const uint32_t *)t.int_first, (const uint32_t *)t.body_parent,
+                           t.groups);
+    }
 }
 
 }  // namespace nb
