@@ -124,6 +124,13 @@ int nb_kick(nb_ctx *ctx, double dt);
  * end.  algorithm = NB_ALGO_*. */
 int nb_steps(nb_ctx *ctx, int algorithm, double dt, uint32_t count);
 
+/* System energies at the current state (nBodyAlgorithm::computeEnergy,
+ * nBodyAlgorithm.cpp:11-86): kinetic sum of 0.5*m*v^2, potential
+ * -sum_{i<j} G m_i m_j / r without softening, per-body terms summed in body
+ * order on the host as the reference does.  Any output pointer may be NULL. */
+int nb_energy(nb_ctx *ctx, double *kinetic, double *potential, double *total,
+              double *virial);
+
 /* ---- timing (TimeMeasurement keys, BarnesHutAlgorithm.cpp:81-100) --------- */
 enum {
     NB_PHASE_AABB = 0,        /* "AABB creation" */

@@ -37,6 +37,7 @@ struct nb_ctx {
     uint32_t *targets_dev = nullptr;
     double *targets_out = nullptr;
     uint32_t targets_cap = 0;
+    double *energy_terms = nullptr;
     std::vector<hipEvent_t> events;
     std::vector<std::pair<int, size_t>> marks;  // (phase, index of start event)
     double phase[NB_PHASE_COUNT] = {0};
@@ -234,8 +235,8 @@ int bh_forces_enqueue(nb_ctx *c) {
                             c->p.sort_bodies, c->exact_stack, c->exact_cap, c->ax, c->ay, c->az,
                             c->inter, c->stream);
     } else {
-        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->p.eps2, c->p.theta, c->ax, c->ay,
-                            c->az, c->inter, c->stream);
+        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta, c->ax,
+                            c->ay, c->az, c->inter, c->stream);
     }
     mark_end(c, NB_PHASE_ACCEL);
     return NB_OK;
@@ -548,6 +549,34 @@ int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
     int rc = finish(c, bh);
     c->tree_valid = false;  // positions moved after the last build
     return rc;
+}
+
+int nb_energy(nb_ctx *c, double *kinetic, double *potential, double *total, double *virial) {
+    if (!c) return NB_E_INVALID_ARG;
+    if (!c->energy_terms) {
+        int rc = dev_alloc(c, &c->energy_terms, 2ull * c->n);
+        if (rc) return rc;
+    }
+    const uint32_t n = c->n;
+    double *ekin = c->energy_terms, *epot = c->energy_terms + n;
+    nb::launch_energy_terms(n, c->x, c->y, c->z, c->vx, c->vy, c->vz, c->m, c->p.G, ekin, epot,
+                            c->stream);
+    std::vector<double> hk(n), hp(n);
+    NB_HIP(c, hipMemcpyAsync(hk.data(), ekin, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipMemcpyAsync(hp.data(), epot, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    // nBodyAlgorithm.cpp:71-85
+    double E_kin_result = 0, E_pot_result = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        E_kin_result += hk[i];
+        E_pot_result += hp[i];
+    }
+    E_pot_result *= -1;
+    if (kinetic) *kinetic = E_kin_result;
+    if (potential) *potential = E_pot_result;
+    if (total) *total = E_kin_result + E_pot_result;
+    if (virial) *virial = (2.0 * E_kin_result) / std::abs(E_pot_result);
+    return NB_OK;
 }
 
 int nb_phase_ms(const nb_ctx *c, double *ms) {

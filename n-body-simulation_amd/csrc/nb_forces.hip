@@ -296,7 +296,7 @@ __device__ inline uint64_t lanes_below() {
 
 __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
     float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
     TreeHdr *h) {
@@ -351,7 +351,26 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
         const float gxh = (float)cx, gyh = (float)cy, gzh = (float)cz;
         const float gxl = (float)(cx - (double)gxh), gyl = (float)(cy - (double)gyh),
                     gzl = (float)(cz - (double)gzh);
-        const float rx = (float)(px - cx), ry = (float)(py - cy), rz = (float)(pz - cz);
+        // The body's own offset is computed with exactly the operations that
+        // produce its leaf's list entry (record hi/lo of (x*m)/m, k_records),
+        // so the self pair has a distance of exactly 0 and adds nothing; with
+        // eps^2 = 1e-22 any rounding residual would be a huge spurious force.
+        float rx, ry, rz;
+        {
+            const double mb = m[bi];
+            const double c3[3] = {(px * mb) / mb, (py * mb) / mb, (pz * mb) / mb};
+            const float gh[3] = {gxh, gyh, gzh}, gl[3] = {gxl, gyl, gzl};
+            float r3[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float hi = (float)c3[c];
+                const float lo = (float)(c3[c] - (double)hi);
+                r3[c] = (hi - gh[c]) + (lo - gl[c]);
+            }
+            rx = r3[0];
+            ry = r3[1];
+            rz = r3[2];
+        }
 
         double acc_x = 0, acc_y = 0, acc_z = 0;
         uint32_t n_inter = 0;
@@ -519,11 +538,11 @@ void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const 
 }
 
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     double eps2, double theta, double *ax, double *ay, double *az,
-                     uint32_t *interactions, hipStream_t s) {
+                     const double *m, double eps2, double theta, double *ax, double *ay,
+                     double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
     hipLaunchKernelGGL(k_bh_group, dim3(blocks), dim3(kGroup), 0, s, t.n,
-                       (const uint32_t *)t.sidx, x, y, z, (const NodeF32 *)t.rec,
+                       (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
                        interactions, t.hdr);
 }

@@ -147,8 +147,13 @@ void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const 
                      uint32_t stack_cap, double *ax, double *ay, double *az,
                      uint32_t *interactions, hipStream_t s);
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     double eps2, double theta, double *ax, double *ay, double *az,
-                     uint32_t *interactions, hipStream_t s);
+                     const double *m, double eps2, double theta, double *ax, double *ay,
+                     double *az, uint32_t *interactions, hipStream_t s);
+
+// per-body kinetic and potential energy terms (reference arithmetic)
+void launch_energy_terms(uint32_t n, const double *x, const double *y, const double *z,
+                         const double *vx, const double *vy, const double *vz, const double *m,
+                         double G, double *ekin, double *epot, hipStream_t s);
 
 // nb_integrate.hip
 void launch_kick_drift(uint32_t n, double dt, const double *ax, const double *ay,

@@ -1,0 +1,376 @@
+#include "nBodyAlgorithm.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <ctime>
+#include <filesystem>
+#include <iostream>
+#include <stdexcept>
+
+void nbCheck(nb_ctx *ctx, int rc, const char *what) {
+    if (rc == NB_OK) return;
+    std::string msg = std::string(what) + " failed: " + nb_strerror(rc);
+    if (ctx) msg += std::string(" (") + nb_last_error(ctx) + ")";
+    throw std::runtime_error(msg);
+}
+
+nBodyAlgorithm::nBodyAlgorithm(double dt, double t_end, double visualizationStepWidth,
+                               std::string &outputDirectory)
+    : outputDirectory(outputDirectory), dt(dt), t_end(t_end),
+      visualizationStepWidth(visualizationStepWidth) {
+    G = 6.67428 * std::pow(10, -11);
+    const double meter_AU = 1.0 / (1.49597870691 * std::pow(10, 11));
+    const double second_Days = 1.0 / 86400;
+    G = G * (std::pow(meter_AU, 3) / std::pow(second_Days, 2));
+}
+
+nBodyAlgorithm::~nBodyAlgorithm() = default;
+
+namespace {
+
+double phase(nb_ctx *ctx, int p) {
+    double ms[NB_PHASE_COUNT];
+    nbCheck(ctx, nb_phase_ms(ctx, ms), "nb_phase_ms");
+    return ms[p];
+}
+
+struct CtxGuard {
+    nb_ctx *ctx = nullptr;
+    ~CtxGuard() {
+        if (ctx) nb_destroy(ctx);
+    }
+};
+
+}  // namespace
+
+void nBodyAlgorithm::recordStepTimes(nb_ctx *ctx, int algorithm, bool loop) {
+    double ms[NB_PHASE_COUNT];
+    nbCheck(ctx, nb_phase_ms(ctx, ms), "nb_phase_ms");
+    if (loop) {
+        timer.addTimeToSequence("Leapfrog Part 1", ms[NB_PHASE_LF1]);
+        timer.addTimeToSequence("Leapfrog Part 2", ms[NB_PHASE_LF2]);
+    }
+    if (algorithm == NB_ALGO_BARNES_HUT) {
+        timer.addTimeToSequence("Octree creation", ms[NB_PHASE_OCTREE]);
+        timer.addTimeToSequence("AABB creation", ms[NB_PHASE_AABB]);
+        timer.addTimeToSequence("Build octree", ms[NB_PHASE_KEYS_SORT] + ms[NB_PHASE_TOPOLOGY]);
+        timer.addTimeToSequence("Compute center of mass", ms[NB_PHASE_COM]);
+        timer.addTimeToSequence("Total Time", ms[NB_PHASE_OCTREE] + ms[NB_PHASE_ACCEL]);
+    }
+    timer.addTimeToSequence("Acceleration Kernel Time", ms[NB_PHASE_ACCEL]);
+}
+
+void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int algorithm) {
+    const d_type::int_t N = configuration::numberOfBodies;
+    if (!configuration::use_GPUs)
+        throw std::invalid_argument(
+            "use_gpus=false: this implementation runs the simulation on the GPU only");
+
+    positions_x[0] = simulationData.positions_x;
+    positions_y[0] = simulationData.positions_y;
+    positions_z[0] = simulationData.positions_z;
+    velocities_x[0] = simulationData.velocities_x;
+    velocities_y[0] = simulationData.velocities_y;
+    velocities_z[0] = simulationData.velocities_z;
+    adjustVelocities(simulationData);
+
+    nb_params p;
+    nb_default_params(&p);
+    p.G = G;
+    p.eps2 = configuration::epsilon2;
+    p.theta = configuration::barnes_hut_algorithm::theta;
+    p.precision = (uint32_t)configuration::precision;
+    p.sort_bodies = configuration::barnes_hut_algorithm::sortBodies ? 1 : 0;
+    p.storage_size_param = (uint32_t)configuration::barnes_hut_algorithm::storageSizeParam;
+    p.stack_size_param = (uint32_t)configuration::barnes_hut_algorithm::stackSizeParam;
+    CtxGuard guard;
+    nbCheck(nullptr, nb_create(&guard.ctx, N, &p), "nb_create");
+    nb_ctx *ctx = guard.ctx;
+    nbCheck(ctx,
+            nb_set_state(ctx, simulationData.positions_x.data(), simulationData.positions_y.data(),
+                         simulationData.positions_z.data(), simulationData.velocities_x.data(),
+                         simulationData.velocities_y.data(), simulationData.velocities_z.data(),
+                         simulationData.mass.data()),
+            "nb_set_state");
+
+    char devname[256] = "unknown";
+    nb_device_name(devname, sizeof devname);
+    std::string device = devname;
+    d_type::int_t bodyCount = N;
+    timer.setProperties(description, bodyCount, device);
+    timer.addTimingSequence("Acceleration Kernel Time");
+    timer.addTimingSequence("Leapfrog Part 1");
+    timer.addTimingSequence("Leapfrog Part 2");
+    if (algorithm == NB_ALGO_BARNES_HUT) {
+        timer.addTimingSequence("Total Time");
+        timer.addTimingSequence("Octree creation");
+        timer.addTimingSequence("AABB creation");
+        timer.addTimingSequence("Compute center of mass");
+        timer.addTimingSequence("Build octree");
+    }
+
+    std::vector<double> ax(N), ay(N), az(N);
+    double time = 0.0;
+    double timeSinceLastVisualization = 0.0;
+    d_type::int_t currentStep = 0;
+
+    // initial accelerations (NaiveAlgorithm.cpp:86-113, BarnesHutAlgorithm.cpp:105-133)
+    if (algorithm == NB_ALGO_BARNES_HUT) {
+        nbCheck(ctx, nb_bh_build(ctx), "nb_bh_build");
+        const double octree = phase(ctx, NB_PHASE_OCTREE);
+        timer.addTimeToSequence("Octree creation", octree);
+        timer.addTimeToSequence("AABB creation", phase(ctx, NB_PHASE_AABB));
+        timer.addTimeToSequence("Build octree",
+                                phase(ctx, NB_PHASE_KEYS_SORT) + phase(ctx, NB_PHASE_TOPOLOGY));
+        timer.addTimeToSequence("Compute center of mass", phase(ctx, NB_PHASE_COM));
+        nbCheck(ctx, nb_bh_forces(ctx), "nb_bh_forces");
+        const double accel = phase(ctx, NB_PHASE_ACCEL);
+        timer.addTimeToSequence("Acceleration Kernel Time", accel);
+        timer.addTimeToSequence("Total Time", octree + accel);
+        std::cout << std::endl;
+    } else {
+        nbCheck(ctx, nb_forces_direct(ctx), "nb_forces_direct");
+        timer.addTimeToSequence("Acceleration Kernel Time", phase(ctx, NB_PHASE_ACCEL));
+    }
+    if (configuration::compute_energy) computeEnergy(ctx, currentStep);
+    nbCheck(ctx, nb_get_accel(ctx, ax.data(), ay.data(), az.data()), "nb_get_accel");
+    storeAccelerations(currentStep, ax, ay, az);
+    std::cout << "Finished initial step " << currentStep << std::endl << std::endl;
+
+    time += dt;
+    timeSinceLastVisualization += dt;
+    currentStep += 1;
+
+    std::vector<double> px(N), py(N), pz(N), vx(N), vy(N), vz(N);
+    while (time <= t_end + 0.000001) {
+        const bool visualizeCurrentStep =
+            (std::abs(timeSinceLastVisualization - visualizationStepWidth) < 0.000001);
+        // kick-drift, [octree], forces, kick -- one synchronisation per step
+        nbCheck(ctx, nb_steps(ctx, algorithm, dt, 1), "nb_steps");
+        recordStepTimes(ctx, algorithm, true);
+        if (visualizeCurrentStep) {
+            std::cout << "Finished step " << currentStep << std::endl << std::endl;
+            nbCheck(ctx, nb_get_state(ctx, px.data(), py.data(), pz.data(), vx.data(), vy.data(), vz.data()),
+                    "nb_get_state");
+            positions_x[currentStep] = px;
+            positions_y[currentStep] = py;
+            positions_z[currentStep] = pz;
+            nbCheck(ctx, nb_get_accel(ctx, ax.data(), ay.data(), az.data()), "nb_get_accel");
+            storeAccelerations(currentStep, ax, ay, az);
+            velocities_x[currentStep] = vx;
+            velocities_y[currentStep] = vy;
+            velocities_z[currentStep] = vz;
+            if (configuration::compute_energy) computeEnergy(ctx, currentStep);
+            currentStep += 1;
+            timeSinceLastVisualization = 0.0;
+        }
+        time += dt;
+        timeSinceLastVisualization += dt;
+    }
+}
+
+void nBodyAlgorithm::computeEnergy(nb_ctx *ctx, d_type::int_t currentStep) {
+    double k, u, e, v;
+    nbCheck(ctx, nb_energy(ctx, &k, &u, &e, &v), "nb_energy");
+    kineticEnergy[currentStep] = k;
+    potentialEnergy[currentStep] = u;
+    totalEnergy[currentStep] = e;
+    virialEquilibrium[currentStep] = v;
+}
+
+void nBodyAlgorithm::storeAccelerations(d_type::int_t currentStep, const std::vector<double> &ax,
+                                        const std::vector<double> &ay,
+                                        const std::vector<double> &az) {
+    std::vector<double> &out = acceleration[currentStep];
+    out.resize(ax.size());
+    for (size_t i = 0; i < ax.size(); ++i)
+        out[i] = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+}
+
+void nBodyAlgorithm::adjustVelocities(const SimulationData &simulationData) {
+    double sumMasses = 0, smx = 0, smy = 0, smz = 0;
+    for (d_type::int_t i = 0; i < configuration::numberOfBodies; ++i) {
+        sumMasses += simulationData.mass[i];
+        smx += simulationData.mass[i] * simulationData.velocities_x[i];
+        smy += simulationData.mass[i] * simulationData.velocities_y[i];
+        smz += simulationData.mass[i] * simulationData.velocities_z[i];
+    }
+    const double ui_x = smx / sumMasses, ui_y = smy / sumMasses, ui_z = smz / sumMasses;
+    for (d_type::int_t i = 0; i < configuration::numberOfBodies; ++i) {
+        velocities_x[0][i] -= ui_x;
+        velocities_y[0][i] -= ui_y;
+        velocities_z[0][i] -= ui_z;
+    }
+}
+
+void nBodyAlgorithm::generateParaViewOutput(const SimulationData &simulationData) {
+    std::time_t currentTime = std::time(nullptr);
+    std::string time = static_cast<std::string>(std::ctime(&currentTime));
+    std::replace(time.begin(), time.end(), ' ', '_');
+    time = time.substr(0, time.size() - 1);
+    const std::string filePathBase = outputDirectory + '/' + time + '/';
+    std::filesystem::create_directories(filePathBase);
+
+    timer.exportJSON(filePathBase + "times.json");
+    outputLastState(filePathBase + "lastState.csv");
+
+    std::ofstream pvdFile(filePathBase + "/simulation" + ".pvd");
+    pvdFile << "<?xml version=\"1.0\"?>" << '\n'
+            << R"(<VTKFile type="Collection" version="0.1" byte_order="LittleEndian" compressor="vtkZLibDataCompressor">)"
+            << '\n'
+            << "<Collection>" << '\n';
+    for (d_type::int_t i = 0; i < positions_x.size(); ++i) {
+        const std::string vtpFileName = "simulation_step" + std::to_string(i) + ".vtp";
+        std::ofstream vtpFile(filePathBase + vtpFileName);
+        pvdFile << "<DataSet timestep=\"" << i << "\" " << R"(group="" part="0" file=")"
+                << vtpFileName << "\"/>" << '\n';
+        writeFileHeader(i, vtpFile);
+        writePositions(i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << "</Points>" << '\n'
+                << "<PointData>" << '\n'
+                << R"(<DataArray type="Int32" Name="body_id" NumberOfComponents="1" format="ascii">)"
+                << '\n';
+        writeIDs(i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="velocity" NumberOfComponents="3" format="ascii">)"
+                << '\n';
+        writeVelocities(i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="acceleration" NumberOfComponents="1" format="ascii">)"
+                << '\n';
+        writeAccelerations(i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="mass" NumberOfComponents="1" format="ascii">)"
+                << '\n';
+        writeMasses(simulationData, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="String" Name="name" NumberOfComponents="1" format="ascii">)"
+                << '\n';
+        writeNamesASCII(simulationData, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Int32" Name="orbit_class" NumberOfComponents="1" format="ascii">)"
+                << '\n';
+        writeBodyClasses(simulationData, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << "</PointData>" << '\n'
+                << "<Verts>" << '\n'
+                << R"(<DataArray type="Int64" Name="offsets">)" << '\n';
+        writeOffsets(i, vtpFile);
+        vtpFile << '\n'
+                << "</DataArray>" << '\n'
+                << R"(<DataArray type="Int64" Name="connectivity">)" << '\n';
+        writeConnectivity(i, vtpFile);
+        vtpFile << '\n'
+                << "</DataArray>" << '\n'
+                << "</Verts>" << '\n'
+                << "</Piece>" << '\n'
+                << "<FieldData>" << '\n'
+                << R"(<DataArray type="Float64" Name="kinetic energy" NumberOfTuples="1" format="ascii">)"
+                << '\n';
+        writeFieldValue(kineticEnergy, i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="potential energy" NumberOfTuples="1" format="ascii">)"
+                << '\n';
+        writeFieldValue(potentialEnergy, i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="total energy" NumberOfTuples="1" format="ascii">)"
+                << '\n';
+        writeFieldValue(totalEnergy, i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << R"(<DataArray type="Float64" Name="virial equilibrium" NumberOfTuples="1" format="ascii">)"
+                << '\n';
+        writeFieldValue(virialEquilibrium, i, vtpFile);
+        vtpFile << "</DataArray>" << '\n'
+                << "</FieldData>" << '\n'
+                << "</PolyData>" << '\n'
+                << "</VTKFile>" << '\n';
+    }
+    pvdFile << "</Collection>" << '\n' << "</VTKFile>" << '\n';
+}
+
+void nBodyAlgorithm::writeFieldValue(const std::map<d_type::int_t, double> &values, size_t i,
+                                     std::ofstream &vtpFile) {
+    if (configuration::compute_energy) {
+        auto it = values.find((d_type::int_t)i);
+        vtpFile << (it != values.end() ? it->second : 0.0) << '\n';
+    } else {
+        vtpFile << 0 << '\n';
+    }
+}
+
+void nBodyAlgorithm::writeConnectivity(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << std::to_string(j) << ' ';
+}
+
+void nBodyAlgorithm::writeOffsets(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 1; j <= positions_x[i].size(); ++j) vtpFile << std::to_string(j) << ' ';
+}
+
+void nBodyAlgorithm::writeBodyClasses(const SimulationData &simulationData,
+                                      std::ofstream &vtpFile) {
+    // object classes of https://pdssbn.astro.umd.edu/data_other/objclass.shtml
+    // plus stars, dwarf planets, planets and moons (nBodyAlgorithm.cpp:298-342)
+    static const std::map<std::string, int> code = {
+        {"AMO", 1}, {"APO", 2},  {"ATE", 3},  {"IEO", 4},  {"MCA", 5},  {"IMB", 6},
+        {"MBA", 7}, {"OMB", 8},  {"CEN", 9},  {"TJN", 10}, {"TNO", 11}, {"AST", 12},
+        {"PAA", 13}, {"HYA", 14}, {"STA", 15}, {"DWA", 16}, {"PLA", 17}, {"SAT", 18}};
+    for (const std::string &bodyClass : simulationData.body_classes) {
+        auto it = code.find(bodyClass);
+        vtpFile << (it != code.end() ? it->second : 0) << '\n';
+    }
+}
+
+void nBodyAlgorithm::writeNamesASCII(const SimulationData &simulationData,
+                                     std::ofstream &vtpFile) const {
+    for (const std::string &name : simulationData.names) {
+        for (char c : name) vtpFile << (int)c << ' ';
+        vtpFile << " 0" << '\n';
+    }
+}
+
+void nBodyAlgorithm::writeMasses(const SimulationData &simulationData,
+                                 std::ofstream &vtpFile) const {
+    for (double m : simulationData.mass) vtpFile << m << '\n';
+}
+
+void nBodyAlgorithm::writeAccelerations(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << acceleration[i][j] << '\n';
+}
+
+void nBodyAlgorithm::writeVelocities(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 0; j < velocities_x[i].size(); ++j)
+        vtpFile << velocities_x[i].at(j) << " " << velocities_y[i].at(j) << " "
+                << velocities_z[i].at(j) << '\n';
+}
+
+void nBodyAlgorithm::writeIDs(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << j << '\n';
+}
+
+void nBodyAlgorithm::writePositions(size_t i, std::ofstream &vtpFile) {
+    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j)
+        vtpFile << positions_x[i].at(j) << " " << positions_y[i].at(j) << " "
+                << positions_z[i].at(j) << '\n';
+}
+
+void nBodyAlgorithm::writeFileHeader(size_t i, std::ofstream &vtpFile) {
+    vtpFile << "<?xml version=\"1.0\"?>" << '\n'
+            << R"(<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian" header_type="UInt64">)"
+            << '\n'
+            << "<PolyData>" << '\n'
+            << "<Piece NumberOfPoints=\"" << positions_x[i].size() << "\" NumberOfVerts=\""
+            << positions_x[i].size() << "\">" << '\n'
+            << "<Points>" << '\n'
+            << R"(<DataArray type="Float64" Name="position" NumberOfComponents="3" format="ascii">)"
+            << '\n';
+}
+
+void nBodyAlgorithm::outputLastState(const std::string &path) {
+    std::ofstream csvFile(path);
+    const d_type::int_t i = (d_type::int_t)positions_x.size() - 1;
+    csvFile << "position_x, position_y, position_z \n";
+    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j)
+        csvFile << positions_x[i].at(j) << "," << positions_y[i].at(j) << ","
+                << positions_z[i].at(j) << '\n';
+}

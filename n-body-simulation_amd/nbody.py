@@ -38,7 +38,7 @@ EXPORTS = [
     "nb_get_state", "nb_get_accel", "nb_set_positions", "nb_forces_direct",
     "nb_forces_direct_targets", "nb_bh_build", "nb_bh_forces", "nb_kick_drift", "nb_kick",
     "nb_steps", "nb_phase_ms", "nb_tree_info", "nb_tree_sorted", "nb_tree_canonical",
-    "nb_bh_interactions", "nb_gen_star_disk", "nb_gen_plummer",
+    "nb_bh_interactions", "nb_gen_star_disk", "nb_gen_plummer", "nb_energy",
 ]
 
 
@@ -97,6 +97,7 @@ def load(path=LIB_PATH):
         "nb_tree_canonical": (ctypes.c_uint64, [vp, ctypes.c_uint64, _u8p, _u8p, _u32p]
                               + [_dp] * 4 + [_u32p]),
         "nb_bh_interactions": (ctypes.c_int, [vp, _u32p]),
+        "nb_energy": (ctypes.c_int, [vp] + [_dp] * 4),
         "nb_gen_star_disk": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
         "nb_gen_plummer": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
     }
@@ -249,6 +250,11 @@ class Context:
         out = np.zeros(len(PHASES))
         self._check(self.lib.nb_phase_ms(self.h, _d(out)), "nb_phase_ms")
         return dict(zip(PHASES, out))
+
+    def energy(self):
+        out = [ctypes.c_double() for _ in range(4)]
+        self._check(self.lib.nb_energy(self.h, *[ctypes.byref(o) for o in out]), "nb_energy")
+        return dict(zip(["kinetic", "potential", "total", "virial"], [o.value for o in out]))
 
     def tree_info(self):
         nodes = ctypes.c_uint64()

@@ -165,6 +165,38 @@ void orc_kick(uint32_t n, double dt, const double *ax, const double *ay, const d
 }
 
 // ---------------------------------------------------------------------------
+// energy: nBodyAlgorithm.cpp:48-85 (per-body terms in parallel, host sums in
+// body order)
+void orc_energy(uint32_t n, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *m,
+                double G, double out[4]) {
+    std::vector<double> E_KIN(n), E_POT(n);
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads())
+    for (int64_t j = 0; j < (int64_t)n; ++j) {
+        double v = vx[j] * vx[j] + vy[j] * vy[j] + vz[j] * vz[j];
+        E_KIN[j] = 0.5 * m[j] * v;
+        E_POT[j] = 0;
+        for (int64_t i = 0; i < j; ++i) {
+            double r_x = x[j] - x[i];
+            double r_y = y[j] - y[i];
+            double r_z = z[j] - z[i];
+            double r = std::sqrt(r_x * r_x + r_y * r_y + r_z * r_z);
+            E_POT[j] += G * m[i] * m[j] / r;
+        }
+    }
+    double E_kin_result = 0, E_pot_result = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        E_kin_result += E_KIN[i];
+        E_pot_result += E_POT[i];
+    }
+    E_pot_result *= -1;
+    out[0] = E_kin_result;
+    out[1] = E_pot_result;
+    out[2] = E_kin_result + E_pot_result;
+    out[3] = (2.0 * E_kin_result) / std::abs(E_pot_result);
+}
+
+// ---------------------------------------------------------------------------
 // AABB: BarnesHutOctree.cpp:45-191.  Partial extrema start at 0.0 (the
 // vectors at :58-72 are value-initialised), so the box always contains the
 // origin; then the box is made a cube by growing the shorter edges

@@ -69,6 +69,12 @@ void orc_kick(uint32_t n, double dt,
               const double *vhx, const double *vhy, const double *vhz,
               double *vx, double *vy, double *vz);
 
+/* ---- energy (nBodyAlgorithm.cpp:11-86) ----------------------------------- */
+/* out = {kinetic, potential, total, virial 2K/|U|}; no softening. */
+void orc_energy(uint32_t n, const double *x, const double *y, const double *z,
+                const double *vx, const double *vy, const double *vz, const double *m,
+                double G, double out[4]);
+
 /* ---- AABB (BarnesHutOctree.cpp:45-191) ---------------------------------- */
 /* out = {min_x, min_y, min_z, max_x, max_y, max_z, edge} after cubify. */
 int orc_aabb(uint32_t n, int work_items, const double *x, const double *y, const double *z,

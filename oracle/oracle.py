@@ -46,6 +46,7 @@ def _load():
                                             + [ctypes.c_uint32, _u32p] + [_dp] * 3)
     lib.orc_kick_drift.argtypes = [ctypes.c_uint32, ctypes.c_double] + [_dp] * 12
     lib.orc_kick.argtypes = [ctypes.c_uint32, ctypes.c_double] + [_dp] * 9
+    lib.orc_energy.argtypes = [ctypes.c_uint32] + [_dp] * 7 + [ctypes.c_double, _dp]
     lib.orc_aabb.argtypes = [ctypes.c_uint32, ctypes.c_int] + [_dp] * 4
     lib.orc_aabb.restype = ctypes.c_int
     lib.orc_tree_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
@@ -146,6 +147,14 @@ def kick(dt, a, vh):
     v = [np.empty(n) for _ in range(3)]
     lib().orc_kick(n, dt, *map(_d, a), *map(_d, vh), *map(_d, v))
     return v
+
+
+def energy(x, y, z, vx, vy, vz, m, G=None):
+    """{kinetic, potential, total, virial} as nBodyAlgorithm::computeEnergy."""
+    arrs = [_c64(a) for a in (x, y, z, vx, vy, vz, m)]
+    out = np.zeros(4)
+    lib().orc_energy(arrs[0].size, *map(_d, arrs), G_ref() if G is None else G, _d(out))
+    return dict(zip(["kinetic", "potential", "total", "virial"], out))
 
 
 def aabb(x, y, z, work_items=1024):

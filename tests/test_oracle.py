@@ -338,3 +338,29 @@ def test_stack_size_rule(oracle):
     assert oracle.stack_size(16384, 16) == 16 * 14
     assert oracle.stack_size(1000, 16) == 16 * 10 + 500
     assert oracle.stack_size(1 << 20, 16) == 320
+
+
+# ---- energy ------------------------------------------------------------------------
+def test_energy_two_body_closed_form(oracle):
+    G = oracle.G_ref()
+    M, m, r = 1.98847e30, 5.97e24, 1.0
+    v = math.sqrt(G * M / r)
+    e = oracle.energy([0.0, r], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, v], [0.0, 0.0], [M, m])
+    assert math.isclose(e["kinetic"], 0.5 * m * v * v, rel_tol=1e-15)
+    assert math.isclose(e["potential"], -G * M * m / r, rel_tol=1e-15)
+    assert math.isclose(e["total"], e["kinetic"] + e["potential"], rel_tol=1e-15)
+    # circular orbit: 2K = |U| for the orbiting body
+    assert math.isclose(e["virial"], 1.0, rel_tol=1e-12)
+
+
+def test_energy_matches_numpy(oracle):
+    d = uniform_cube(300, 21)
+    e = oracle.energy(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    G = oracle.G_ref()
+    p = np.stack([d["x"], d["y"], d["z"]], 1)
+    dist = np.linalg.norm(p[:, None, :] - p[None, :, :], axis=2)
+    iu = np.triu_indices(300, 1)
+    U = -(G * d["m"][iu[0]] * d["m"][iu[1]] / dist[iu]).sum()
+    K = 0.5 * (d["m"] * (d["vx"] ** 2 + d["vy"] ** 2 + d["vz"] ** 2)).sum()
+    assert math.isclose(e["potential"], U, rel_tol=1e-12)
+    assert math.isclose(e["kinetic"], K, rel_tol=1e-12)
