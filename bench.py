@@ -63,15 +63,34 @@ def cpu_baseline(d, algorithm, theta, dt):
     import oracle as orc
     n = d["m"].size
     threads = orc.get_threads()
-    log(f"cpu baseline: {algorithm} n={n}, 1 loop iteration on {threads} threads ...")
-    r = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt, dt,
-                     theta=theta)
+    iters = 4
+    log(f"cpu baseline: {algorithm} n={n}, {iters} loop iterations on {threads} threads ...")
+    r = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt,
+                     iters * dt, theta=theta)
     secs = r["loop_ms"] / 1000.0
     return {"value": n * r["steps"] / secs, "unit": "body-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{r['steps']} loop iteration(s) (kick-drift, forces"
                       f"{', octree build' if algorithm == 'BarnesHut' else ''}, kick) of the same "
                       f"n={n} input, FP64, t=0 force untimed; {secs:.1f} s"}
+
+
+def copy_calibration(mib=256, reps=3):
+    """A plain device-to-device copy of known size (hipMemcpy's blit kernel),
+    run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
 def main():
@@ -81,7 +100,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {args.gpus}).", file=sys.stderr)

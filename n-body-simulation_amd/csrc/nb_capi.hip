@@ -178,6 +178,8 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.int_local, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_first, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.groups, n))) return rc;
+    if ((rc = dev_alloc(c, &t.group_size, n))) return rc;
+    if ((rc = dev_alloc(c, &t.group_slot, n))) return rc;
     // a cell crossing a COM block boundary holds that boundary's two bodies:
     // at most kMaxDepth such cells per boundary
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
@@ -368,7 +370,8 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
                         t.int_nchild, t.int_arrive, t.int_count, t.int_sums, t.int_local,
-                        t.int_first, t.groups, t.spanning, t.ref, t.cm, t.edge, t.rec,
+                        t.int_first, t.groups, t.group_size, t.group_slot, t.spanning, t.ref,
+                        t.cm, t.edge, t.rec,
                         t.radix_hist, t.scan_part};
         (void)hipStreamSynchronize(c->stream);
         for (void *b : bufs) dev_free(c, b);

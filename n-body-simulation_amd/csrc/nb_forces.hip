@@ -289,6 +289,12 @@ constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 
 constexpr int kListCap = 2 * kGroup;
 constexpr int kGroupBlocks = 8192;  // persistent waves
 
+// Every traversal starts with fresh queue cursors (a kernel, not a memset,
+// so the sequence can be captured in a graph).
+__global__ void k_group_queue_reset(TreeHdr *h) {
+    if (threadIdx.x == 0) h->group_next = 0;
+}
+
 __device__ inline uint64_t lanes_below() {
     const uint32_t lane = threadIdx.x & 63u;
     return lane == 0 ? 0ull : (~0ull >> (64u - lane));
@@ -311,6 +317,11 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t peak = 0;
     bool overflow = false;
 
+    // One queue over the octree-ordered group list: waves running at the same
+    // time walk neighbouring groups, whose interaction lists overlap, so the
+    // records they read are shared in L2.  (Splitting the list into one chunk
+    // per XCD cut HBM fetches 3x but lengthened the kernel by 40%: the tail
+    // of the slowest chunk dominates.)
     while (!overflow) {
         uint32_t slot = 0;
         if (lane == 0)
@@ -541,6 +552,7 @@ void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const 
                      const double *m, double eps2, double theta, double *ax, double *ay,
                      double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
+    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr);
     hipLaunchKernelGGL(k_bh_group, dim3(blocks), dim3(kGroup), 0, s, t.n,
                        (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,

@@ -46,7 +46,7 @@ struct TreeHdr {
     uint32_t spanning_count;                           // cells crossing a COM block boundary
     uint32_t stack_peak;                               // deepest group-traversal stack seen
     uint32_t group_count;                              // traversal groups
-    uint32_t group_next;                               // work-queue cursor of the traversal
+    uint32_t group_next;                               // traversal work-queue cursor
 };
 
 // Bodies per traversal group: a group is an octree cell holding at most
@@ -90,6 +90,8 @@ struct TreeBufs {
     uint8_t *int_local = nullptr;   // 1 if the cell's bodies lie in one kComBlock block
     uint32_t *int_first = nullptr;  // first sorted position of the cell's bodies
     uint2 *groups = nullptr;        // traversal groups {first sorted position, body count}
+    uint32_t *group_size = nullptr; // per sorted position: size of the group starting there
+    uint32_t *group_slot = nullptr; // per sorted position: 1 if a group starts there, then slots
     uint32_t *spanning = nullptr;   // cells crossing a block boundary
     uint32_t spanning_cap = 0;
     int32_t *ref = nullptr;

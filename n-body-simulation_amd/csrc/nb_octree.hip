@@ -515,13 +515,29 @@ __global__ void __launch_bounds__(kComThreads) k_com_spanning(
 // ---- traversal groups -------------------------------------------------------------
 // A group is a cell with at most kGroupMax bodies whose parent holds more, or
 // a lone body whose parent holds more.  Groups partition the bodies into
-// spatially compact runs of the octree order.
-__global__ void __launch_bounds__(kThreads) k_groups(uint32_t n, uint32_t icap, TreeHdr *h,
-                                                     const uint32_t *__restrict__ counts,
-                                                     const uint32_t *__restrict__ int_parent,
-                                                     const uint32_t *__restrict__ int_first,
-                                                     const uint32_t *__restrict__ body_parent,
-                                                     uint2 *groups) {
+// spatially compact runs of the octree order.  Every sorted position starts
+// at most one group: mark the starts, scan, scatter -- the group list comes
+// out in octree order, so neighbouring groups share most of their walk.
+// pass 1: every sorted position; lone bodies start a group of one
+__global__ void __launch_bounds__(kThreads) k_group_marks_bodies(
+    uint32_t n, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ body_parent,
+    uint32_t *group_size, uint32_t *group_slot) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pk = body_parent[i];
+    const bool lone = pk == 0xffffffffu ? (n == 1) : counts[pk] > kGroupMax;
+    group_size[i] = lone ? 1u : 0u;
+    group_slot[i] = lone ? 1u : 0u;
+}
+
+// pass 2: group cells mark their first sorted position (never a lone body's)
+__global__ void __launch_bounds__(kThreads) k_group_marks_cells(uint32_t n, uint32_t icap,
+                                                                const TreeHdr *__restrict__ h,
+                                                                const uint32_t *__restrict__ counts,
+                                                                const uint32_t *__restrict__ int_parent,
+                                                                const uint32_t *__restrict__ int_first,
+                                                                uint32_t *group_size,
+                                                                uint32_t *group_slot) {
     const uint32_t I = min(h->internal_count, icap);
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     const uint64_t t0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -529,18 +545,23 @@ __global__ void __launch_bounds__(kThreads) k_groups(uint32_t n, uint32_t icap, 
         const uint32_t c = counts[k];
         const uint32_t pc = k == 0 ? 0xffffffffu : counts[int_parent[k]];
         if (c <= kGroupMax && pc > kGroupMax) {
-            const uint32_t slot = atomicAdd(&h->group_count, 1u);
-            if (slot < n) groups[slot] = make_uint2(int_first[k], c);
+            const uint32_t f = int_first[k];
+            if (f < n) {
+                group_size[f] = c;
+                group_slot[f] = 1u;
+            }
         }
     }
-    for (uint64_t i = t0; i < n; i += stride) {
-        const uint32_t pk = body_parent[i];
-        const bool lone = pk == 0xffffffffu ? (n == 1) : counts[pk] > kGroupMax;
-        if (lone) {
-            const uint32_t slot = atomicAdd(&h->group_count, 1u);
-            if (slot < n) groups[slot] = make_uint2((uint32_t)i, 1u);
-        }
-    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
+                                                            const uint32_t *__restrict__ group_size,
+                                                            const uint32_t *__restrict__ group_slot,
+                                                            uint2 *groups) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = group_size[i];
+    if (s) groups[group_slot[i]] = make_uint2(i, s);
 }
 
 // ---- 8. node records -----------------------------------------------------------
@@ -656,10 +677,19 @@ void launch_node_records(const TreeBufs &t, const double *x, const double *y, co
                        (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, G,
                        (int)f64, (int)f32, t.cm, t.edge, t.rec);
     if (f32) {
+        const dim3 grid(ceil_div(t.n, kThreads));
+        hipLaunchKernelGGL(k_group_marks_bodies, grid, dim3(kThreads), 0, s, t.n,
+                           (const uint32_t *)t.int_count, (const uint32_t *)t.body_parent,
+                           t.group_size, t.group_slot);
         const uint32_t gb = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(t.n, kThreads)));
-        hipLaunchKernelGGL(k_groups, dim3(gb), dim3(kThreads), 0, s, t.n, t.icap, t.hdr,
-                           (const uint32_t *)t.int_count, (const uint32_t *)t.int_parent,
-                           (const uint32_t *)t.int_first, (const uint32_t *)t.body_parent,
+        hipLaunchKernelGGL(k_group_marks_cells, dim3(gb), dim3(kThreads), 0, s, t.n, t.icap,
+                           (const TreeHdr *)t.hdr, (const uint32_t *)t.int_count,
+                           (const uint32_t *)t.int_parent, (const uint32_t *)t.int_first,
+                           t.group_size, t.group_slot);
+        scan_exclusive_u32(t.group_slot, t.group_slot, t.n, &t.hdr->group_count, t.scan_part,
+                           t.scan_part_cap, s);
+        hipLaunchKernelGGL(k_group_scatter, grid, dim3(kThreads), 0, s, t.n,
+                           (const uint32_t *)t.group_size, (const uint32_t *)t.group_slot,
                            t.groups);
     }
 }

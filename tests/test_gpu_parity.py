@@ -302,3 +302,17 @@ def test_energy_bitexact(gpu, oracle, kind, n):
     r = oracle.energy(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
     for k in ("kinetic", "potential", "total", "virial"):
         assert np.float64(e[k]).view(np.int64) == np.float64(r[k]).view(np.int64), k
+
+
+def test_bh_forces_repeatable(gpu):
+    # the traversal must not depend on state left by an earlier traversal of
+    # the same octree (work-queue cursors, accumulators)
+    d = make(gpu, "plummer", 20000)
+    for prec in (gpu.PREC_FP32C, gpu.PREC_FP64):
+        with ctx_for(gpu, d, theta=0.6, precision=prec) as c:
+            c.bh_build()
+            a1 = c.bh_forces()
+            a2 = c.bh_forces()
+        for u, v in zip(a1, a2):
+            np.testing.assert_array_equal(bits(u), bits(v))
+        assert np.all(np.isfinite(a1[0])) and np.abs(a1[0]).max() > 0
