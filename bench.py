@@ -27,6 +27,13 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention, reference README.md:158
 ERR_SAMPLE = 16384
 
+# HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE runs of this script scaled by a device copy of known size
+# (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0).
+TRAFFIC_BYTES = {
+    "bh_disk_1m": 3.33e9,
+}
+
 WORKLOADS = {
     # name: (generator, n, seed, algorithm, theta, dt, precision)
     "bh_disk_1m": ("disk", 1 << 20, 0x4E424F44590001, "BarnesHut", 0.5, 1.0 / 24, nb.PREC_FP32C),
@@ -144,7 +151,7 @@ def main():
     achieved = flops / (accel_ms * 1e-3) / 1e12
     peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
     roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": None,
+            "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get(args.workload),
             "kernel": "k_bh_group" if algorithm == "BarnesHut" else
             ("k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"),
             "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops}
