@@ -23,9 +23,18 @@ import nbody as nb  # noqa: E402
 METRIC = "body-steps/s at median rel. force err ≤1e-4 vs FP64 direct sum"
 FP32_PEAK_TFLOPS = 157.3
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # dense matrix-core peak
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention, reference README.md:158
+MLP_FLOP_PER_PAIR = 18432  # 2 * (13*64 + 64*64 + 64*64 + 64*3) per network evaluation
 ERR_SAMPLE = 16384
+
+# Learned correction: training set (Plummer snapshots, accepted (cell, body)
+# pairs of a per-body walk at theta 1.0, targets from the exact cell sums) and
+# the plain Barnes-Hut opening angle it is compared with.
+MLP_TRAIN = dict(snapshots=4, n=262144, targets=1800, pairs=2_000_000, theta=1.0, epochs=40,
+                 batch=4096)
+MLP_PLAIN_THETA = 0.4
 
 # HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this script scaled by a device copy of known size
@@ -40,6 +49,9 @@ WORKLOADS = {
     "naive_disk_16k": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP64),
     "naive_disk_16k_fp32": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP32C),
     "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", 0.5, 1e-3, nb.PREC_FP32C),
+    # theta 1.0 with the learned far-field correction on every accepted cell
+    "bh_plummer_4m_mlp": ("plummer", 1 << 22, 0x4E424F44590003, "BarnesHut", 1.0, 1e-3,
+                          nb.PREC_FP32C),
 }
 
 
@@ -65,12 +77,12 @@ def median_rel_err(ctx, n, rng):
 
 def cpu_baseline(d, algorithm, theta, dt):
     """The reference's CPU path (oracle/ restatement, OpenMP on all host cores),
-    one loop iteration of the same workload."""
+    a few loop iterations of the same workload (about 10 s of CPU time)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     n = d["m"].size
     threads = orc.get_threads()
-    iters = 4
+    iters = max(1, (4 << 20) // n) if algorithm == "BarnesHut" else 4
     log(f"cpu baseline: {algorithm} n={n}, {iters} loop iterations on {threads} threads ...")
     r = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt,
                      iters * dt, theta=theta)
@@ -118,48 +130,40 @@ def main():
               f"--gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     kind, n, seed, algorithm, theta, dt, prec = WORKLOADS[args.workload]
+    mlp = args.workload.endswith("_mlp")
     log(f"device: {nb.device_name()}")
+    net, train_info = (train_correction() if mlp else (None, None))
     log(f"workload {args.workload}: generating {kind} n={n} ...")
     d = gen(kind, n, seed)
     ctx = nb.Context(n, theta=theta, precision=prec)
-    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
-    # t = 0 forces (outside the loop, as in the reference)
-    if algorithm == "BarnesHut":
-        ctx.bh_build()
-        ctx.bh_forces()
-    else:
-        ctx.forces_direct()
-    if args.warmup:
-        ctx.steps(algorithm, dt, args.warmup)
-    log(f"timing {args.steps} steps ...")
-    t0 = time.perf_counter()
-    ctx.steps(algorithm, dt, args.steps)  # returns after the stream is idle
-    t1 = time.perf_counter()
-    phases = ctx.phase_ms()
-    secs = t1 - t0
+    if net is not None:
+        ctx.set_correction(net, 0.0)
+    secs, phases, err, sample, inter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
     value = n * args.steps / secs
-    inter = ctx.interactions().astype(np.float64) if algorithm == "BarnesHut" else None
-    err, sample = median_rel_err(ctx, n, np.random.default_rng(1))
-    log(f"median rel err {err:.3e} over {sample} targets")
 
     # roofline of the dominant kernel: the force kernel
     accel_ms = phases["Acceleration Kernel Time"] / args.steps
-    if algorithm == "BarnesHut":
+    if mlp:
+        pairs = ctx.correction_pairs()  # network evaluations of the last traversal
+        flops = MLP_FLOP_PER_PAIR * float(pairs)
+        peak, bound, kernel = BF16_PEAK_TFLOPS, "mfma", "k_bh_group<corrected>"
+    elif algorithm == "BarnesHut":
         flops = FLOP_PER_INTERACTION * float(inter.sum())
+        peak, bound, kernel = FP32_PEAK_TFLOPS, "valu", "k_bh_group"
     else:
         flops = FLOP_PER_INTERACTION * float(n) * float(n)
+        peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
+        bound, kernel = "valu", "k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"
     achieved = flops / (accel_ms * 1e-3) / 1e12
-    peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
-    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+    roof = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get(args.workload),
-            "kernel": "k_bh_group" if algorithm == "BarnesHut" else
-            ("k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"),
-            "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops}
+            "kernel": kernel, "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops}
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if prec == nb.PREC_FP32C else "f64", "data": "synthetic",
+        "dtype": ("bf16+f32+f64" if mlp else "f32+f64") if prec == nb.PREC_FP32C else "f64",
+        "data": "synthetic",
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
                    "dt_days": dt, "median_rel_err": err, "err_sample": sample,
@@ -167,10 +171,79 @@ def main():
                    "mean_interactions": None if inter is None else float(inter.mean())},
         "roofline": roof,
     }
+    if mlp:
+        # the same input with plain Barnes-Hut at the comparison opening angle
+        ctx.set_correction(None)
+        ctx.set_params(theta=MLP_PLAIN_THETA)
+        psecs, pph, perr, _, pinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
+        pvalue = n * args.steps / psecs
+        out["config"]["learned_correction"] = dict(
+            train_info, tau=0.0, network_evals_per_step=pairs,
+            plain_theta=MLP_PLAIN_THETA, plain_value=round(pvalue, 1),
+            plain_median_rel_err=perr, plain_mean_interactions=float(pinter.mean()),
+            speedup_vs_plain=round(value / pvalue, 4))
+        log(f"plain theta={MLP_PLAIN_THETA}: {pvalue:.4g} body-steps/s, err {perr:.3e}; "
+            f"corrected: {value:.4g}, err {err:.3e}")
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, algorithm, theta, dt)
     print(json.dumps(out), flush=True)
+
+
+def timed_run(ctx, d, algorithm, dt, steps, warmup):
+    """Set the initial state, t = 0 force (untimed, as in the reference),
+    `warmup` untimed steps, then `steps` timed loop iterations.  Returns
+    (seconds, phase ms summed over the timed steps, median error, sample size,
+    interactions per body)."""
+    n = d["m"].size
+    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    if algorithm == "BarnesHut":
+        ctx.bh_build()
+        ctx.bh_forces()
+    else:
+        ctx.forces_direct()
+    if warmup:
+        ctx.steps(algorithm, dt, warmup)
+    log(f"timing {steps} steps ...")
+    t0 = time.perf_counter()
+    ctx.steps(algorithm, dt, steps)  # returns after the stream is idle
+    t1 = time.perf_counter()
+    phases = ctx.phase_ms()
+    inter = ctx.interactions().astype(np.float64) if algorithm == "BarnesHut" else None
+    err, sample = median_rel_err(ctx, n, np.random.default_rng(1))
+    log(f"median rel err {err:.3e} over {sample} targets")
+    return t1 - t0, phases, err, sample, inter
+
+
+def train_correction():
+    """Train the correction network on the GPU (outside the timed region):
+    pairs from MLP_TRAIN['snapshots'] Plummer snapshots, MLP_TRAIN['pairs'] in all."""
+    c = MLP_TRAIN
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    per = c["pairs"] // c["snapshots"]
+    for s in range(c["snapshots"]):
+        d = nb.gen_plummer(c["n"], 0x4E424F44591000 + s)
+        ctx = nb.Context(c["n"], theta=c["theta"], precision=nb.PREC_FP32C)
+        ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+        ctx.bh_build()
+        t = np.sort(np.random.default_rng(s).choice(c["n"], c["targets"], replace=False))
+        pr = ctx.mlp_pairs(c["theta"], t.astype(np.uint32), per)
+        Xs.append(pr["features"])
+        Ts.append(pr["residuals"])
+        ctx.close()
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    t1 = time.perf_counter()
+    net = nb.Mlp(seed=7)
+    losses = net.train(X, T, epochs=c["epochs"], batch=c["batch"], seed=9)
+    t2 = time.perf_counter()
+    var = float(np.mean(T.astype(np.float64) ** 2))
+    log(f"correction trained: {X.shape[0]} pairs ({t1 - t0:.1f} s), {c['epochs']} epochs in "
+        f"{t2 - t1:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e} (target mean square {var:.3e})")
+    return net, dict(train_pairs=int(X.shape[0]), train_snapshots=c["snapshots"],
+                     train_bodies=c["n"], train_epochs=c["epochs"],
+                     train_seconds=round(t2 - t1, 3), train_loss=float(losses[-1]),
+                     train_target_mean_square=var)
 
 
 if __name__ == "__main__":

@@ -165,6 +165,84 @@ uint64_t nb_tree_canonical(nb_ctx *ctx, uint64_t cap, uint8_t *depth, uint8_t *k
  * indexed by body id. */
 int nb_bh_interactions(nb_ctx *ctx, uint32_t *count);
 
+/* ---- learned far-field correction ---------------------------------------
+ * No counterpart in the reference.  A network of 13 inputs -> 64 -> 64 -> 64
+ * (SiLU) -> 3 outputs, with BF16 weights on the matrix cores and FP32
+ * accumulation, predicts for an accepted octree cell and a target body the
+ * residual of the monopole approximation:
+ *   correction = net(f) * G M e^2 / d^4
+ * Here r = com - target, d = |r|, M is the cell mass and e the cell edge. The
+ * features f are:
+ *   f[0..2]  = r / d
+ *   f[3]     = e / d
+ *   f[4]     = ln(M / M_root) / 16
+ *   f[5..10] = Q / (M e^2), Q = sum m (3 s s^T - |s|^2 I) about the cell's
+ *              centre of mass, as {xx, yy, zz, xy, xz, yz}
+ *   f[11]    = o . r/d
+ *   f[12]    = |o - (o . r/d) r/d|, with o = (geometric centre - com) / e.
+ * Training targets are (a_exact - a_monopole) * d^4 / (G M e^2), with a_exact
+ * summed over the cell's bodies in FP64.
+ *
+ * Parameters, 9411 floats, row-major with the input index first:
+ *   W1[13][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][3] b4[3]
+ */
+typedef struct nb_mlp nb_mlp;
+
+typedef struct nb_mlp_train_opts {
+    uint32_t epochs;  /* passes over the data set */
+    uint32_t batch;   /* samples per Adam step */
+    float lr;         /* initial learning rate (cosine decay to lr_final) */
+    float lr_final;
+    float beta1, beta2, eps;  /* Adam */
+    uint32_t augment; /* nonzero: rotate every sample by a fresh random rotation
+                         each epoch (direction, quadrupole and target together;
+                         the physics is rotation invariant) */
+    uint64_t seed;    /* data-order permutations and rotations */
+} nb_mlp_train_opts;
+
+void nb_mlp_default_train_opts(nb_mlp_train_opts *o);
+/* Weights uniform in +-sqrt(6/fan_in) from a splitmix64 stream, zero biases. */
+int nb_mlp_create(nb_mlp **out, uint64_t seed);
+int nb_mlp_destroy(nb_mlp *mlp);
+const char *nb_mlp_last_error(const nb_mlp *mlp);
+int nb_mlp_set_params(nb_mlp *mlp, const float *params);  /* also resets Adam state */
+int nb_mlp_get_params(nb_mlp *mlp, float *params);
+/* out[count][3] = net(features[count][13]) on the GPU. */
+int nb_mlp_forward(nb_mlp *mlp, uint64_t count, const float *features, float *out);
+/* Gradient of L = mean over samples and components of (net(f) - t)^2 (the
+ * training loss) for all 9411 parameters, computed by the training kernels. */
+int nb_mlp_grad(nb_mlp *mlp, uint32_t count, const float *features, const float *targets,
+                float *grad, float *loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8, step number t >= 1) from a given gradient. */
+int nb_mlp_adam_step(nb_mlp *mlp, const float *grad, float lr, uint32_t t);
+/* Train on the GPU; epoch_loss (may be NULL) receives the mean batch loss of each epoch. */
+int nb_mlp_train(nb_mlp *mlp, uint64_t count, const float *features, const float *targets,
+                 const nb_mlp_train_opts *opts, float *epoch_loss);
+
+/* Training pairs from the context's current octree (after nb_bh_build): for
+ * each listed target body, a depth-first walk accepts internal cells with
+ * e / d < theta (d to the cell's centre of mass) and opens the others.  For
+ * up to cap pairs it writes the cell (internal-cell index = its rank among
+ * internal cells in depth-first pre-order with children in digit order), the
+ * body, the 13 features and the 3 normalised residuals.  *count receives the
+ * number of pairs found; only min(*count, cap) are written. */
+int nb_mlp_pairs(nb_ctx *ctx, double theta, uint32_t nt, const uint32_t *targets, uint64_t cap,
+                 uint32_t *cell, uint32_t *body, float *features, float *residuals,
+                 uint64_t *count);
+/* Attach a trained network (NULL detaches) to the FP32C Barnes-Hut traversal.
+ * Every accepted cell with e / d_group > tau (d_group is the distance to the
+ * group's box) then also gets the learned correction for each body.  The
+ * network must outlive the attachment. */
+int nb_set_correction(nb_ctx *ctx, nb_mlp *mlp, double tau);
+/* Network evaluations ((cell, body) pairs) of the last corrected nb_bh_forces
+ * (18,432 flop each, forward). */
+int nb_correction_pairs(nb_ctx *ctx, uint64_t *pairs);
+/* Quadrupole moments Q = sum m (3 s s^T - |s|^2 I) of every node, about its
+ * centre of mass, as {xx, yy, zz, xy, xz, yz}, in the canonical order of
+ * nb_tree_canonical (zero for leaves).  q has 6 * cap doubles.  Returns the
+ * record count (0 if cap is too small). */
+uint64_t nb_tree_quadrupoles(nb_ctx *ctx, uint64_t cap, double *q);
+
 /* ---- synthetic inputs (seeded; same bits on every machine) -------------- */
 /* One star of 1 Msun at rest at the origin plus n-1 bodies on Kepler orbits:
  * a ~ U[2.1, 3.3] AU, e ~ U[0, 0.3], i ~ U[0, 20 deg], angles ~ U[0, 2pi),

@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "nb_internal.h"
+#include "nb_mlp.h"
 
 using nb::TreeBufs;
 using nb::TreeHdr;
@@ -38,6 +38,12 @@ struct nb_ctx {
     double *targets_out = nullptr;
     uint32_t targets_cap = 0;
     double *energy_terms = nullptr;
+    // learned far-field correction
+    nb_mlp *corr = nullptr;
+    float corr_tau2 = 0.f;
+    double *quad = nullptr;          // 6 per internal cell
+    nb::MlpCell *mcell = nullptr;    // per internal cell
+    bool quad_valid = false;         // quadrupoles match the current octree
     std::vector<hipEvent_t> events;
     std::vector<std::pair<int, size_t>> marks;  // (phase, index of start event)
     double phase[NB_PHASE_COUNT] = {0};
@@ -218,7 +224,20 @@ int build_tree(nb_ctx *c) {
     nb::launch_com(t, c->x, c->y, c->z, c->m, c->stream);
     const bool fp64 = c->p.precision == NB_PREC_FP64;
     nb::launch_node_records(t, c->x, c->y, c->z, c->m, c->p.G, fp64, !fp64, c->stream);
+    c->quad_valid = false;
+    if (c->corr && !fp64) {
+        nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);
+        c->quad_valid = true;
+    }
     mark_end(c, NB_PHASE_COM);
+    return NB_OK;
+}
+
+int ensure_quad_buffers(nb_ctx *c) {
+    if (c->quad && c->mcell) return NB_OK;
+    int rc;
+    if ((rc = dev_alloc(c, &c->quad, 6ull * c->tree.icap))) return rc;
+    if ((rc = dev_alloc(c, &c->mcell, c->tree.icap))) return rc;
     return NB_OK;
 }
 
@@ -236,6 +255,11 @@ int bh_forces_enqueue(nb_ctx *c) {
         nb::launch_bh_exact(c->tree, c->x, c->y, c->z, c->p.G, c->p.eps2, c->p.theta,
                             c->p.sort_bodies, c->exact_stack, c->exact_cap, c->ax, c->ay, c->az,
                             c->inter, c->stream);
+    } else if (c->corr && c->quad_valid) {
+        const nb::MlpWeights w = nb::mlp_weights(c->corr);
+        const nb::CorrArgs ca{w.pack, w.master, c->mcell, c->corr_tau2};
+        nb::launch_bh_group_corr(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta, ca,
+                                 c->ax, c->ay, c->az, c->inter, c->stream);
     } else {
         nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta, c->ax,
                             c->ay, c->az, c->inter, c->stream);
@@ -375,10 +399,16 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
                         t.radix_hist, t.scan_part};
         (void)hipStreamSynchronize(c->stream);
         for (void *b : bufs) dev_free(c, b);
+        dev_free(c, c->quad);
+        dev_free(c, c->mcell);
+        c->quad = nullptr;
+        c->mcell = nullptr;
+        c->quad_valid = false;
         c->tree = TreeBufs();
         c->tree_valid = false;
         int rc = alloc_tree(c);
         if (rc) return rc;
+        if (c->corr && (rc = ensure_quad_buffers(c))) return rc;
         NB_HIP(c, hipStreamSynchronize(c->stream));
     }
     c->tree_valid = false;  // opening radii depend on theta
@@ -683,6 +713,141 @@ uint64_t nb_tree_canonical(nb_ctx *c, uint64_t cap, uint8_t *depth, uint8_t *kin
         ++k;
     }
     return k;
+}
+
+// ---- learned far-field correction: context side ---------------------------------
+
+int nb_set_correction(nb_ctx *c, nb_mlp *mlp, double tau) {
+    if (!c || !std::isfinite(tau) || tau < 0)
+        return fail(c, NB_E_INVALID_ARG, "nb_set_correction: bad arguments");
+    if (mlp) {
+        int rc = ensure_quad_buffers(c);
+        if (rc) return rc;
+    }
+    c->corr = mlp;
+    c->corr_tau2 = (float)(tau * tau);
+    c->tree_valid = false;  // the next build adds the quadrupoles
+    c->quad_valid = false;
+    return NB_OK;
+}
+
+int nb_correction_pairs(nb_ctx *c, uint64_t *pairs) {
+    if (!c || !pairs) return NB_E_INVALID_ARG;
+    NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                             c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    *pairs = c->hdr_host->corr_pairs;
+    return NB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// quadrupoles of the current octree, computed now if the build did not
+int quad_now(nb_ctx *c) {
+    if (!c->tree_valid) return fail(c, NB_E_NOT_BUILT, "build the octree first");
+    if (c->quad_valid) return NB_OK;
+    int rc = ensure_quad_buffers(c);
+    if (rc) return rc;
+    nb::launch_quadrupoles(c->tree, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);
+    NB_HIP(c, hipGetLastError());
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    c->quad_valid = true;
+    return NB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t nb_tree_quadrupoles(nb_ctx *c, uint64_t cap, double *q) {
+    if (!c || !q || quad_now(c)) return 0;
+    if (hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+        return 0;
+    const uint32_t I = std::min(c->hdr_host->internal_count, c->tree.icap);
+    const uint64_t nodes = 1 + 8ull * I;
+    if (cap < nodes) return 0;
+    std::vector<int32_t> ref(nodes);
+    std::vector<double> quad(6ull * I);
+    bool ok = hipMemcpy(ref.data(), c->tree.ref, nodes * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (I) ok = ok && hipMemcpy(quad.data(), c->quad, quad.size() * 8, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) return 0;
+    std::vector<uint64_t> stack{0};
+    uint64_t k = 0;
+    while (!stack.empty()) {
+        const uint64_t node = stack.back();
+        stack.pop_back();
+        const int32_t r = ref[node];
+        for (int u = 0; u < 6; ++u) q[6 * k + u] = r >= 0 ? quad[6ull * r + u] : 0.0;
+        if (r >= 0)
+            for (int dg = 7; dg >= 0; --dg) stack.push_back(1ull + 8ull * (uint64_t)r + dg);
+        ++k;
+    }
+    return k;
+}
+
+int nb_mlp_pairs(nb_ctx *c, double theta, uint32_t nt, const uint32_t *targets, uint64_t cap,
+                 uint32_t *cell, uint32_t *body, float *features, float *residuals,
+                 uint64_t *count) {
+    if (!c || !count || !(theta > 0) || (nt && !targets) ||
+        (cap && (!cell || !body || !features || !residuals)))
+        return fail(c, NB_E_INVALID_ARG, "nb_mlp_pairs: bad arguments");
+    *count = 0;
+    for (uint32_t t = 0; t < nt; ++t)
+        if (targets[t] >= c->n) return fail(c, NB_E_INVALID_ARG, "target index out of range");
+    int rc = quad_now(c);
+    if (rc) return rc;
+    if (nt == 0) return NB_OK;
+    const uint32_t stack_cap = 1 + 7 * nb::kMaxDepth + 8;
+    // scratch of this call only
+    struct Scratch {
+        nb_ctx *c;
+        std::vector<void *> p;
+        ~Scratch() {
+            for (void *q : p) dev_free(c, q);
+        }
+    } sc{c, {}};
+    uint32_t *tdev = nullptr, *stack = nullptr, *pcell = nullptr, *pbody = nullptr;
+    unsigned long long *counter = nullptr;
+    float *F = nullptr, *T = nullptr;
+    const uint64_t cap1 = std::max<uint64_t>(cap, 1);
+    if ((rc = dev_alloc(c, &tdev, nt))) return rc;
+    sc.p.push_back(tdev);
+    if ((rc = dev_alloc(c, &stack, (uint64_t)nt * stack_cap))) return rc;
+    sc.p.push_back(stack);
+    if ((rc = dev_alloc(c, &counter, 1))) return rc;
+    sc.p.push_back(counter);
+    if ((rc = dev_alloc(c, &pcell, cap1))) return rc;
+    sc.p.push_back(pcell);
+    if ((rc = dev_alloc(c, &pbody, cap1))) return rc;
+    sc.p.push_back(pbody);
+    if ((rc = dev_alloc(c, &F, cap1 * nb::kMlpIn))) return rc;
+    sc.p.push_back(F);
+    if ((rc = dev_alloc(c, &T, cap1 * nb::kMlpOut))) return rc;
+    sc.p.push_back(T);
+    NB_HIP(c, hipMemcpyAsync(tdev, targets, (size_t)nt * 4, hipMemcpyHostToDevice, c->stream));
+    NB_HIP(c, hipMemsetAsync(counter, 0, sizeof(unsigned long long), c->stream));
+    nb::launch_pair_walk(c->tree, nt, tdev, c->x, c->y, c->z, theta, stack, stack_cap, cap, pcell,
+                         pbody, counter, c->stream);
+    unsigned long long found = 0;
+    NB_HIP(c, hipMemcpyAsync(&found, counter, sizeof(found), hipMemcpyDeviceToHost, c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    const uint64_t m = std::min<uint64_t>(found, cap);
+    nb::launch_pair_targets(c->tree, m, pcell, pbody, c->x, c->y, c->z, c->m, c->quad, c->mcell,
+                            c->p.G, c->p.eps2, F, T, c->stream);
+    NB_HIP(c, hipGetLastError());
+    if (m) {
+        NB_HIP(c, hipMemcpyAsync(cell, pcell, m * 4, hipMemcpyDeviceToHost, c->stream));
+        NB_HIP(c, hipMemcpyAsync(body, pbody, m * 4, hipMemcpyDeviceToHost, c->stream));
+        NB_HIP(c, hipMemcpyAsync(features, F, m * nb::kMlpIn * 4, hipMemcpyDeviceToHost, c->stream));
+        NB_HIP(c, hipMemcpyAsync(residuals, T, m * nb::kMlpOut * 4, hipMemcpyDeviceToHost, c->stream));
+    }
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    *count = found;
+    return NB_OK;
 }
 
 }  // extern "C"

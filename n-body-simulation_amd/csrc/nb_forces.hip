@@ -16,7 +16,7 @@
 #include <algorithm>
 #include <cmath>
 
-#include "nb_internal.h"
+#include "nb_mlp.h"
 
 namespace nb {
 namespace {
@@ -292,7 +292,10 @@ constexpr int kGroupBlocks = 8192;  // persistent waves
 // Every traversal starts with fresh queue cursors (a kernel, not a memset,
 // so the sequence can be captured in a graph).
 __global__ void k_group_queue_reset(TreeHdr *h) {
-    if (threadIdx.x == 0) h->group_next = 0;
+    if (threadIdx.x == 0) {
+        h->group_next = 0;
+        h->corr_pairs = 0;
+    }
 }
 
 __device__ inline uint64_t lanes_below() {
@@ -300,20 +303,36 @@ __device__ inline uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64u - lane));
 }
 
+// With the learned correction (kCorr), accepted cells that are close enough
+// (e^2 > tau2 * d_group^2) are also queued in a second LDS list; every 16 such
+// cells, the wave evaluates the network for the 16 cells x each body of the
+// group (one 16-sample matrix-core tile per body) and adds the corrections of
+// a body in a fixed order.
+constexpr int kCorrTile = 16;
+constexpr int kCellCap = kCorrTile + kGroup;
+
+template <bool kCorr>
 __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
     float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
-    TreeHdr *h) {
+    TreeHdr *h, CorrArgs corr) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
+    __shared__ float4 cl_e[kCorr ? kCellCap : 1];   // accepted cell: offset from group centre, G*M
+    __shared__ uint32_t cl_k[kCorr ? kCellCap : 1]; // accepted cell: internal index
+    __shared__ float4 boff[kCorr ? kGroup : 1];     // body offsets from the group centre
+    __shared__ float4 bcorr[kCorr ? kGroup : 1];    // learned correction per body
     const uint32_t lane = threadIdx.x;
     const uint64_t below = lanes_below();
     const uint32_t ngroups = min(h->group_count, n);
     // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
     const double ro_root = h->edge / theta;
     const float ro2_root = (float)(ro_root * ro_root);
+    const float e2_root = (float)(h->edge * h->edge);
+    MlpForward w;
+    if constexpr (kCorr) w.load(corr.pack, corr.master, (int)lane);
     uint32_t peak = 0;
     bool overflow = false;
 
@@ -386,6 +405,12 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
         double acc_x = 0, acc_y = 0, acc_z = 0;
         uint32_t n_inter = 0;
         uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
+        uint32_t cn = 0;          // wave-uniform size of the correction's cell list
+        uint32_t ncorr = 0;       // network evaluations of this group
+        if constexpr (kCorr) {
+            if (valid && q == 0) boff[sub] = make_float4(rx, ry, rz, 0.f);
+            bcorr[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
 
         auto flush = [&](uint32_t cnt) {
             float fx = 0.f, fy = 0.f, fz = 0.f;
@@ -396,8 +421,45 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             n_inter += cnt;
         };
 
-        // 0 skip, 1 list, 2 open
-        auto classify = [&](const NodeF32 &r, int depth, float4 &entry) -> int {
+        // learned correction of cells cl[base .. base+cnt) (cnt <= 16) for every body
+        auto corr_flush = [&](uint32_t base, uint32_t cnt) {
+            const int g = (int)(lane >> 4), p = (int)(lane & 15);
+            const bool pv = (uint32_t)p < cnt;
+            const float4 ce = cl_e[base + p];
+            MlpCell mc{};
+            if (pv) mc = corr.mcell[cl_k[base + p]];
+            const float e2c = mc.c.z * mc.c.z;
+            ncorr += cnt * s;
+            bf16x4 zero4;
+            for (int i = 0; i < 4; ++i) zero4[i] = (__bf16)0.f;
+            for (uint32_t b = 0; b < s; ++b) {
+                const float4 bo = boff[b];
+                float inv_d;
+                bf16x4 xin = mlp_features(ce.x - bo.x, ce.y - bo.y, ce.z - bo.z, mc, g, inv_d);
+                if (!pv) xin = zero4;
+                MlpActs act;
+                const f32x4 yv = mlp_forward_tile(w, xin, act);
+                const float inv2 = inv_d * inv_d;
+                const float sc = (pv && g == 0) ? ce.w * e2c * (inv2 * inv2) : 0.f;
+                float cx = yv[0] * sc, cy = yv[1] * sc, cz = yv[2] * sc;
+                for (int off = 8; off >= 1; off >>= 1) {
+                    cx += __shfl_xor(cx, off, 64);
+                    cy += __shfl_xor(cy, off, 64);
+                    cz += __shfl_xor(cz, off, 64);
+                }
+                if (lane == 0) {
+                    float4 t = bcorr[b];
+                    t.x += cx;
+                    t.y += cy;
+                    t.z += cz;
+                    bcorr[b] = t;
+                }
+            }
+        };
+
+        // 0 skip, 1 list, 2 open; d2 = squared distance to the group box (cells)
+        auto classify = [&](const NodeF32 &r, int depth, float4 &entry, float &d2) -> int {
+            d2 = 0.f;
             if (r.ref == kRefEmpty) return 0;
             const float dx = (r.x - gxh) + (r.lx - gxl);
             const float dy = (r.y - gyh) + (r.ly - gyl);
@@ -407,17 +469,27 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             const float ex = fmaxf(fabsf(dx) - hx, 0.f);
             const float ey = fmaxf(fabsf(dy) - hy, 0.f);
             const float ez = fmaxf(fabsf(dz) - hz, 0.f);
-            const float d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
+            d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
             return d2 > ldexpf(ro2_root, -2 * depth) ? 1 : 2;
         };
 
         {
             float4 e;
+            float d2r;
             const NodeF32 root = rec[0];
-            const int cls = classify(root, 0, e);  // every lane evaluates the root alike
+            const int cls = classify(root, 0, e, d2r);  // every lane evaluates the root alike
             if (cls == 1) {
                 if (lane == 0) lst[0] = e;
                 ln = 1;
+                if constexpr (kCorr) {
+                    if (root.ref >= 0 && e2_root > corr.tau2 * d2r) {
+                        if (lane == 0) {
+                            cl_e[0] = e;
+                            cl_k[0] = (uint32_t)root.ref;
+                        }
+                        cn = 1;
+                    }
+                }
             } else if (cls == 2) {
                 if (lane == 0) stk[0] = ((uint32_t)root.ref << 6);
                 sp = 1;
@@ -451,13 +523,25 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
                 break;
             }
             float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-            const int cls = classify(r, depth, e);
+            float d2 = 0.f;
+            const int cls = classify(r, depth, e, d2);
             const uint64_t bl = __ballot(cls == 1);
             const uint64_t bo = __ballot(cls == 2);
             if (cls == 1) lst[ln + __popcll(bl & below)] = e;
             if (cls == 2) {
                 const uint32_t p = sp + __popcll(bo & below);
                 if (p < kStackCap) stk[p] = ((uint32_t)r.ref << 6) | (uint32_t)depth;
+            }
+            if constexpr (kCorr) {
+                const bool cc = cls == 1 && r.ref >= 0 &&
+                                ldexpf(e2_root, -2 * depth) > corr.tau2 * d2;
+                const uint64_t bc = __ballot(cc);
+                if (cc) {
+                    const uint32_t p = cn + __popcll(bc & below);
+                    cl_e[p] = e;
+                    cl_k[p] = (uint32_t)r.ref;
+                }
+                cn += __popcll(bc);  // < kCorrTile before, so at most kCellCap - 1 after
             }
             ln += __popcll(bl);
             sp += __popcll(bo);
@@ -477,8 +561,26 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
                 ln = rest;
                 __syncthreads();
             }
+            if constexpr (kCorr) {
+                if (cn >= (uint32_t)kCorrTile) {
+                    uint32_t base = 0;
+                    for (; cn - base >= (uint32_t)kCorrTile; base += kCorrTile)
+                        corr_flush(base, kCorrTile);
+                    __syncthreads();
+                    const uint32_t rest = cn - base;
+                    if (lane < rest) {
+                        cl_e[lane] = cl_e[base + lane];
+                        cl_k[lane] = cl_k[base + lane];
+                    }
+                    cn = rest;
+                    __syncthreads();
+                }
+            }
         }
         if (ln > 0) flush(ln);
+        if constexpr (kCorr) {
+            if (cn > 0) corr_flush(0, cn);
+        }
         __syncthreads();
         // add up the list shares of each body (fixed order: deterministic)
         for (uint32_t off = s2; off < (uint32_t)kGroup; off <<= 1) {
@@ -486,7 +588,16 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             acc_y += __shfl_xor(acc_y, off, 64);
             acc_z += __shfl_xor(acc_z, off, 64);
         }
+        if constexpr (kCorr) {
+            if (lane == 0 && ncorr) atomicAdd(&h->corr_pairs, (unsigned long long)ncorr);
+        }
         if (valid && q == 0 && !overflow) {
+            if constexpr (kCorr) {
+                const float4 c = bcorr[sub];
+                acc_x += (double)c.x;
+                acc_y += (double)c.y;
+                acc_z += (double)c.z;
+            }
             ax[bi] = acc_x;
             ay[bi] = acc_y;
             az[bi] = acc_z;
@@ -553,10 +664,22 @@ void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const 
                      double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
     hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr);
-    hipLaunchKernelGGL(k_bh_group, dim3(blocks), dim3(kGroup), 0, s, t.n,
+    hipLaunchKernelGGL(k_bh_group<false>, dim3(blocks), dim3(kGroup), 0, s, t.n,
                        (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
-                       interactions, t.hdr);
+                       interactions, t.hdr, CorrArgs{});
+}
+
+void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, const double *z,
+                          const double *m, double eps2, double theta, const CorrArgs &corr,
+                          double *ax, double *ay, double *az, uint32_t *interactions,
+                          hipStream_t s) {
+    const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
+    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr);
+    hipLaunchKernelGGL(k_bh_group<true>, dim3(blocks), dim3(kGroup), 0, s, t.n,
+                       (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
+                       (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                       interactions, t.hdr, corr);
 }
 
 }  // namespace nb

@@ -47,6 +47,7 @@ struct TreeHdr {
     uint32_t stack_peak;                               // deepest group-traversal stack seen
     uint32_t group_count;                              // traversal groups
     uint32_t group_next;                               // traversal work-queue cursor
+    unsigned long long corr_pairs;                     // learned-correction evaluations
 };
 
 // Bodies per traversal group: a group is an octree cell holding at most
@@ -56,6 +57,26 @@ constexpr uint32_t kGroupMax = 64;
 struct InternalSums {
     double x, y, z, m;  // centre-of-mass numerators and mass
 };
+
+// An octree cell: minimum corner and edge.
+struct Cell {
+    double mx, my, mz, e;
+};
+
+// One level of the reference's classification chain; returns the digit and
+// moves the cell to the child (Subtrees.cpp:256-315, :348-358).
+__device__ inline uint32_t descend(Cell &c, double px, double py, double pz) {
+    const double h = c.e / 2.0;
+    const double midx = c.mx + h, midy = c.my + h, midz = c.mz + h;
+    const bool up = py > midy;
+    const bool right = px > midx;
+    const bool back = pz < midz;
+    if (up) c.my = midy;
+    if (right) c.mx = midx;
+    if (!back) c.mz = midz;
+    c.e = h;
+    return (uint32_t)up * 4u + (uint32_t)right * 2u + (uint32_t)back;
+}
 
 // Traversal record of one node (32 bytes, children of a cell are adjacent):
 // FP32 centre of mass split into a high part and the FP32 rounding of the

@@ -180,25 +180,6 @@ __global__ void __launch_bounds__(kThreads) k_aabb_final(const double *part, uin
 }
 
 // ---- 2. keys ---------------------------------------------------------------
-struct Cell {
-    double mx, my, mz, e;
-};
-
-// One level of the reference's classification chain; returns the digit and
-// moves the cell to the child (Subtrees.cpp:256-315, :348-358).
-__device__ inline uint32_t descend(Cell &c, double px, double py, double pz) {
-    const double h = c.e / 2.0;
-    const double midx = c.mx + h, midy = c.my + h, midz = c.mz + h;
-    const bool up = py > midy;
-    const bool right = px > midx;
-    const bool back = pz < midz;
-    if (up) c.my = midy;
-    if (right) c.mx = midx;
-    if (!back) c.mz = midz;
-    c.e = h;
-    return (uint32_t)up * 4u + (uint32_t)right * 2u + (uint32_t)back;
-}
-
 __device__ inline uint64_t key_from(Cell &c, double px, double py, double pz) {
     uint64_t key = 0;
 #pragma unroll

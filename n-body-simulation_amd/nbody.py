@@ -39,7 +39,20 @@ EXPORTS = [
     "nb_forces_direct_targets", "nb_bh_build", "nb_bh_forces", "nb_kick_drift", "nb_kick",
     "nb_steps", "nb_phase_ms", "nb_tree_info", "nb_tree_sorted", "nb_tree_canonical",
     "nb_bh_interactions", "nb_gen_star_disk", "nb_gen_plummer", "nb_energy",
+    "nb_mlp_default_train_opts", "nb_mlp_create", "nb_mlp_destroy", "nb_mlp_last_error",
+    "nb_mlp_set_params", "nb_mlp_get_params", "nb_mlp_forward", "nb_mlp_grad",
+    "nb_mlp_adam_step", "nb_mlp_train", "nb_mlp_pairs", "nb_set_correction",
+    "nb_tree_quadrupoles", "nb_correction_pairs",
 ]
+
+MLP_IN, MLP_OUT, MLP_PARAMS = 13, 3, 9411
+
+
+class TrainOpts(ctypes.Structure):
+    _fields_ = [("epochs", ctypes.c_uint32), ("batch", ctypes.c_uint32), ("lr", ctypes.c_float),
+                ("lr_final", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("eps", ctypes.c_float), ("augment", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64)]
 
 
 class Params(ctypes.Structure):
@@ -58,6 +71,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
+_fp = ctypes.POINTER(ctypes.c_float)
 _lib = None
 
 
@@ -100,6 +114,22 @@ def load(path=LIB_PATH):
         "nb_energy": (ctypes.c_int, [vp] + [_dp] * 4),
         "nb_gen_star_disk": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
         "nb_gen_plummer": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint64] + [_dp] * 7),
+        "nb_mlp_default_train_opts": (None, [ctypes.POINTER(TrainOpts)]),
+        "nb_mlp_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_uint64]),
+        "nb_mlp_destroy": (ctypes.c_int, [vp]),
+        "nb_mlp_last_error": (ctypes.c_char_p, [vp]),
+        "nb_mlp_set_params": (ctypes.c_int, [vp, _fp]),
+        "nb_mlp_get_params": (ctypes.c_int, [vp, _fp]),
+        "nb_mlp_forward": (ctypes.c_int, [vp, ctypes.c_uint64, _fp, _fp]),
+        "nb_mlp_grad": (ctypes.c_int, [vp, ctypes.c_uint32, _fp, _fp, _fp, _fp]),
+        "nb_mlp_adam_step": (ctypes.c_int, [vp, _fp, ctypes.c_float, ctypes.c_uint32]),
+        "nb_mlp_train": (ctypes.c_int, [vp, ctypes.c_uint64, _fp, _fp, ctypes.POINTER(TrainOpts),
+                                        _fp]),
+        "nb_mlp_pairs": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_uint32, _u32p,
+                                        ctypes.c_uint64, _u32p, _u32p, _fp, _fp, _u64p]),
+        "nb_set_correction": (ctypes.c_int, [vp, vp, ctypes.c_double]),
+        "nb_tree_quadrupoles": (ctypes.c_uint64, [vp, ctypes.c_uint64, _dp]),
+        "nb_correction_pairs": (ctypes.c_int, [vp, _u64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -290,3 +320,118 @@ class Context:
         if k != cap:
             raise NBodyError(-1, "nb_tree_canonical", f"returned {k}, expected {cap}")
         return out
+
+    # ---- learned far-field correction --------------------------------------------
+    def tree_quadrupoles(self):
+        """Quadrupole moments of every node in canonical order, shape (nodes, 6)."""
+        cap = self.tree_info()["node_count"]
+        q = np.zeros((cap, 6))
+        k = self.lib.nb_tree_quadrupoles(self.h, cap, _d(q))
+        if k != cap:
+            raise NBodyError(-1, "nb_tree_quadrupoles", f"returned {k}, expected {cap}: "
+                             + self.lib.nb_last_error(self.h).decode())
+        return q
+
+    def mlp_pairs(self, theta, targets, cap):
+        """Training pairs of the current octree: dict(cell, body, features, residuals, found)."""
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        cell = np.zeros(cap, np.uint32)
+        body = np.zeros(cap, np.uint32)
+        F = np.zeros((cap, MLP_IN), np.float32)
+        T = np.zeros((cap, MLP_OUT), np.float32)
+        found = ctypes.c_uint64()
+        self._check(self.lib.nb_mlp_pairs(self.h, theta, t.size, t.ctypes.data_as(_u32p), cap,
+                                          cell.ctypes.data_as(_u32p), body.ctypes.data_as(_u32p),
+                                          F.ctypes.data_as(_fp), T.ctypes.data_as(_fp),
+                                          ctypes.byref(found)), "nb_mlp_pairs")
+        m = min(found.value, cap)
+        return dict(cell=cell[:m], body=body[:m], features=F[:m], residuals=T[:m],
+                    found=found.value)
+
+    def correction_pairs(self):
+        out = ctypes.c_uint64()
+        self._check(self.lib.nb_correction_pairs(self.h, ctypes.byref(out)), "nb_correction_pairs")
+        return out.value
+
+    def set_correction(self, mlp, tau=0.0):
+        """Attach a trained Mlp (None detaches) to the FP32C Barnes-Hut traversal."""
+        self._corr = mlp  # the network must outlive the attachment
+        self._check(self.lib.nb_set_correction(self.h, mlp.h if mlp is not None else None, tau),
+                    "nb_set_correction")
+
+
+def default_train_opts(**kw):
+    o = TrainOpts()
+    load().nb_mlp_default_train_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Mlp:
+    """The correction network (weights on the current HIP device)."""
+
+    def __init__(self, seed=1):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        rc = self.lib.nb_mlp_create(ctypes.byref(h), seed)
+        if rc:
+            raise NBodyError(rc, "nb_mlp_create", strerror(rc))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise NBodyError(rc, what, self.lib.nb_mlp_last_error(self.h).decode())
+
+    @property
+    def params(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.lib.nb_mlp_get_params(self.h, p.ctypes.data_as(_fp)), "nb_mlp_get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == MLP_PARAMS
+        self._check(self.lib.nb_mlp_set_params(self.h, p.ctypes.data_as(_fp)), "nb_mlp_set_params")
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty((X.shape[0], MLP_OUT), np.float32)
+        self._check(self.lib.nb_mlp_forward(self.h, X.shape[0], X.ctypes.data_as(_fp),
+                                            Y.ctypes.data_as(_fp)), "nb_mlp_forward")
+        return Y
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.nb_mlp_grad(self.h, X.shape[0], X.ctypes.data_as(_fp),
+                                         T.ctypes.data_as(_fp), g.ctypes.data_as(_fp),
+                                         ctypes.byref(loss)), "nb_mlp_grad")
+        return g, loss.value
+
+    def adam_step(self, grad, lr, t):
+        g = np.ascontiguousarray(grad, dtype=np.float32)
+        self._check(self.lib.nb_mlp_adam_step(self.h, g.ctypes.data_as(_fp), lr, t),
+                    "nb_mlp_adam_step")
+
+    def train(self, X, T, **opts):
+        """Train on the GPU; returns the mean batch loss of each epoch."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        o = default_train_opts(**opts)
+        losses = np.zeros(o.epochs, np.float32)
+        self._check(self.lib.nb_mlp_train(self.h, X.shape[0], X.ctypes.data_as(_fp),
+                                          T.ctypes.data_as(_fp), ctypes.byref(o),
+                                          losses.ctypes.data_as(_fp)), "nb_mlp_train")
+        return losses

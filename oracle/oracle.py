@@ -72,6 +72,13 @@ def _load():
                                  + [ctypes.c_double] * 3 + [ctypes.c_int] * 3 + [_dp] * 3
                                  + [_i32p, _dp, _dp])
     lib.orc_simulate.restype = ctypes.c_int
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.orc_mlp_forward.argtypes = [_fp, ctypes.c_uint32, _fp, _fp, ctypes.c_int]
+    lib.orc_mlp_grad.argtypes = [_fp, ctypes.c_uint32, _fp, _fp, _dp, _dp, ctypes.c_int]
+    lib.orc_mlp_adam.argtypes = ([ctypes.c_uint32, _fp, _fp, _fp, _dp] + [ctypes.c_float] * 4
+                                 + [ctypes.c_uint32])
+    lib.orc_mlp_pair.argtypes = ([ctypes.c_uint32] + [_dp] * 5 + [ctypes.c_double] * 4
+                                 + [_dp, _fp, _dp])
     return lib
 
 
@@ -272,3 +279,51 @@ def simulate(algorithm, x, y, z, vx, vy, vz, m, dt, t_end, theta=1.05, storage_p
         raise OracleError(rc, "orc_simulate")
     return dict(x=st[0], y=st[1], z=st[2], vx=st[3], vy=st[4], vz=st[5], ax=acc[0], ay=acc[1],
                 az=acc[2], steps=steps.value, loop_ms=loop.value, phase_ms=ph)
+
+
+# ---- learned far-field correction (nbody_mlp_oracle.h) -------------------------
+MLP_IN, MLP_HIDDEN, MLP_OUT, MLP_PARAMS = 13, 64, 3, 9411
+_fp32p = ctypes.POINTER(ctypes.c_float)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def mlp_forward(params, X, bf16=True):
+    """Y[count, 3] of the 13-64-64-64-3 SiLU network (FP64 math, BF16 operand
+    rounding where the GPU rounds when bf16 is true)."""
+    p, X = _f32(params), _f32(X)
+    Y = np.empty((X.shape[0], MLP_OUT), np.float32)
+    lib().orc_mlp_forward(p.ctypes.data_as(_fp32p), X.shape[0], X.ctypes.data_as(_fp32p),
+                          Y.ctypes.data_as(_fp32p), int(bf16))
+    return Y
+
+
+def mlp_grad(params, X, T, bf16=True):
+    """(grad[9411] float64, loss) of L = mean (y - t)^2."""
+    p, X, T = _f32(params), _f32(X), _f32(T)
+    g = np.empty(MLP_PARAMS)
+    loss = ctypes.c_double(0)
+    lib().orc_mlp_grad(p.ctypes.data_as(_fp32p), X.shape[0], X.ctypes.data_as(_fp32p),
+                       T.ctypes.data_as(_fp32p), _d(g), ctypes.byref(loss), int(bf16))
+    return g, loss.value
+
+
+def mlp_adam(w, m, v, grad, lr, t, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays w, m, v."""
+    g = _c64(grad)
+    lib().orc_mlp_adam(w.size, w.ctypes.data_as(_fp32p), m.ctypes.data_as(_fp32p),
+                       v.ctypes.data_as(_fp32p), _d(g), lr, beta1, beta2, eps, t)
+
+
+def mlp_pair(x, y, z, m, centre, edge, m_root, G, eps2, p):
+    """(features[13] float32, target[3] float64) of one (cell, target) pair,
+    by brute force over the cell's bodies."""
+    x, y, z, m = _c64(x), _c64(y), _c64(z), _c64(m)
+    c, q = _c64(centre), _c64(p)
+    f = np.empty(13, np.float32)
+    t = np.empty(3)
+    lib().orc_mlp_pair(m.size, _d(x), _d(y), _d(z), _d(m), _d(c), edge, m_root, G, eps2, _d(q),
+                       f.ctypes.data_as(_fp32p), _d(t))
+    return f, t
