@@ -36,6 +36,12 @@ MLP_TRAIN = dict(snapshots=4, n=262144, targets=1800, pairs=2_000_000, theta=1.0
                  batch=4096)
 MLP_PLAIN_THETA = 0.4
 
+# Ensemble: 512 independent systems, sizes floor(2^U[10, 16]), alternating
+# star + disk (dt = 1 h) and Plummer (dt = 1e-3); direct sum below the
+# autotuned threshold, Barnes-Hut theta = 0.5 above it.
+ENSEMBLE = dict(nsys=512, seed=0x4E424F44590004, lo_log2=10, hi_log2=16, theta=0.5,
+                err_systems=32, err_targets=64, cpu_systems=12)
+
 # HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this script scaled by a device copy of known size
 # (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0).
@@ -117,7 +123,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS) + ["ensemble_512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration)")
@@ -129,6 +135,9 @@ def main():
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for "
               f"--gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ensemble_512":
+        run_ensemble(args)
+        return
     kind, n, seed, algorithm, theta, dt, prec = WORKLOADS[args.workload]
     mlp = args.workload.endswith("_mlp")
     log(f"device: {nb.device_name()}")
@@ -188,6 +197,106 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, algorithm, theta, dt)
     print(json.dumps(out), flush=True)
+
+
+def run_ensemble(args):
+    """512 independent systems in one persistent force launch per step."""
+    c = ENSEMBLE
+    log(f"device: {nb.device_name()}")
+    t0 = time.perf_counter()
+    sizes, st, dts = nb.gen_ensemble(c["nsys"], c["seed"], c["lo_log2"], c["hi_log2"])
+    n = int(sizes.sum())
+    log(f"ensemble: {c['nsys']} systems, {n} bodies (generated in {time.perf_counter() - t0:.1f} s)")
+    e = nb.Ensemble(sizes, theta=c["theta"])
+    keys = ("x", "y", "z", "vx", "vy", "vz", "m")
+
+    def reset():
+        e.set_state(*[st[k] for k in keys], dts)
+        e.forces()  # t = 0 forces, untimed as in the reference
+
+    reset()
+    t1 = time.perf_counter()
+    tune, tune_ms = e.autotune(steps=2)
+    log(f"autotune: {tune} at {tune_ms:.3f} ms/step ({time.perf_counter() - t1:.1f} s)")
+    reset()
+    if args.warmup:
+        e.steps(args.warmup)
+    log(f"timing {args.steps} steps ...")
+    t0 = time.perf_counter()
+    e.steps(args.steps)
+    secs = time.perf_counter() - t0
+    phases = e.phase_ms()
+    value = n * args.steps / secs
+    inter = e.interactions().astype(np.float64)
+    uses_bh = e.uses_bh()
+    # error: sampled targets of sampled systems vs the FP64 direct sum
+    x, y, z, vx, vy, vz = e.state()
+    ax, ay, az = e.accel()
+    off = np.concatenate([[0], np.cumsum(sizes, dtype=np.int64)])
+    rng = np.random.default_rng(1)
+    errs = []
+    for s in rng.choice(c["nsys"], c["err_systems"], replace=False):
+        sl = slice(off[s], off[s + 1])
+        ns = int(sizes[s])
+        ctx = nb.Context(ns)
+        ctx.set_state(x[sl], y[sl], z[sl], vx[sl], vy[sl], vz[sl], st["m"][sl])
+        t = np.sort(rng.choice(ns, min(ns, c["err_targets"]), replace=False)).astype(np.uint32)
+        rx, ry, rz = ctx.forces_direct_targets(t)
+        ctx.close()
+        gx, gy, gz = ax[sl][t], ay[sl][t], az[sl][t]
+        errs.append(np.sqrt((gx - rx) ** 2 + (gy - ry) ** 2 + (gz - rz) ** 2) /
+                    np.sqrt(rx ** 2 + ry ** 2 + rz ** 2))
+    errs = np.concatenate(errs)
+    err = float(np.median(errs))
+    log(f"median rel err {err:.3e} over {errs.size} targets of {c['err_systems']} systems")
+    accel_ms = phases["Acceleration Kernel Time"] / args.steps
+    flops = FLOP_PER_INTERACTION * float(inter.sum())
+    achieved = flops / (accel_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": "ensemble_512", "systems": c["nsys"], "bodies": n,
+                   "sizes": f"floor(2^U[{c['lo_log2']},{c['hi_log2']}])",
+                   "distribution": "alternating star+disk (dt 1 h) and plummer (dt 1e-3)",
+                   "theta": c["theta"], "tuning": tune, "tuned_ms_per_step": round(tune_ms, 4),
+                   "bh_systems": int(uses_bh.sum()), "median_rel_err": err,
+                   "err_sample": int(errs.size),
+                   "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
+                   "mean_interactions": float(inter.mean())},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": TRAFFIC_BYTES.get("ensemble_512"), "kernel": "k_ens_forces",
+                     "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops},
+    }
+    e.close()
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = ensemble_cpu_baseline(sizes, st, dts, uses_bh, off)
+    print(json.dumps(out), flush=True)
+
+
+def ensemble_cpu_baseline(sizes, st, dts, uses_bh, off):
+    """The reference CPU path (oracle restatement) runs one system at a time:
+    one loop iteration of a sample of the systems, Barnes-Hut or naive as the
+    tuned split chose, timed and scaled to body-steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    c = ENSEMBLE
+    pick = np.random.default_rng(2).choice(c["nsys"], c["cpu_systems"], replace=False)
+    bodies, ms = 0, 0.0
+    for s in pick:
+        sl = slice(off[s], off[s + 1])
+        algo = "BarnesHut" if uses_bh[s] else "naive"
+        r = orc.simulate(algo, st["x"][sl], st["y"][sl], st["z"][sl], st["vx"][sl], st["vy"][sl],
+                         st["vz"][sl], st["m"][sl], dts[s], dts[s], theta=c["theta"])
+        bodies += int(sizes[s]) * r["steps"]
+        ms += r["loop_ms"]
+    return {"value": bodies / (ms * 1e-3), "unit": "body-steps/s", "cores": orc.get_threads(),
+            "kind": "port",
+            "sample": f"1 loop iteration of {c['cpu_systems']} of the systems ({bodies} body-steps), "
+                      f"FP64, naive below the tuned threshold and Barnes-Hut above; "
+                      f"{ms / 1e3:.1f} s"}
 
 
 def timed_run(ctx, d, algorithm, dt, steps, warmup):

@@ -243,6 +243,66 @@ int nb_correction_pairs(nb_ctx *ctx, uint64_t *pairs);
  * record count (0 if cap is too small). */
 uint64_t nb_tree_quadrupoles(nb_ctx *ctx, uint64_t cap, double *q);
 
+/* ---- ensembles of independent systems -------------------------------------
+ * No counterpart in the reference, which runs one system per process.  Many
+ * systems of uneven size advance together: the states are concatenated
+ * (system s holds bodies [sum of sizes[0..s), + sizes[s]) of every array),
+ * each system has its own time step, and one loop iteration of the whole
+ * ensemble is one kick-drift, one octree build of every Barnes-Hut system
+ * (one workgroup per system), one persistent force launch over a work queue
+ * of traversal groups and direct-sum tiles, and one kick -- the reference's
+ * loop body (NaiveAlgorithm.cpp:131-259, BarnesHutAlgorithm.cpp:149-277)
+ * for every system at once.  Forces are FP32 pair arithmetic with FP64
+ * accumulation (NB_PREC_FP32C); octree, integrator and sums are FP64.
+ * Systems with fewer than bh_threshold bodies use the direct sum. */
+typedef struct nb_ensemble nb_ensemble;
+
+typedef struct nb_ensemble_tuning {
+    uint32_t bh_threshold; /* systems with n >= bh_threshold use Barnes-Hut */
+    uint32_t waves;        /* persistent wavefronts of the force launch */
+} nb_ensemble_tuning;
+
+/* p->precision must be NB_PREC_FP32C; G, eps2, theta and storage_size_param
+ * apply to every system.  At most 65535 systems, 2^32 bodies in all. */
+int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
+                       const nb_params *p);
+int nb_ensemble_destroy(nb_ensemble *e);
+const char *nb_ensemble_last_error(const nb_ensemble *e);
+uint64_t nb_ensemble_body_count(const nb_ensemble *e);
+/* Concatenated host arrays; dt has one entry per system. */
+int nb_ensemble_set_state(nb_ensemble *e, const double *x, const double *y, const double *z,
+                          const double *vx, const double *vy, const double *vz, const double *m,
+                          const double *dt);
+int nb_ensemble_get_state(nb_ensemble *e, double *x, double *y, double *z, double *vx, double *vy,
+                          double *vz);
+int nb_ensemble_get_accel(nb_ensemble *e, double *ax, double *ay, double *az);
+/* Forces at the current positions (the t = 0 evaluation). */
+int nb_ensemble_forces(nb_ensemble *e);
+/* `count` loop iterations of every system, one host synchronisation. */
+int nb_ensemble_steps(nb_ensemble *e, uint32_t count);
+/* GPU time per phase of the last call (NB_PHASE_OCTREE, _ACCEL, _LF1, _LF2). */
+int nb_ensemble_phase_ms(const nb_ensemble *e, double *ms);
+/* Per body, last force evaluation: accepted cells and bodies (Barnes-Hut
+ * systems) or source bodies (direct-sum systems). */
+int nb_ensemble_interactions(nb_ensemble *e, uint32_t *count);
+int nb_ensemble_set_tuning(nb_ensemble *e, const nb_ensemble_tuning *t);
+int nb_ensemble_get_tuning(const nb_ensemble *e, nb_ensemble_tuning *t);
+/* flags[s] = 1 if system s uses Barnes-Hut under the current tuning. */
+int nb_ensemble_uses_bh(const nb_ensemble *e, uint8_t *flags);
+/* Octree of Barnes-Hut system s from the last force evaluation: node count,
+ * and the canonical dump in the format of nb_tree_canonical (body indices
+ * local to the system).  0 if s is not a Barnes-Hut system or cap is small. */
+uint64_t nb_ensemble_node_count(nb_ensemble *e, uint32_t s);
+uint64_t nb_ensemble_tree_canonical(nb_ensemble *e, uint32_t s, uint64_t cap, uint8_t *depth,
+                                    uint8_t *kind, uint32_t *body, double *mass, double *com_x,
+                                    double *com_y, double *com_z, uint32_t *count);
+/* Autotuner: times `steps` loop iterations from the current state for each
+ * candidate direct-sum/Barnes-Hut threshold, then each persistent grid size
+ * at the best threshold; keeps the fastest and restores the state.  *best and
+ * *best_step_ms (either may be NULL) receive the choice and its time. */
+int nb_ensemble_autotune(nb_ensemble *e, uint32_t steps, nb_ensemble_tuning *best,
+                         double *best_step_ms);
+
 /* ---- synthetic inputs (seeded; same bits on every machine) -------------- */
 /* One star of 1 Msun at rest at the origin plus n-1 bodies on Kepler orbits:
  * a ~ U[2.1, 3.3] AU, e ~ U[0, 0.3], i ~ U[0, 20 deg], angles ~ U[0, 2pi),

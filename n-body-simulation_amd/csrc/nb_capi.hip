@@ -678,41 +678,9 @@ uint64_t nb_tree_canonical(nb_ctx *c, uint64_t cap, uint8_t *depth, uint8_t *kin
     ok = ok && hipMemcpy(hz.data(), c->z, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
     ok = ok && hipMemcpy(hm.data(), c->m, (size_t)n * 8, hipMemcpyDeviceToHost) == hipSuccess;
     if (!ok) return 0;
-    std::vector<std::pair<uint64_t, uint32_t>> stack{{0, 0}};
-    uint64_t k = 0;
-    while (!stack.empty()) {
-        const auto [node, d] = stack.back();
-        stack.pop_back();
-        if (k >= cap) return 0;
-        const int32_t r = ref[node];
-        depth[k] = (uint8_t)d;
-        if (r >= 0) {
-            kind[k] = 2;
-            body[k] = n;
-            mass[k] = sums[r].m;
-            com_x[k] = sums[r].x;
-            com_y[k] = sums[r].y;
-            com_z[k] = sums[r].z;
-            count[k] = cnt[r];
-            for (int dg = 7; dg >= 0; --dg) stack.push_back({1ull + 8ull * (uint64_t)r + dg, d + 1});
-        } else if (r != nb::kRefEmpty) {
-            const uint32_t b = nb::body_of_ref(r);
-            kind[k] = 1;
-            body[k] = b;
-            mass[k] = hm[b];
-            com_x[k] = hx[b] * hm[b];
-            com_y[k] = hy[b] * hm[b];
-            com_z[k] = hz[b] * hm[b];
-            count[k] = 1;
-        } else {
-            kind[k] = 0;
-            body[k] = n;
-            mass[k] = com_x[k] = com_y[k] = com_z[k] = 0.0;
-            count[k] = 0;
-        }
-        ++k;
-    }
-    return k;
+    return nb::canonical_dump(ref.data(), sums.data(), cnt.data(), hx.data(), hy.data(), hz.data(),
+                              hm.data(), n, cap, depth, kind, body, mass, com_x, com_y, com_z,
+                              count);
 }
 
 // ---- learned far-field correction: context side ---------------------------------

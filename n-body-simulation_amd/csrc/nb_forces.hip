@@ -303,6 +303,16 @@ __device__ inline uint64_t lanes_below() {
     return lane == 0 ? 0ull : (~0ull >> (64u - lane));
 }
 
+// A value every lane of the wave holds alike, made visibly uniform (SGPR).
+// The walks below are wave-collective (ballots, shuffles, one LDS list per
+// wave): every branch that steers them must be uniform to the compiler, or
+// control-flow structurisation may let parts of the wave run different
+// iterations with partial exec masks.
+__device__ inline uint32_t wave_uniform(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// lane 0's value of v, uniform
+__device__ inline uint32_t lane0_uniform(uint32_t v) { return wave_uniform(__shfl(v, 0, 64)); }
+
 // With the learned correction (kCorr), accepted cells that are close enough
 // (e^2 > tau2 * d_group^2) are also queued in a second LDS list; every 16 such
 // cells, the wave evaluates the network for the 16 cells x each body of the
@@ -311,55 +321,59 @@ __device__ inline uint64_t lanes_below() {
 constexpr int kCorrTile = 16;
 constexpr int kCellCap = kCorrTile + kGroup;
 
-template <bool kCorr>
-__global__ void __launch_bounds__(kGroup) k_bh_group(
-    uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
-    float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
-    TreeHdr *h, CorrArgs corr) {
-    __shared__ uint32_t stk[kStackCap];
-    __shared__ float4 lst[kListCap];
-    __shared__ float4 cl_e[kCorr ? kCellCap : 1];   // accepted cell: offset from group centre, G*M
-    __shared__ uint32_t cl_k[kCorr ? kCellCap : 1]; // accepted cell: internal index
-    __shared__ float4 boff[kCorr ? kGroup : 1];     // body offsets from the group centre
-    __shared__ float4 bcorr[kCorr ? kGroup : 1];    // learned correction per body
-    const uint32_t lane = threadIdx.x;
-    const uint64_t below = lanes_below();
-    const uint32_t ngroups = min(h->group_count, n);
-    // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
-    const double ro_root = h->edge / theta;
-    const float ro2_root = (float)(ro_root * ro_root);
-    const float e2_root = (float)(h->edge * h->edge);
-    MlpForward w;
-    if constexpr (kCorr) w.load(corr.pack, corr.master, (int)lane);
-    uint32_t peak = 0;
-    bool overflow = false;
+// One octree as the group walk sees it (body indices of sidx and of the node
+// references index x, y, z, m, ax, ay, az, inter).
+struct GroupTree {
+    uint32_t n;
+    const uint32_t *sidx;
+    const double *x, *y, *z, *m;
+    const NodeF32 *rec;
+    uint32_t icap;
+    float ro2_root;  // (root edge / theta)^2
+    float e2_root;   // root edge^2
+    double *ax, *ay, *az;
+    uint32_t *inter;
+};
 
-    // One queue over the octree-ordered group list: waves running at the same
-    // time walk neighbouring groups, whose interaction lists overlap, so the
-    // records they read are shared in L2.  (Splitting the list into one chunk
-    // per XCD cut HBM fetches 3x but lengthened the kernel by 40%: the tail
-    // of the slowest chunk dominates.)
-    while (!overflow) {
-        uint32_t slot = 0;
-        if (lane == 0)
-            slot = __hip_atomic_fetch_add(&h->group_next, 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t g = __shfl(slot, 0, 64);
-        if (g >= ngroups) break;
-        const uint2 grp = groups[g];
-        if (grp.y == 0 || grp.y > (uint32_t)kGroup || grp.x >= n || grp.y > n - grp.x) {
-            overflow = true;  // corrupt group table: report instead of reading out of bounds
-            break;
-        }
-        const uint32_t s = grp.y;
+// LDS of one wave's walk.
+struct GroupLds {
+    uint32_t *stk;
+    float4 *lst;
+    float4 *cl_e;    // correction: accepted cell offset from the group centre, G*M
+    uint32_t *cl_k;  // correction: accepted cell's internal index
+    float4 *boff;    // correction: body offsets from the group centre
+    float4 *bcorr;   // correction: learned correction per body
+};
+
+// Forces on the bodies of one group grp = {first sorted position, count}
+// (checked by the caller).  *ncorr_out accumulates network evaluations.
+template <bool kCorr>
+__device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float eps2,
+                                           const CorrArgs &corr, const MlpForward &w,
+                                           const GroupLds &L, uint32_t &peak, bool &overflow,
+                                           uint32_t &ncorr_out) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = lanes_below();
+    const uint32_t *__restrict__ sidx = T.sidx;
+    const double *__restrict__ x = T.x, *__restrict__ y = T.y, *__restrict__ z = T.z,
+                 *__restrict__ m = T.m;
+    const NodeF32 *__restrict__ rec = T.rec;
+    const uint32_t icap = T.icap;
+    const float ro2_root = T.ro2_root, e2_root = T.e2_root;
+    double *ax = T.ax, *ay = T.ay, *az = T.az;
+    uint32_t *inter = T.inter;
+    uint32_t *stk = L.stk;
+    float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
+    uint32_t *cl_k = L.cl_k;
+    {
+        const uint32_t g0 = wave_uniform(grp.x);
+        const uint32_t s = wave_uniform(grp.y);
         const uint32_t s2 = s <= 1 ? 1u : 1u << (32 - __clz(s - 1));
         const uint32_t sub = lane & (s2 - 1);  // which body of the group
         const uint32_t q = lane / s2;          // which share of the list
         const uint32_t lpb = kGroup / s2;      // lanes per body
         const bool valid = sub < s;
-        const uint32_t bi = sidx[grp.x + (valid ? sub : 0)];
+        const uint32_t bi = sidx[g0 + (valid ? sub : 0)];
         const double px = x[bi], py = y[bi], pz = z[bi];
 
         // bounding box (FP64, exact min/max), centre and half extent
@@ -477,12 +491,13 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             float4 e;
             float d2r;
             const NodeF32 root = rec[0];
-            const int cls = classify(root, 0, e, d2r);  // every lane evaluates the root alike
+            // every lane evaluates the root alike
+            const int cls = (int)wave_uniform((uint32_t)classify(root, 0, e, d2r));
             if (cls == 1) {
                 if (lane == 0) lst[0] = e;
                 ln = 1;
                 if constexpr (kCorr) {
-                    if (root.ref >= 0 && e2_root > corr.tau2 * d2r) {
+                    if (wave_uniform(root.ref >= 0 && e2_root > corr.tau2 * d2r)) {
                         if (lane == 0) {
                             cl_e[0] = e;
                             cl_k[0] = (uint32_t)root.ref;
@@ -588,10 +603,12 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             acc_y += __shfl_xor(acc_y, off, 64);
             acc_z += __shfl_xor(acc_z, off, 64);
         }
-        if constexpr (kCorr) {
-            if (lane == 0 && ncorr) atomicAdd(&h->corr_pairs, (unsigned long long)ncorr);
-        }
-        if (valid && q == 0 && !overflow) {
+        ncorr_out += ncorr;
+        // Stored also after an overflow (the caller reports it and the values
+        // are then undefined): a predicate holding `overflow` would let the
+        // compiler route the storing lanes past the caller's exit test, a
+        // per-lane split of the wave's loop.
+        if (valid && q == 0) {
             if constexpr (kCorr) {
                 const float4 c = bcorr[sub];
                 acc_x += (double)c.x;
@@ -604,8 +621,170 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             if (inter) inter[bi] = n_inter;
         }
     }
+}
+
+// a group {first, count} that a walk may trust
+__device__ inline bool group_ok(uint2 grp, uint32_t n) {
+    return grp.y != 0 && grp.y <= (uint32_t)kGroup && grp.x < n && grp.y <= n - grp.x;
+}
+
+template <bool kCorr>
+__global__ void __launch_bounds__(kGroup) k_bh_group(
+    uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
+    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
+    float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
+    TreeHdr *h, CorrArgs corr) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ float4 lst[kListCap];
+    __shared__ float4 cl_e[kCorr ? kCellCap : 1];
+    __shared__ uint32_t cl_k[kCorr ? kCellCap : 1];
+    __shared__ float4 boff[kCorr ? kGroup : 1];
+    __shared__ float4 bcorr[kCorr ? kGroup : 1];
+    const GroupLds L{stk, lst, cl_e, cl_k, boff, bcorr};
+    const uint32_t lane = threadIdx.x;
+    const uint32_t ngroups = min(h->group_count, n);
+    // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
+    const double ro_root = h->edge / theta;
+    const GroupTree T{n, sidx, x, y, z, m, rec, icap, (float)(ro_root * ro_root),
+                      (float)(h->edge * h->edge), ax, ay, az, inter};
+    MlpForward w;
+    if constexpr (kCorr) w.load(corr.pack, corr.master, (int)lane);
+    uint32_t peak = 0, ncorr = 0;
+    bool overflow = false;
+
+    // One queue over the octree-ordered group list: waves running at the same
+    // time walk neighbouring groups, whose interaction lists overlap, so the
+    // records they read are shared in L2.  (Splitting the list into one chunk
+    // per XCD cut HBM fetches 3x but lengthened the kernel by 40%: the tail
+    // of the slowest chunk dominates.)
+    for (;;) {
+        uint32_t slot = 0;
+        if (lane == 0)
+            slot = __hip_atomic_fetch_add(&h->group_next, 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t g = lane0_uniform(slot);
+        if (g >= ngroups) break;
+        const uint2 grp = groups[g];
+        if (!group_ok(grp, n)) {
+            overflow = true;  // corrupt group table: report instead of reading out of bounds
+            break;
+        }
+        group_walk<kCorr>(T, grp, eps2, corr, w, L, peak, overflow, ncorr);
+        if (overflow) break;
+    }
+    if constexpr (kCorr) {
+        if (lane == 0 && ncorr) atomicAdd(&h->corr_pairs, (unsigned long long)ncorr);
+    }
     if (overflow) raise_flag(h, kFlagStack);
     if (lane == 0) atomicMax(&h->stack_peak, peak);
+}
+
+// ---- ensemble of independent systems -------------------------------------------
+__global__ void __launch_bounds__(64) k_ens_items(uint32_t nq, const uint32_t *__restrict__ qorder,
+                                                  const uint32_t *__restrict__ items,
+                                                  uint32_t *item_off, uint32_t *cursor,
+                                                  uint32_t *status) {
+    if (threadIdx.x != 0) return;
+    uint32_t run = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        item_off[q] = run;
+        run += items[qorder[q]];
+    }
+    item_off[nq] = run;
+    *cursor = 0;
+    (void)status;
+}
+
+// Direct sum for targets [tile*64, tile*64+64) of one system: FP32 pairs on
+// offsets from the tile's first body (FP64 subtraction, so nearby pairs keep
+// their precision; a body's own offset equals its source offset, so the self
+// pair adds exactly 0), FP64 sum of each 64-source chunk.
+__device__ inline void direct_tile(const double *__restrict__ x, const double *__restrict__ y,
+                                   const double *__restrict__ z, const double *__restrict__ m,
+                                   uint32_t n, uint32_t tile, double G, float eps2, double *ax,
+                                   double *ay, double *az, float4 *stage) {
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t t0 = tile * kEnsTile;
+    const uint32_t i = t0 + lane;
+    const bool valid = i < n;
+    const uint32_t ii = valid ? i : t0;
+    const double rx = x[t0], ry = y[t0], rz = z[t0];
+    const float px = (float)(x[ii] - rx), py = (float)(y[ii] - ry), pz = (float)(z[ii] - rz);
+    double acc_x = 0, acc_y = 0, acc_z = 0;
+    for (uint32_t c = 0; c < n; c += kEnsTile) {
+        const uint32_t j = c + lane;
+        float4 sv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < n)
+            sv = make_float4((float)(x[j] - rx), (float)(y[j] - ry), (float)(z[j] - rz),
+                             (float)(G * m[j]));
+        __syncthreads();
+        stage[lane] = sv;
+        __syncthreads();
+        const uint32_t cnt = min(kEnsTile, n - c);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (uint32_t k = 0; k < cnt; ++k) pair_f32(stage[k], px, py, pz, eps2, fx, fy, fz);
+        acc_x += (double)fx;
+        acc_y += (double)fy;
+        acc_z += (double)fz;
+    }
+    if (valid) {
+        ax[i] = acc_x;
+        ay[i] = acc_y;
+        az[i] = acc_z;
+    }
+}
+
+// Persistent: every wave takes items from the queue until it is empty.
+__global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
+    __shared__ uint32_t stk[kStackCap];
+    __shared__ float4 lst[kListCap];
+    const GroupLds L{stk, lst, nullptr, nullptr, nullptr, nullptr};
+    const uint32_t lane = threadIdx.x;
+    const uint32_t total = a.item_off[a.nq];
+    const MlpForward w{};
+    const CorrArgs corr{};
+    uint32_t peak = 0, ncorr = 0;
+    bool overflow = false;
+    for (;;) {
+        uint32_t slot = 0;
+        if (lane == 0)
+            slot = __hip_atomic_fetch_add(a.cursor, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t it = lane0_uniform(slot);
+        if (it >= total) break;
+        // queue position q with item_off[q] <= it < item_off[q + 1]
+        uint32_t lo = 0, hi = a.nq;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.item_off[mid] <= it) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t s = a.qorder[lo];
+        const EnsSystem S = a.sys[s];
+        const uint32_t local = it - a.item_off[lo];
+        const uint32_t bo = S.body_off;
+        if (S.bh) {
+            const double edge = a.fb.hdr[s].edge;
+            const double ro = edge / a.theta;
+            const GroupTree T{S.n, a.fb.sidx + S.fb_off, a.x + bo, a.y + bo, a.z + bo, a.m + bo,
+                              a.fb.rec + S.node_off, S.icap, (float)(ro * ro),
+                              (float)(edge * edge), a.ax + bo, a.ay + bo, a.az + bo,
+                              a.inter + bo};
+            const uint2 grp = a.fb.groups[S.fb_off + local];
+            if (!group_ok(grp, S.n)) {
+                overflow = true;
+                break;
+            }
+            group_walk<false>(T, grp, a.eps2, corr, w, L, peak, overflow, ncorr);
+            if (overflow) break;
+        } else {
+            direct_tile(a.x + bo, a.y + bo, a.z + bo, a.m + bo, S.n, local, a.G, a.eps2, a.ax + bo,
+                        a.ay + bo, a.az + bo, lst);
+            const uint32_t i = local * kEnsTile + lane;
+            if (i < S.n) a.inter[bo + i] = S.n;
+        }
+    }
+    if (overflow && lane == 0) atomicOr(a.status, kFlagStack);
 }
 
 }  // namespace
@@ -680,6 +859,16 @@ void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, c
                        (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
                        interactions, t.hdr, corr);
+}
+
+void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items,
+                      uint32_t *item_off, uint32_t *cursor, uint32_t *status, hipStream_t s) {
+    hipLaunchKernelGGL(k_ens_items, dim3(1), dim3(64), 0, s, nq, qorder, items, item_off, cursor,
+                       status);
+}
+
+void launch_ens_forces(const EnsForceArgs &a, uint32_t waves, hipStream_t s) {
+    hipLaunchKernelGGL(k_ens_forces, dim3(std::max<uint32_t>(1, waves)), dim3(kGroup), 0, s, a);
 }
 
 }  // namespace nb

@@ -12,14 +12,17 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// dt_sys != nullptr: an ensemble, body i advances by dt_sys[sys_of[i]]
 __global__ void __launch_bounds__(kThreads) k_kick_drift(
-    uint32_t n, double delta_t, const double *__restrict__ ax, const double *__restrict__ ay,
+    uint32_t n, double dt, const double *__restrict__ dt_sys, const uint16_t *__restrict__ sys_of,
+    const double *__restrict__ ax, const double *__restrict__ ay,
     const double *__restrict__ az, const double *__restrict__ vx, const double *__restrict__ vy,
     const double *__restrict__ vz, double *__restrict__ vhx, double *__restrict__ vhy,
     double *__restrict__ vhz, double *__restrict__ x, double *__restrict__ y,
     double *__restrict__ z) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
+    const double delta_t = dt_sys ? dt_sys[sys_of[i]] : dt;
     const double hx = vx[i] + ax[i] * (delta_t / 2.0);
     const double hy = vy[i] + ay[i] * (delta_t / 2.0);
     const double hz = vz[i] + az[i] * (delta_t / 2.0);
@@ -32,12 +35,14 @@ __global__ void __launch_bounds__(kThreads) k_kick_drift(
 }
 
 __global__ void __launch_bounds__(kThreads) k_kick(
-    uint32_t n, double delta_t, const double *__restrict__ ax, const double *__restrict__ ay,
+    uint32_t n, double dt, const double *__restrict__ dt_sys, const uint16_t *__restrict__ sys_of,
+    const double *__restrict__ ax, const double *__restrict__ ay,
     const double *__restrict__ az, const double *__restrict__ vhx,
     const double *__restrict__ vhy, const double *__restrict__ vhz, double *__restrict__ vx,
     double *__restrict__ vy, double *__restrict__ vz) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
+    const double delta_t = dt_sys ? dt_sys[sys_of[i]] : dt;
     vx[i] = vhx[i] + ax[i] * (delta_t / 2.0);
     vy[i] = vhy[i] + ay[i] * (delta_t / 2.0);
     vz[i] = vhz[i] + az[i] * (delta_t / 2.0);
@@ -49,15 +54,32 @@ void launch_kick_drift(uint32_t n, double dt, const double *ax, const double *ay
                        const double *az, const double *vx, const double *vy, const double *vz,
                        double *vhx, double *vhy, double *vhz, double *x, double *y, double *z,
                        hipStream_t s) {
-    hipLaunchKernelGGL(k_kick_drift, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt, ax,
-                       ay, az, vx, vy, vz, vhx, vhy, vhz, x, y, z);
+    hipLaunchKernelGGL(k_kick_drift, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt,
+                       (const double *)nullptr, (const uint16_t *)nullptr, ax, ay, az, vx, vy, vz,
+                       vhx, vhy, vhz, x, y, z);
 }
 
 void launch_kick(uint32_t n, double dt, const double *ax, const double *ay, const double *az,
                  const double *vhx, const double *vhy, const double *vhz, double *vx,
                  double *vy, double *vz, hipStream_t s) {
-    hipLaunchKernelGGL(k_kick, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt, ax, ay,
-                       az, vhx, vhy, vhz, vx, vy, vz);
+    hipLaunchKernelGGL(k_kick, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, dt,
+                       (const double *)nullptr, (const uint16_t *)nullptr, ax, ay, az, vhx, vhy,
+                       vhz, vx, vy, vz);
+}
+
+void launch_ens_kick_drift(uint32_t n, const double *dt_sys, const uint16_t *sys_of,
+                           const double *ax, const double *ay, const double *az, const double *vx,
+                           const double *vy, const double *vz, double *vhx, double *vhy,
+                           double *vhz, double *x, double *y, double *z, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick_drift, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, 0.0,
+                       dt_sys, sys_of, ax, ay, az, vx, vy, vz, vhx, vhy, vhz, x, y, z);
+}
+
+void launch_ens_kick(uint32_t n, const double *dt_sys, const uint16_t *sys_of, const double *ax,
+                     const double *ay, const double *az, const double *vhx, const double *vhy,
+                     const double *vhz, double *vx, double *vy, double *vz, hipStream_t s) {
+    hipLaunchKernelGGL(k_kick, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, 0.0, dt_sys,
+                       sys_of, ax, ay, az, vhx, vhy, vhz, vx, vy, vz);
 }
 
 }  // namespace nb

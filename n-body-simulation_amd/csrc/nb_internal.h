@@ -36,8 +36,9 @@ constexpr int32_t kRefEmpty = -1;  // empty leaf
 __host__ __device__ inline int32_t ref_of_body(uint32_t b) { return -2 - (int32_t)b; }
 __host__ __device__ inline uint32_t body_of_ref(int32_t r) { return (uint32_t)(-2 - r); }
 
-// Written by the kernels, read by the host after a build.
-struct TreeHdr {
+// Written by the kernels, read by the host after a build.  One 128-byte line
+// per header, so headers of an ensemble's systems never share a cache line.
+struct alignas(128) TreeHdr {
     double minx, miny, minz, maxx, maxy, maxz, edge;  // root cube (cubified AABB)
     uint32_t internal_count;                           // internal nodes I; nodes = 1 + 8 I
     uint32_t max_depth;                                // deepest leaf
@@ -126,6 +127,38 @@ struct TreeBufs {
     uint32_t scan_part_cap = 0;
 };
 
+// ---- ensemble of independent systems ------------------------------------------
+// One concatenated state (bodies of system s at [body_off, body_off + n)); a
+// Barnes-Hut system owns per-body forest entries [fb_off, fb_off + n),
+// internal cells [int_off, int_off + icap) and nodes [node_off, node_off +
+// 1 + 8 icap) of the forest arrays, each range starting on a fresh 64-entry
+// boundary so that no cache line holds two systems' data; its body indices in
+// sidx and node references are local to the system.
+struct EnsSystem {
+    uint32_t body_off, n;
+    uint32_t int_off, icap;
+    uint64_t node_off;
+    uint32_t fb_off;
+    uint32_t bh;     // 1: Barnes-Hut, 0: direct sum
+};
+constexpr uint32_t kEnsAlign = 64;
+
+struct ForestBufs {
+    TreeHdr *hdr;  // one per system
+    // per body (offset body_off)
+    uint64_t *skey, *skey_alt, *klo;
+    uint32_t *sidx, *sidx_alt, *chain, *body_parent, *group_size, *group_slot;
+    int32_t *lcp;
+    uint2 *groups;
+    // per internal cell (offset int_off)
+    uint8_t *int_depth, *int_local;
+    uint32_t *int_parent, *int_node, *int_nchild, *int_arrive, *int_count, *int_first;
+    InternalSums *int_sums;
+    // per node (offset node_off)
+    int32_t *ref;
+    NodeF32 *rec;
+};
+
 // ---- launchers (each .hip file defines the kernels it launches) -------------
 // nb_scan.hip
 // Exclusive scan of n u32 values (out may alias in); *total_dev receives the sum.
@@ -152,6 +185,14 @@ void launch_com(const TreeBufs &t, const double *x, const double *y, const doubl
 // f32: packed FP32 records and traversal groups for the group traversal.
 void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
                          const double *m, double G, bool f64, bool f32, hipStream_t s);
+
+// Octree of every Barnes-Hut system of an ensemble, one workgroup per system
+// (order: the BH systems' indices, largest first); items[s] receives the
+// system's traversal-group count.
+void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
+                         const double *x, const double *y, const double *z, const double *m,
+                         double G, const ForestBufs &fb, uint32_t *items, uint32_t *status,
+                         hipStream_t s);
 
 // nb_forces.hip
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
@@ -186,7 +227,89 @@ void launch_kick_drift(uint32_t n, double dt, const double *ax, const double *ay
 void launch_kick(uint32_t n, double dt, const double *ax, const double *ay, const double *az,
                  const double *vhx, const double *vhy, const double *vhz, double *vx,
                  double *vy, double *vz, hipStream_t s);
+// ensemble: body i advances by dt_sys[sys_of[i]]
+void launch_ens_kick_drift(uint32_t n, const double *dt_sys, const uint16_t *sys_of,
+                           const double *ax, const double *ay, const double *az, const double *vx,
+                           const double *vy, const double *vz, double *vhx, double *vhy,
+                           double *vhz, double *x, double *y, double *z, hipStream_t s);
+void launch_ens_kick(uint32_t n, const double *dt_sys, const uint16_t *sys_of, const double *ax,
+                     const double *ay, const double *az, const double *vhx, const double *vhy,
+                     const double *vhz, double *vx, double *vy, double *vz, hipStream_t s);
+
+// ensemble forces: one persistent launch over a queue of work items -- the
+// traversal groups of Barnes-Hut systems and 64-body direct-sum tiles of the
+// others -- in queue order qorder (item_off[q] = first item of position q).
+struct EnsForceArgs {
+    const EnsSystem *sys;
+    const uint32_t *qorder;
+    const uint32_t *item_off;  // nq + 1 entries
+    uint32_t nq;
+    const double *x, *y, *z, *m;
+    double *ax, *ay, *az;
+    ForestBufs fb;
+    double G, theta;
+    float eps2;
+    uint32_t *cursor;  // work-queue cursor (reset by launch_ens_items)
+    uint32_t *status;  // kFlag* bits
+    uint32_t *inter;   // per body: accepted interactions (Barnes-Hut) or sources (direct)
+};
+constexpr uint32_t kEnsTile = 64;  // targets per direct-sum item
+// item_off = exclusive prefix of items[qorder[q]]; resets the cursor and status
+void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items,
+                      uint32_t *item_off, uint32_t *cursor, uint32_t *status, hipStream_t s);
+void launch_ens_forces(const EnsForceArgs &a, uint32_t waves, hipStream_t s);
 
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Host: canonical depth-first dump of an octree (children in digit order),
+// from host copies of its node references, cell sums and counts and of the
+// bodies.  Returns the record count, 0 if cap is too small.
+inline uint64_t canonical_dump(const int32_t *ref, const InternalSums *sums, const uint32_t *cnt,
+                               const double *hx, const double *hy, const double *hz,
+                               const double *hm, uint32_t n, uint64_t cap, uint8_t *depth,
+                               uint8_t *kind, uint32_t *body, double *mass, double *com_x,
+                               double *com_y, double *com_z, uint32_t *count) {
+    struct Item {
+        uint64_t node;
+        uint32_t d;
+    };
+    Item stack[1 + 7 * (kMaxDepth + 1)];
+    int sp = 0;
+    stack[sp++] = {0, 0};
+    uint64_t k = 0;
+    while (sp > 0) {
+        const Item it = stack[--sp];
+        if (k >= cap) return 0;
+        const int32_t r = ref[it.node];
+        depth[k] = (uint8_t)it.d;
+        if (r >= 0) {
+            kind[k] = 2;
+            body[k] = n;
+            mass[k] = sums[r].m;
+            com_x[k] = sums[r].x;
+            com_y[k] = sums[r].y;
+            com_z[k] = sums[r].z;
+            count[k] = cnt[r];
+            if (sp + 8 > (int)(sizeof(stack) / sizeof(stack[0]))) return 0;  // deeper than 42 levels
+            for (int dg = 7; dg >= 0; --dg) stack[sp++] = {1ull + 8ull * (uint64_t)r + dg, it.d + 1};
+        } else if (r != kRefEmpty) {
+            const uint32_t b = body_of_ref(r);
+            kind[k] = 1;
+            body[k] = b;
+            mass[k] = hm[b];
+            com_x[k] = hx[b] * hm[b];
+            com_y[k] = hy[b] * hm[b];
+            com_z[k] = hz[b] * hm[b];
+            count[k] = 1;
+        } else {
+            kind[k] = 0;
+            body[k] = n;
+            mass[k] = com_x[k] = com_y[k] = com_z[k] = 0.0;
+            count[k] = 0;
+        }
+        ++k;
+    }
+    return k;
+}
 
 }  // namespace nb

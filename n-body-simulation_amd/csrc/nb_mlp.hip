@@ -414,6 +414,45 @@ __global__ void __launch_bounds__(kThreads) k_pair_walk(
     }
 }
 
+// Features and target of pair pi (cell k, body at p): the monopole of the
+// cell is subtracted from the exact force a of the cell's bodies.
+__device__ inline void pair_record(uint64_t pi, uint32_t k, double px, double py, double pz,
+                                   double ax, double ay, double az,
+                                   const InternalSums *__restrict__ sums,
+                                   const double *__restrict__ quad,
+                                   const MlpCell *__restrict__ mcell, double G, double eps2,
+                                   float *features, float *targets) {
+    const InternalSums s = sums[k];
+    const double M = s.m;
+    const double rx = s.x / M - px, ry = s.y / M - py, rz = s.z / M - pz;
+    const double d2 = rx * rx + ry * ry + rz * rz;
+    const double d = sqrt(d2);
+    double den = d2 + eps2;
+    den = den * den * den;
+    den = 1.0 / sqrt(den);
+    const MlpCell c = mcell[k];
+    const double e = (double)c.c.z;
+    const double ux = rx / d, uy = ry / d, uz = rz / d;
+    const double qn = 1.0 / (M * e * e);
+    float *f = features + pi * kMlpIn;
+    f[0] = (float)ux;
+    f[1] = (float)uy;
+    f[2] = (float)uz;
+    f[3] = (float)(e / d);
+    f[4] = c.c.y;
+    for (int u = 0; u < 6; ++u) f[5 + u] = (float)(quad[6ull * k + u] * qn);
+    const double ox = c.b.z, oy = c.b.w, oz = c.c.x;
+    const double par = ox * ux + oy * uy + oz * uz;
+    const double perp2 = ox * ox + oy * oy + oz * oz - par * par;
+    f[11] = (float)par;
+    f[12] = (float)sqrt(perp2 > 0 ? perp2 : 0.0);
+    const double scale = d2 * d2 / (G * M * e * e);
+    float *tg = targets + pi * kMlpOut;
+    tg[0] = (float)((G * ax - G * M * (rx * den)) * scale);
+    tg[1] = (float)((G * ay - G * M * (ry * den)) * scale);
+    tg[2] = (float)((G * az - G * M * (rz * den)) * scale);
+}
+
 // One wavefront per pair: exact force of the cell's bodies on the target
 // (FP64, the direct-sum arithmetic), monopole, features and target.
 __global__ void __launch_bounds__(64) k_pair_targets(
@@ -444,36 +483,11 @@ __global__ void __launch_bounds__(64) k_pair_targets(
             ay += __shfl_xor(ay, off, 64);
             az += __shfl_xor(az, off, 64);
         }
-        if (lane != 0) continue;
-        const InternalSums s = sums[k];
-        const double M = s.m;
-        const double rx = s.x / M - px, ry = s.y / M - py, rz = s.z / M - pz;
-        const double d2 = rx * rx + ry * ry + rz * rz;
-        const double d = sqrt(d2);
-        double den = d2 + eps2;
-        den = den * den * den;
-        den = 1.0 / sqrt(den);
-        const MlpCell c = mcell[k];
-        const double e = (double)c.c.z;
-        const double ux = rx / d, uy = ry / d, uz = rz / d;
-        const double qn = 1.0 / (M * e * e);
-        float *f = features + pi * kMlpIn;
-        f[0] = (float)ux;
-        f[1] = (float)uy;
-        f[2] = (float)uz;
-        f[3] = (float)(e / d);
-        f[4] = c.c.y;
-        for (int u = 0; u < 6; ++u) f[5 + u] = (float)(quad[6ull * k + u] * qn);
-        const double ox = c.b.z, oy = c.b.w, oz = c.c.x;
-        const double par = ox * ux + oy * uy + oz * uz;
-        const double perp2 = ox * ox + oy * oy + oz * oz - par * par;
-        f[11] = (float)par;
-        f[12] = (float)sqrt(perp2 > 0 ? perp2 : 0.0);
-        const double scale = d2 * d2 / (G * M * e * e);
-        float *tg = targets + pi * kMlpOut;
-        tg[0] = (float)((G * ax - G * M * (rx * den)) * scale);
-        tg[1] = (float)((G * ay - G * M * (ry * den)) * scale);
-        tg[2] = (float)((G * az - G * M * (rz * den)) * scale);
+        // lane 0 writes the pair; no early `continue` of the other lanes, so
+        // the wave stays together for the next pair's shuffles
+        if (lane == 0)
+            pair_record(pi, k, px, py, pz, ax, ay, az, sums, quad, mcell, G, eps2, features,
+                        targets);
     }
 }
 

@@ -110,40 +110,9 @@ __global__ void __launch_bounds__(kThreads) k_aabb_partial(const double *__restr
     }
 }
 
-// Final reduction + cubify (BarnesHutOctree.cpp:161-190) on one thread per
-// component, then one thread writes the cube.
-__global__ void __launch_bounds__(kThreads) k_aabb_final(const double *part, uint32_t np,
-                                                         TreeHdr *h) {
-    __shared__ double sh[kThreads / 64][6];
-    __shared__ double ext[6];
-    // every partial already includes the origin, so 0.0 is a neutral start
-    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (uint32_t k = threadIdx.x; k < np; k += kThreads) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            v[c] = sycl_min(v[c], part[(uint64_t)c * np + k]);
-            v[3 + c] = sycl_max(v[3 + c], part[(uint64_t)(3 + c) * np + k]);
-        }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            v[c] = sycl_min(v[c], __shfl_xor(v[c], off, 64));
-            v[3 + c] = sycl_max(v[3 + c], __shfl_xor(v[3 + c], off, 64));
-        }
-    }
-    if ((threadIdx.x & 63) == 0)
-        for (int c = 0; c < 6; ++c) sh[threadIdx.x >> 6][c] = v[c];
-    __syncthreads();
-    if (threadIdx.x < 6) {
-        double r = sh[0][threadIdx.x];
-        for (int w = 1; w < kThreads / 64; ++w)
-            r = threadIdx.x < 3 ? sycl_min(r, sh[w][threadIdx.x]) : sycl_max(r, sh[w][threadIdx.x]);
-        ext[threadIdx.x] = r;
-    }
-    __syncthreads();
-    if (threadIdx.x != 0) return;
+// Cubify the extents ext = {min_x, min_y, min_z, max_x, max_y, max_z}
+// (BarnesHutOctree.cpp:161-190) and store the root cube.
+__device__ inline void cubify(const double ext[6], TreeHdr *h) {
     double min_x = ext[0], min_y = ext[1], min_z = ext[2];
     double max_x = ext[3], max_y = ext[4], max_z = ext[5];
     const double x_length = fabs(max_x - min_x);
@@ -179,6 +148,43 @@ __global__ void __launch_bounds__(kThreads) k_aabb_final(const double *part, uin
     h->edge = maxEdgeLength;
 }
 
+// Final reduction on one thread per component, then one thread cubifies.
+__global__ void __launch_bounds__(kThreads) k_aabb_final(const double *part, uint32_t np,
+                                                         TreeHdr *h) {
+    __shared__ double sh[kThreads / 64][6];
+    __shared__ double ext[6];
+    // every partial already includes the origin, so 0.0 is a neutral start
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = threadIdx.x; k < np; k += kThreads) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = sycl_min(v[c], part[(uint64_t)c * np + k]);
+            v[3 + c] = sycl_max(v[3 + c], part[(uint64_t)(3 + c) * np + k]);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            v[c] = sycl_min(v[c], __shfl_xor(v[c], off, 64));
+            v[3 + c] = sycl_max(v[3 + c], __shfl_xor(v[3 + c], off, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 6; ++c) sh[threadIdx.x >> 6][c] = v[c];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        double r = sh[0][threadIdx.x];
+        for (int w = 1; w < kThreads / 64; ++w)
+            r = threadIdx.x < 3 ? sycl_min(r, sh[w][threadIdx.x]) : sycl_max(r, sh[w][threadIdx.x]);
+        ext[threadIdx.x] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const double e6[6] = {ext[0], ext[1], ext[2], ext[3], ext[4], ext[5]};
+    cubify(e6, h);
+}
+
 // ---- 2. keys ---------------------------------------------------------------
 __device__ inline uint64_t key_from(Cell &c, double px, double py, double pz) {
     uint64_t key = 0;
@@ -205,13 +211,13 @@ __device__ inline int lcp_depth(uint64_t a, uint64_t b) {
     return (__clzll((long long)(a ^ b)) - 1) / 3;
 }
 
-__global__ void __launch_bounds__(kThreads) k_lcp(const double *__restrict__ x,
-                                                  const double *__restrict__ y,
-                                                  const double *__restrict__ z, uint32_t n,
-                                                  TreeHdr *h, uint64_t *skey, uint32_t *sidx,
-                                                  uint64_t *klo, int32_t *lcp) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i + 1 >= n) return;
+// lcp of adjacent pair i (i + 1 < n); the first pair of a run of equal keys
+// extends and sorts the run.  Shared by the one-system build and the
+// workgroup-per-system ensemble build.
+__device__ inline void lcp_at(uint32_t i, const double *__restrict__ x,
+                              const double *__restrict__ y, const double *__restrict__ z,
+                              uint32_t n, const Cell &root, TreeHdr *h, uint64_t *skey,
+                              uint32_t *sidx, uint64_t *klo, int32_t *lcp) {
     const uint64_t k0 = skey[i], k1 = skey[i + 1];
     if (k0 != k1) {
         lcp[i] = lcp_depth(k0, k1);
@@ -224,7 +230,7 @@ __global__ void __launch_bounds__(kThreads) k_lcp(const double *__restrict__ x,
     uint32_t b = i + 1;
     while (b + 1 < n && skey[b + 1] == k0) ++b;
     atomicAdd(&h->deep_runs, 1u);
-    Cell c0{h->minx, h->miny, h->minz, h->edge};
+    Cell c0 = root;
     {
         const uint32_t q = sidx[a];
         key_from(c0, x[q], y[q], z[q]);  // walk to the shared level-21 cell
@@ -258,14 +264,29 @@ __global__ void __launch_bounds__(kThreads) k_lcp(const double *__restrict__ x,
     }
 }
 
+__global__ void __launch_bounds__(kThreads) k_lcp(const double *__restrict__ x,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ z, uint32_t n,
+                                                  TreeHdr *h, uint64_t *skey, uint32_t *sidx,
+                                                  uint64_t *klo, int32_t *lcp) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i + 1 >= n) return;
+    const Cell root{h->minx, h->miny, h->minz, h->edge};
+    lcp_at(i, x, y, z, n, root, h, skey, sidx, klo, lcp);
+}
+
 // chain[i] = number of internal cells owned by pair i = max(0, lcp[i] - lcp[i-1])
+__device__ inline void chain_at(uint32_t i, const int32_t *__restrict__ lcp, uint32_t *chain) {
+    const int cp = i > 0 ? lcp[i - 1] : -1;
+    const int cn = lcp[i];
+    chain[i] = cn > cp ? (uint32_t)(cn - cp) : 0u;
+}
+
 __global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ lcp, uint32_t n,
                                                     uint32_t *chain) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i + 1 >= n) return;
-    const int cp = i > 0 ? lcp[i - 1] : -1;
-    const int cn = lcp[i];
-    chain[i] = cn > cp ? (uint32_t)(cn - cp) : 0u;
+    chain_at(i, lcp, chain);
 }
 
 // ---- 6. topology --------------------------------------------------------------
@@ -319,17 +340,15 @@ __global__ void __launch_bounds__(kThreads) k_fill(TreeHdr *h, uint32_t icap, in
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, TreeHdr *h,
-                                                   KeyView kv, const uint32_t *__restrict__ sidx,
-                                                   const int32_t *__restrict__ lcp,
-                                                   const uint32_t *__restrict__ off,
-                                                   int32_t *ref, uint8_t *int_depth,
-                                                   uint32_t *int_parent, uint32_t *int_node,
-                                                   uint32_t *nchild, uint32_t *body_parent,
-                                                   uint8_t *int_local, uint32_t *spanning,
-                                                   uint32_t spanning_cap, uint32_t *int_first) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
+// Internal cells owned by pair i and the leaf of sorted body i.  A cell is
+// "local" when its bodies stay inside one com_block-body block of the sorted
+// order; the others are listed in `spanning`.
+__device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h, KeyView kv,
+                               const uint32_t *__restrict__ sidx, const int32_t *__restrict__ lcp,
+                               const uint32_t *__restrict__ off, int32_t *ref, uint8_t *int_depth,
+                               uint32_t *int_parent, uint32_t *int_node, uint32_t *nchild,
+                               uint32_t *body_parent, uint8_t *int_local, uint32_t *spanning,
+                               uint32_t spanning_cap, uint32_t *int_first, uint32_t com_block) {
     if (n == 1) {  // the root is a leaf holding the only body
         ref[0] = ref_of_body(sidx[0]);
         body_parent[0] = 0xffffffffu;
@@ -353,7 +372,7 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
         int_first[k] = i;
         // The cell's bodies start at i; it crosses into the next COM block iff
         // that block's first body shares its l-digit prefix with body i.
-        const uint32_t next_block = (i / kComBlock + 1) * kComBlock;
+        const uint32_t next_block = (i / com_block + 1) * com_block;
         const bool local = !(next_block < n && kv.same(next_block, i, l));
         int_local[k] = local ? 1 : 0;
         if (!local) {
@@ -391,6 +410,21 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
     body_parent[i] = pk;
     atomicAdd(&nchild[pk], 1u);
     atomicMax(&h->max_depth, (uint32_t)(D + 1));
+}
+
+__global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, TreeHdr *h,
+                                                   KeyView kv, const uint32_t *__restrict__ sidx,
+                                                   const int32_t *__restrict__ lcp,
+                                                   const uint32_t *__restrict__ off,
+                                                   int32_t *ref, uint8_t *int_depth,
+                                                   uint32_t *int_parent, uint32_t *int_node,
+                                                   uint32_t *nchild, uint32_t *body_parent,
+                                                   uint8_t *int_local, uint32_t *spanning,
+                                                   uint32_t spanning_cap, uint32_t *int_first) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    link_at(i, n, icap, h, kv, sidx, lcp, off, ref, int_depth, int_parent, int_node, nchild,
+            body_parent, int_local, spanning, spanning_cap, int_first, kComBlock);
 }
 
 // ---- 7. centre of mass --------------------------------------------------------
@@ -440,6 +474,40 @@ __device__ inline uint32_t arrive_workgroup(uint32_t *p) {
 // write-back or invalidation across the chip's XCDs).
 constexpr int kComThreads = 1024;
 
+// Climb from sorted body i's leaf while the cells are local to this workgroup.
+__device__ inline void com_climb(uint32_t i, uint32_t icap, TreeHdr *h,
+                                 const double *__restrict__ x, const double *__restrict__ y,
+                                 const double *__restrict__ z, const double *__restrict__ m,
+                                 const uint32_t *__restrict__ sidx,
+                                 const uint32_t *__restrict__ body_parent,
+                                 const int32_t *__restrict__ ref,
+                                 const uint32_t *__restrict__ int_parent,
+                                 const uint32_t *__restrict__ nchild,
+                                 const uint8_t *__restrict__ int_local, uint32_t *arrive,
+                                 InternalSums *sums, uint32_t *counts) {
+    if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
+    uint32_t pk = body_parent[i];
+    if (pk == 0xffffffffu) return;  // single body (root leaf) or overflow
+    // a leaf has at most kMaxDepth ancestors; the bound guards a corrupt chain
+    for (int level = 0; level <= kMaxDepth; ++level) {
+        if (!int_local[pk]) break;  // crosses a block boundary: pass 2
+        const uint32_t seen = arrive_workgroup(&arrive[pk]);
+        // The child counts were built with device-scope atomics (in L2); when
+        // the same kernel produced them (ensemble build), a plain load could
+        // hit a stale L1 line, so read them the way they were written.
+        const uint32_t want = __hip_atomic_load(&nchild[pk], __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+        if (seen + 1 != want) break;  // a sibling finishes this cell
+        sum_children(pk, x, y, z, m, ref, sums, counts, h);
+        if (pk == 0) break;
+        pk = int_parent[pk];
+        if (pk >= icap) {
+            raise_flag(h, kFlagNodeOverflow);
+            break;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kComThreads) k_com_local(
     uint32_t n, uint32_t icap, TreeHdr *h, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
@@ -449,24 +517,9 @@ __global__ void __launch_bounds__(kComThreads) k_com_local(
     uint32_t *arrive, InternalSums *sums, uint32_t *counts) {
     const uint32_t s = blockIdx.x * kComBlock;
     const uint32_t e = min(n, s + kComBlock);
-    for (uint32_t i = s + threadIdx.x; i < e; i += kComThreads) {
-        if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
-        uint32_t pk = body_parent[i];
-        if (pk == 0xffffffffu) continue;  // single body (root leaf) or overflow
-        // a leaf has at most kMaxDepth ancestors; the bound guards a corrupt chain
-        for (int level = 0; level <= kMaxDepth; ++level) {
-            if (!int_local[pk]) break;  // crosses a block boundary: pass 2
-            const uint32_t seen = arrive_workgroup(&arrive[pk]);
-            if (seen + 1 != nchild[pk]) break;  // a sibling finishes this cell
-            sum_children(pk, x, y, z, m, ref, sums, counts, h);
-            if (pk == 0) break;
-            pk = int_parent[pk];
-            if (pk >= icap) {
-                raise_flag(h, kFlagNodeOverflow);
-                break;
-            }
-        }
-    }
+    for (uint32_t i = s + threadIdx.x; i < e; i += kComThreads)
+        com_climb(i, icap, h, x, y, z, m, sidx, body_parent, ref, int_parent, nchild, int_local,
+                  arrive, sums, counts);
 }
 
 // Pass 2: one workgroup finishes the cells that cross block boundaries,
@@ -500,18 +553,39 @@ __global__ void __launch_bounds__(kComThreads) k_com_spanning(
 // at most one group: mark the starts, scan, scatter -- the group list comes
 // out in octree order, so neighbouring groups share most of their walk.
 // pass 1: every sorted position; lone bodies start a group of one
-__global__ void __launch_bounds__(kThreads) k_group_marks_bodies(
-    uint32_t n, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ body_parent,
-    uint32_t *group_size, uint32_t *group_slot) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
+__device__ inline void group_mark_body(uint32_t i, uint32_t n, const uint32_t *__restrict__ counts,
+                                       const uint32_t *__restrict__ body_parent,
+                                       uint32_t *group_size, uint32_t *group_slot) {
     const uint32_t pk = body_parent[i];
     const bool lone = pk == 0xffffffffu ? (n == 1) : counts[pk] > kGroupMax;
     group_size[i] = lone ? 1u : 0u;
     group_slot[i] = lone ? 1u : 0u;
 }
 
+__global__ void __launch_bounds__(kThreads) k_group_marks_bodies(
+    uint32_t n, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ body_parent,
+    uint32_t *group_size, uint32_t *group_slot) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    group_mark_body(i, n, counts, body_parent, group_size, group_slot);
+}
+
 // pass 2: group cells mark their first sorted position (never a lone body's)
+__device__ inline void group_mark_cell(uint64_t k, uint32_t n, const uint32_t *__restrict__ counts,
+                                       const uint32_t *__restrict__ int_parent,
+                                       const uint32_t *__restrict__ int_first,
+                                       uint32_t *group_size, uint32_t *group_slot) {
+    const uint32_t c = counts[k];
+    const uint32_t pc = k == 0 ? 0xffffffffu : counts[int_parent[k]];
+    if (c <= kGroupMax && pc > kGroupMax) {
+        const uint32_t f = int_first[k];
+        if (f < n) {
+            group_size[f] = c;
+            group_slot[f] = 1u;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) k_group_marks_cells(uint32_t n, uint32_t icap,
                                                                 const TreeHdr *__restrict__ h,
                                                                 const uint32_t *__restrict__ counts,
@@ -522,17 +596,8 @@ __global__ void __launch_bounds__(kThreads) k_group_marks_cells(uint32_t n, uint
     const uint32_t I = min(h->internal_count, icap);
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     const uint64_t t0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    for (uint64_t k = t0; k < I; k += stride) {
-        const uint32_t c = counts[k];
-        const uint32_t pc = k == 0 ? 0xffffffffu : counts[int_parent[k]];
-        if (c <= kGroupMax && pc > kGroupMax) {
-            const uint32_t f = int_first[k];
-            if (f < n) {
-                group_size[f] = c;
-                group_slot[f] = 1u;
-            }
-        }
-    }
+    for (uint64_t k = t0; k < I; k += stride)
+        group_mark_cell(k, n, counts, int_parent, int_first, group_size, group_slot);
 }
 
 __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
@@ -546,6 +611,44 @@ __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
 }
 
 // ---- 8. node records -----------------------------------------------------------
+__device__ inline void record_at(uint64_t j, double root_edge, const double *__restrict__ x,
+                                 const double *__restrict__ y, const double *__restrict__ z,
+                                 const double *__restrict__ m, const int32_t *__restrict__ ref,
+                                 const InternalSums *__restrict__ sums,
+                                 const uint8_t *__restrict__ int_depth, double G, int write_f64,
+                                 int write_f32, double4 *cm, double *edge, NodeF32 *rec) {
+    const int32_t r = ref[j];
+    double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
+    double e = 0.0;
+    if (r >= 0) {
+        // the reference divides numerator by mass at every visit
+        // (BarnesHutAlgorithm.cpp:351-353); the quotient is the same
+        const InternalSums s = sums[r];
+        c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
+        e = ldexp(root_edge, -(int)int_depth[r]);
+    } else if (r != kRefEmpty) {
+        const uint32_t b = body_of_ref(r);
+        const double mb = m[b];
+        c = make_double4((x[b] * mb) / mb, (y[b] * mb) / mb, (z[b] * mb) / mb, mb);
+    }
+    if (write_f64) {
+        cm[j] = c;
+        edge[j] = e;
+    }
+    if (write_f32) {
+        NodeF32 o;
+        o.x = (float)c.x;
+        o.y = (float)c.y;
+        o.z = (float)c.z;
+        o.gm = (float)(G * c.w);
+        o.ref = r;
+        o.lx = (float)(c.x - (double)o.x);
+        o.ly = (float)(c.y - (double)o.y);
+        o.lz = (float)(c.z - (double)o.z);
+        rec[j] = o;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict__ h,
                                                       uint32_t icap,
                                                       const double *__restrict__ x,
@@ -561,41 +664,299 @@ __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict_
     const uint64_t nodes = 1 + 8ull * I;
     const double root_edge = h->edge;
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride) {
-        const int32_t r = ref[j];
-        double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
-        double e = 0.0;
-        if (r >= 0) {
-            // the reference divides numerator by mass at every visit
-            // (BarnesHutAlgorithm.cpp:351-353); the quotient is the same
-            const InternalSums s = sums[r];
-            c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
-            e = ldexp(root_edge, -(int)int_depth[r]);
-        } else if (r != kRefEmpty) {
-            const uint32_t b = body_of_ref(r);
-            const double mb = m[b];
-            c = make_double4((x[b] * mb) / mb, (y[b] * mb) / mb, (z[b] * mb) / mb, mb);
+    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride)
+        record_at(j, root_edge, x, y, z, m, ref, sums, int_depth, G, write_f64, write_f32, cm,
+                  edge, rec);
+}
+
+// ---- ensemble: one workgroup builds one system's octree ----------------------------
+// The same per-element steps as the one-system build, separated by workgroup
+// barriers instead of kernel boundaries; scans and the radix sort run inside
+// the workgroup, and every cell is local (one centre-of-mass pass).
+constexpr int kForestThreads = 1024;
+constexpr int kForestWaves = kForestThreads / 64;
+constexpr int kSortBits = 5;  // radix digit width: 32 bins x 1024 threads of LDS counters
+constexpr int kSortBins = 1 << kSortBits;
+constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;
+
+// Exclusive scan of v over the workgroup; *total receives the sum (all threads).
+__device__ inline uint32_t wg_scan_value(uint32_t v, uint32_t *wsum, uint32_t *total) {
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(inc, off, 64);
+        if (lane >= (uint32_t)off) inc += o;
+    }
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < kForestWaves; ++k) {
+            const uint32_t t = wsum[k];
+            wsum[k] = run;
+            run += t;
         }
-        if (write_f64) {
-            cm[j] = c;
-            edge[j] = e;
+        wsum[kForestWaves] = run;
+    }
+    __syncthreads();
+    const uint32_t excl = wsum[w] + inc - v;
+    *total = wsum[kForestWaves];
+    __syncthreads();
+    return excl;
+}
+
+// Exclusive scan in place of a[0..m) (global memory, this workgroup only).
+__device__ inline uint32_t wg_scan_array(uint32_t *a, uint32_t m, uint32_t *wsum) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (m + kForestThreads - 1) / kForestThreads;
+    const uint32_t b = min(m, t * per), e = min(m, b + per);
+    uint32_t s = 0;
+    for (uint32_t i = b; i < e; ++i) s += a[i];
+    uint32_t total;
+    uint32_t run = wg_scan_value(s, wsum, &total);
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+    __syncthreads();
+    return total;
+}
+
+// Stable LSD radix sort of (k, v) pairs by the 63-bit keys.  Thread t owns a
+// contiguous slice of the items, so scattering each slice in order keeps the
+// sort stable; hist[d * 1024 + t] counts digit d in slice t.
+__device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2,
+                                     uint32_t n, uint32_t *hist, uint32_t *wsum, uint32_t *dbase) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    const uint32_t per = (n + kForestThreads - 1) / kForestThreads;
+    const uint32_t b = min(n, t * per), e = min(n, b + per);
+    uint64_t *sk = k, *dk = k2;
+    uint32_t *sv = v, *dv = v2;
+    for (int p = 0; p < kSortPasses; ++p) {
+        const int sh = kSortBits * p;
+        for (int d = 0; d < kSortBins; ++d) hist[d * kForestThreads + t] = 0;
+        for (uint32_t i = b; i < e; ++i) hist[((sk[i] >> sh) & (kSortBins - 1)) * kForestThreads + t]++;
+        __syncthreads();
+        // per digit: scan over the slices (wave shuffles), wave totals to LDS
+        uint32_t excl[kSortBins];
+#pragma unroll
+        for (int d = 0; d < kSortBins; ++d) {
+            const uint32_t c = hist[d * kForestThreads + t];
+            uint32_t inc = c;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t o = __shfl_up(inc, off, 64);
+                if (lane >= (uint32_t)off) inc += o;
+            }
+            excl[d] = inc - c;
+            if (lane == 63) wsum[d * kForestWaves + w] = inc;
         }
-        if (write_f32) {
-            NodeF32 o;
-            o.x = (float)c.x;
-            o.y = (float)c.y;
-            o.z = (float)c.z;
-            o.gm = (float)(G * c.w);
-            o.ref = r;
-            o.lx = (float)(c.x - (double)o.x);
-            o.ly = (float)(c.y - (double)o.y);
-            o.lz = (float)(c.z - (double)o.z);
-            rec[j] = o;
+        __syncthreads();
+        // wave offsets within each digit, then digit bases
+        if (t < kSortBins) {
+            uint32_t run = 0;
+            for (int ww = 0; ww < kForestWaves; ++ww) {
+                const uint32_t c = wsum[t * kForestWaves + ww];
+                wsum[t * kForestWaves + ww] = run;
+                run += c;
+            }
+            dbase[t] = run;
         }
+        __syncthreads();
+        if (t == 0) {
+            uint32_t run = 0;
+            for (int d = 0; d < kSortBins; ++d) {
+                const uint32_t c = dbase[d];
+                dbase[d] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < kSortBins; ++d)
+            hist[d * kForestThreads + t] = dbase[d] + wsum[d * kForestWaves + w] + excl[d];
+        __syncthreads();
+        for (uint32_t i = b; i < e; ++i) {
+            const uint64_t key = sk[i];
+            const uint32_t pos = hist[((key >> sh) & (kSortBins - 1)) * kForestThreads + t]++;
+            dk[pos] = key;
+            dv[pos] = sv[i];
+        }
+        __syncthreads();
+        uint64_t *tk = sk;
+        sk = dk;
+        dk = tk;
+        uint32_t *tv = sv;
+        sv = dv;
+        dv = tv;
+    }
+    if (sk != k) {
+        for (uint32_t i = t; i < n; i += kForestThreads) {
+            k[i] = sk[i];
+            v[i] = sv[i];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kForestThreads) k_forest_build(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order,
+    const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
+    const double *__restrict__ gm, double G, ForestBufs fb, uint32_t *items, uint32_t *status) {
+    __shared__ uint32_t hist[kSortBins * kForestThreads];
+    __shared__ uint32_t wsum[kSortBins * kForestWaves + 1];
+    __shared__ uint32_t dbase[kSortBins];
+    __shared__ double red[kForestWaves][6];
+    __shared__ double cube[4];  // root cell {min x, min y, min z, edge}
+    const uint32_t s = order[blockIdx.x];
+    const EnsSystem S = sys[s];
+    const uint32_t n = S.n, t = threadIdx.x;
+    const double *x = gx + S.body_off, *y = gy + S.body_off, *z = gz + S.body_off,
+                 *m = gm + S.body_off;
+    TreeHdr *h = fb.hdr + s;
+    const uint32_t fo = S.fb_off;
+    uint64_t *skey = fb.skey + fo, *skey_alt = fb.skey_alt + fo, *klo = fb.klo + fo;
+    uint32_t *sidx = fb.sidx + fo, *sidx_alt = fb.sidx_alt + fo, *chain = fb.chain + fo,
+             *body_parent = fb.body_parent + fo, *group_size = fb.group_size + fo,
+             *group_slot = fb.group_slot + fo;
+    int32_t *lcp = fb.lcp + fo;
+    uint2 *groups = fb.groups + fo;
+    uint8_t *int_depth = fb.int_depth + S.int_off, *int_local = fb.int_local + S.int_off;
+    uint32_t *int_parent = fb.int_parent + S.int_off, *int_node = fb.int_node + S.int_off,
+             *nchild = fb.int_nchild + S.int_off, *arrive = fb.int_arrive + S.int_off,
+             *counts = fb.int_count + S.int_off, *int_first = fb.int_first + S.int_off;
+    InternalSums *sums = fb.int_sums + S.int_off;
+    int32_t *ref = fb.ref + S.node_off;
+    NodeF32 *rec = fb.rec + S.node_off;
+    const uint32_t icap = S.icap;
+
+    if (t == 0) {
+        h->internal_count = 0;
+        h->max_depth = 0;
+        h->status = 0;
+        h->deep_runs = 0;
+        h->spanning_count = 0;
+        h->stack_peak = 0;
+        h->group_count = 0;
+        h->group_next = 0;
+        h->corr_pairs = 0;
+    }
+    __syncthreads();  // the reset precedes every flag raised below
+    // 1. AABB including the origin (every partial starts at 0.0), then cubify
+    {
+        double mn[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
+        bool bad = false;
+        for (uint32_t i = t; i < n; i += kForestThreads) {
+            const double p[3] = {x[i], y[i], z[i]};
+            for (int c = 0; c < 3; ++c) {
+                bad |= !isfinite(p[c]);
+                mn[c] = sycl_min(mn[c], p[c]);
+                mx[c] = sycl_max(mx[c], p[c]);
+            }
+        }
+        if (bad) raise_flag(h, kFlagAabb);
+        for (int off = 32; off >= 1; off >>= 1)
+            for (int c = 0; c < 3; ++c) {
+                mn[c] = sycl_min(mn[c], __shfl_xor(mn[c], off, 64));
+                mx[c] = sycl_max(mx[c], __shfl_xor(mx[c], off, 64));
+            }
+        if ((t & 63) == 0)
+            for (int c = 0; c < 3; ++c) {
+                red[t >> 6][c] = mn[c];
+                red[t >> 6][3 + c] = mx[c];
+            }
+        __syncthreads();
+        if (t == 0) {
+            double v[6];
+            for (int c = 0; c < 6; ++c) v[c] = red[0][c];
+            for (int w = 1; w < kForestWaves; ++w)
+                for (int c = 0; c < 3; ++c) {
+                    v[c] = sycl_min(v[c], red[w][c]);
+                    v[3 + c] = sycl_max(v[3 + c], red[w][3 + c]);
+                }
+            cubify(v, h);
+            cube[0] = h->minx;
+            cube[1] = h->miny;
+            cube[2] = h->minz;
+            cube[3] = h->edge;
+        }
+        __syncthreads();
+    }
+    // the workgroup reads the root cell from LDS, never back from the header
+    const Cell root{cube[0], cube[1], cube[2], cube[3]};
+    // 2. keys, 3. sort
+    {
+        for (uint32_t i = t; i < n; i += kForestThreads) {
+            Cell c = root;
+            skey[i] = key_from(c, x[i], y[i], z[i]);
+            sidx[i] = i;
+        }
+        __syncthreads();
+        wg_radix_sort(skey, sidx, skey_alt, sidx_alt, n, hist, wsum, dbase);
+    }
+    // 4./5. lcp, deep runs, cell numbering
+    for (uint32_t i = t; i + 1 < n; i += kForestThreads)
+        lcp_at(i, x, y, z, n, root, h, skey, sidx, klo, lcp);
+    __syncthreads();
+    for (uint32_t i = t; i + 1 < n; i += kForestThreads) chain_at(i, lcp, chain);
+    __syncthreads();
+    const uint32_t I_all = n > 1 ? wg_scan_array(chain, n - 1, wsum) : 0u;
+    if (t == 0) h->internal_count = I_all;
+    const uint32_t I = min(I_all, icap);
+    // 6. topology
+    for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
+    for (uint32_t j = t; j < I; j += kForestThreads) {
+        nchild[j] = 0;
+        arrive[j] = 0;
+    }
+    __syncthreads();
+    const KeyView kv{skey, klo};
+    for (uint32_t i = t; i < n; i += kForestThreads)
+        link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node, nchild,
+                body_parent, int_local, nullptr, 0u, int_first, n);
+    __syncthreads();
+    // 7. centre of mass: every cell is local to this workgroup
+    for (uint32_t i = t; i < n; i += kForestThreads)
+        com_climb(i, icap, h, x, y, z, m, sidx, body_parent, ref, int_parent, nchild, int_local,
+                  arrive, sums, counts);
+    __syncthreads();
+    // 8. records and traversal groups
+    const double root_edge = root.e;
+    for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
+        record_at(j, root_edge, x, y, z, m, ref, sums, int_depth, G, 0, 1, nullptr, nullptr, rec);
+    for (uint32_t i = t; i < n; i += kForestThreads)
+        group_mark_body(i, n, counts, body_parent, group_size, group_slot);
+    __syncthreads();
+    for (uint32_t k = t; k < I; k += kForestThreads)
+        group_mark_cell(k, n, counts, int_parent, int_first, group_size, group_slot);
+    __syncthreads();
+    const uint32_t ng = wg_scan_array(group_slot, n, wsum);
+    for (uint32_t i = t; i < n; i += kForestThreads) {
+        const uint32_t gs = group_size[i];
+        if (gs) groups[group_slot[i]] = make_uint2(i, gs);
+    }
+    __syncthreads();
+    if (t == 0) {
+        h->group_count = ng;
+        items[s] = ng;
+        // flags were raised with device-scope atomics: read the word the same way
+        const uint32_t st = __hip_atomic_load(&h->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (st) atomicOr(status, st);
     }
 }
 
 }  // namespace
+
+void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
+                         const double *x, const double *y, const double *z, const double *m,
+                         double G, const ForestBufs &fb, uint32_t *items, uint32_t *status,
+                         hipStream_t s) {
+    if (nbh == 0) return;
+    hipLaunchKernelGGL(k_forest_build, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
+                       m, G, fb, items, status);
+}
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
                  hipStream_t s) {

@@ -435,3 +435,163 @@ class Mlp:
                                           T.ctypes.data_as(_fp), ctypes.byref(o),
                                           losses.ctypes.data_as(_fp)), "nb_mlp_train")
         return losses
+
+
+class EnsembleTuning(ctypes.Structure):
+    _fields_ = [("bh_threshold", ctypes.c_uint32), ("waves", ctypes.c_uint32)]
+
+
+ENSEMBLE_SIG = {
+    "nb_ensemble_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint32, _u32p,
+                                          ctypes.POINTER(Params)]),
+    "nb_ensemble_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "nb_ensemble_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "nb_ensemble_body_count": (ctypes.c_uint64, [ctypes.c_void_p]),
+    "nb_ensemble_set_state": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 8),
+    "nb_ensemble_get_state": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 6),
+    "nb_ensemble_get_accel": (ctypes.c_int, [ctypes.c_void_p] + [_dp] * 3),
+    "nb_ensemble_forces": (ctypes.c_int, [ctypes.c_void_p]),
+    "nb_ensemble_steps": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "nb_ensemble_phase_ms": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "nb_ensemble_interactions": (ctypes.c_int, [ctypes.c_void_p, _u32p]),
+    "nb_ensemble_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EnsembleTuning)]),
+    "nb_ensemble_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(EnsembleTuning)]),
+    "nb_ensemble_uses_bh": (ctypes.c_int, [ctypes.c_void_p, _u8p]),
+    "nb_ensemble_node_count": (ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_uint32]),
+    "nb_ensemble_tree_canonical": (ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_uint32,
+                                                     ctypes.c_uint64, _u8p, _u8p, _u32p]
+                                   + [_dp] * 4 + [_u32p]),
+    "nb_ensemble_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32,
+                                            ctypes.POINTER(EnsembleTuning), _dp]),
+}
+EXPORTS += list(ENSEMBLE_SIG)
+
+
+class Ensemble:
+    """Many independent systems advanced together (concatenated state)."""
+
+    def __init__(self, sizes, params=None, **kw):
+        self.lib = load()
+        for name, (res, args) in ENSEMBLE_SIG.items():
+            fn = getattr(self.lib, name)
+            fn.restype, fn.argtypes = res, args
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes, dtype=np.int64)])
+        self.n = int(self.offsets[-1])
+        p = params if params is not None else default_params(precision=PREC_FP32C)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        h = ctypes.c_void_p()
+        rc = self.lib.nb_ensemble_create(ctypes.byref(h), self.sizes.size,
+                                         self.sizes.ctypes.data_as(_u32p), ctypes.byref(p))
+        if rc:
+            raise NBodyError(rc, "nb_ensemble_create", strerror(rc))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.nb_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc:
+            raise NBodyError(rc, what, self.lib.nb_ensemble_last_error(self.h).decode())
+
+    def set_state(self, x, y, z, vx, vy, vz, m, dt):
+        arrs = [_f64(a) for a in (x, y, z, vx, vy, vz, m)]
+        dt = _f64(dt)
+        assert all(a.size == self.n for a in arrs) and dt.size == self.sizes.size
+        self._keep = arrs + [dt]
+        self._check(self.lib.nb_ensemble_set_state(self.h, *map(_d, arrs), _d(dt)),
+                    "nb_ensemble_set_state")
+
+    def state(self):
+        """[x, y, z, vx, vy, vz] of all systems (concatenated)."""
+        out = [np.empty(self.n) for _ in range(6)]
+        self._check(self.lib.nb_ensemble_get_state(self.h, *map(_d, out)), "nb_ensemble_get_state")
+        return out
+
+    def accel(self):
+        out = [np.empty(self.n) for _ in range(3)]
+        self._check(self.lib.nb_ensemble_get_accel(self.h, *map(_d, out)), "nb_ensemble_get_accel")
+        return out
+
+    def forces(self):
+        self._check(self.lib.nb_ensemble_forces(self.h), "nb_ensemble_forces")
+
+    def steps(self, count):
+        self._check(self.lib.nb_ensemble_steps(self.h, count), "nb_ensemble_steps")
+
+    def phase_ms(self):
+        ms = np.zeros(len(PHASES))
+        self._check(self.lib.nb_ensemble_phase_ms(self.h, _d(ms)), "nb_ensemble_phase_ms")
+        return dict(zip(PHASES, ms))
+
+    def interactions(self):
+        out = np.empty(self.n, np.uint32)
+        self._check(self.lib.nb_ensemble_interactions(self.h, out.ctypes.data_as(_u32p)),
+                    "nb_ensemble_interactions")
+        return out
+
+    def tree_canonical(self, s):
+        """Canonical dump of Barnes-Hut system s (format of Context.tree_canonical)."""
+        cap = self.lib.nb_ensemble_node_count(self.h, s)
+        if cap == 0:
+            raise NBodyError(-1, "nb_ensemble_node_count", f"system {s} has no octree")
+        out = dict(depth=np.zeros(cap, np.uint8), kind=np.zeros(cap, np.uint8),
+                   body=np.zeros(cap, np.uint32), mass=np.zeros(cap), com_x=np.zeros(cap),
+                   com_y=np.zeros(cap), com_z=np.zeros(cap), count=np.zeros(cap, np.uint32))
+        k = self.lib.nb_ensemble_tree_canonical(
+            self.h, s, cap, out["depth"].ctypes.data_as(_u8p), out["kind"].ctypes.data_as(_u8p),
+            out["body"].ctypes.data_as(_u32p), _d(out["mass"]), _d(out["com_x"]),
+            _d(out["com_y"]), _d(out["com_z"]), out["count"].ctypes.data_as(_u32p))
+        if k != cap:
+            raise NBodyError(-1, "nb_ensemble_tree_canonical", f"returned {k}, expected {cap}")
+        return out
+
+    @property
+    def tuning(self):
+        t = EnsembleTuning()
+        self._check(self.lib.nb_ensemble_get_tuning(self.h, ctypes.byref(t)), "nb_ensemble_get_tuning")
+        return dict(bh_threshold=t.bh_threshold, waves=t.waves)
+
+    def set_tuning(self, bh_threshold, waves):
+        t = EnsembleTuning(bh_threshold, waves)
+        self._check(self.lib.nb_ensemble_set_tuning(self.h, ctypes.byref(t)), "nb_ensemble_set_tuning")
+
+    def uses_bh(self):
+        f = np.zeros(self.sizes.size, np.uint8)
+        self._check(self.lib.nb_ensemble_uses_bh(self.h, f.ctypes.data_as(_u8p)), "nb_ensemble_uses_bh")
+        return f.astype(bool)
+
+    def autotune(self, steps=2):
+        t = EnsembleTuning()
+        ms = ctypes.c_double()
+        self._check(self.lib.nb_ensemble_autotune(self.h, steps, ctypes.byref(t), ctypes.byref(ms)),
+                    "nb_ensemble_autotune")
+        return dict(bh_threshold=t.bh_threshold, waves=t.waves), ms.value
+
+
+def gen_ensemble(nsys, seed, lo_log2=10, hi_log2=16):
+    """Sizes floor(2^U[lo, hi]) from a seeded generator; even systems are star +
+    disk (dt = 1 h), odd ones Plummer (dt = 1e-3).  Returns (sizes, state dict,
+    dt per system)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(2.0 ** rng.uniform(lo_log2, hi_log2, nsys)).astype(np.uint32)
+    parts, dts = [], []
+    for s, n in enumerate(sizes):
+        sub = (seed ^ ((s + 1) * 0x9E3779B97F4A7C15)) & 0xFFFFFFFFFFFFFFFF
+        d = gen_star_disk(int(n), sub) if s % 2 == 0 else gen_plummer(int(n), sub)
+        parts.append(d)
+        dts.append(1.0 / 24 if s % 2 == 0 else 1e-3)
+    state = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    return sizes, state, np.array(dts)
