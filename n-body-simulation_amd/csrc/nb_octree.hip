@@ -675,114 +675,148 @@ __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict_
 // the workgroup, and every cell is local (one centre-of-mass pass).
 constexpr int kForestThreads = 1024;
 constexpr int kForestWaves = kForestThreads / 64;
-constexpr int kSortBits = 5;  // radix digit width: 32 bins x 1024 threads of LDS counters
+constexpr int kSortBits = 8;  // radix digit width: 256 bins x 16 waves of LDS counters
 constexpr int kSortBins = 1 << kSortBits;
-constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;
+constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;  // 8: the result ends in place
+static_assert(kSortPasses % 2 == 0, "the sorted pairs must end in the input buffers");
 
-// Exclusive scan of v over the workgroup; *total receives the sum (all threads).
-__device__ inline uint32_t wg_scan_value(uint32_t v, uint32_t *wsum, uint32_t *total) {
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    uint32_t inc = v;
+// lanes below this one, as a ballot mask
+__device__ inline uint64_t forest_lanes_below() {
+    const uint32_t lane = threadIdx.x & 63u;
+    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Per-wave contiguous range [*b, *e) of m items: 16 ranges of whole 64-item
+// groups, so a wave reads 64 consecutive items at a time (coalesced) and the
+// ranges keep the item order wave by wave.
+__device__ inline void wave_range(uint32_t m, uint32_t *b, uint32_t *e) {
+    const uint32_t chunk = ((m + kForestWaves - 1) / kForestWaves + 63u) & ~63u;
+    const uint32_t w = threadIdx.x >> 6;
+    *b = min(m, w * chunk);
+    *e = min(m, *b + chunk);
+}
+
+// Exclusive scan in place of a[0..m) (global memory, this workgroup only):
+// wave totals, their scan, then a running wave scan over each wave's range.
+__device__ inline uint32_t wg_scan_array(uint32_t *a, uint32_t m, uint32_t *wsum) {
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    uint32_t b, e;
+    wave_range(m, &b, &e);
+    uint32_t s = 0;
+    for (uint32_t i = b + lane; i < e; i += 64) s += a[i];
 #pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(inc, off, 64);
-        if (lane >= (uint32_t)off) inc += o;
-    }
-    if (lane == 63) wsum[w] = inc;
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) wsum[w] = s;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (t == 0) {
         uint32_t run = 0;
         for (int k = 0; k < kForestWaves; ++k) {
-            const uint32_t t = wsum[k];
+            const uint32_t c = wsum[k];
             wsum[k] = run;
-            run += t;
+            run += c;
         }
         wsum[kForestWaves] = run;
     }
     __syncthreads();
-    const uint32_t excl = wsum[w] + inc - v;
-    *total = wsum[kForestWaves];
-    __syncthreads();
-    return excl;
-}
-
-// Exclusive scan in place of a[0..m) (global memory, this workgroup only).
-__device__ inline uint32_t wg_scan_array(uint32_t *a, uint32_t m, uint32_t *wsum) {
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (m + kForestThreads - 1) / kForestThreads;
-    const uint32_t b = min(m, t * per), e = min(m, b + per);
-    uint32_t s = 0;
-    for (uint32_t i = b; i < e; ++i) s += a[i];
-    uint32_t total;
-    uint32_t run = wg_scan_value(s, wsum, &total);
-    for (uint32_t i = b; i < e; ++i) {
-        const uint32_t v = a[i];
-        a[i] = run;
-        run += v;
+    uint32_t carry = wsum[w];
+    const uint32_t total = wsum[kForestWaves];
+    for (uint32_t g = b; g < e; g += 64) {
+        const uint32_t i = g + lane;
+        const uint32_t v = i < e ? a[i] : 0u;
+        uint32_t inc = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += o;
+        }
+        if (i < e) a[i] = carry + inc - v;
+        carry += __shfl(inc, 63, 64);
     }
     __syncthreads();
     return total;
 }
 
-// Stable LSD radix sort of (k, v) pairs by the 63-bit keys.  Thread t owns a
-// contiguous slice of the items, so scattering each slice in order keeps the
-// sort stable; hist[d * 1024 + t] counts digit d in slice t.
+// Lanes of the wave holding the same digit as this lane (valid lanes only).
+__device__ inline uint64_t digit_peers(uint32_t dig, bool valid) {
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kSortBits; ++b) {
+        const bool bit = (dig >> b) & 1u;
+        const uint64_t mask = __ballot(bit);
+        peers &= bit ? mask : ~mask;
+    }
+    return peers;
+}
+
+// Stable LSD radix sort of (k, v) pairs by the 63-bit keys, 8-bit digits,
+// (k2, v2) the ping-pong buffers.  Wave w owns a contiguous range and reads
+// it 64 consecutive pairs at a time; a pair's rank among the equal digits of
+// its 64 comes from ballots, and the offsets run digit-major over the waves
+// (whist[w * 256 + d]), so equal digits keep their order: stable.
 __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2,
-                                     uint32_t n, uint32_t *hist, uint32_t *wsum, uint32_t *dbase) {
+                                     uint32_t n, uint32_t *whist, uint32_t *wsum) {
     const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint32_t per = (n + kForestThreads - 1) / kForestThreads;
-    const uint32_t b = min(n, t * per), e = min(n, b + per);
+    const uint64_t below = forest_lanes_below();
+    uint32_t b, e;
+    wave_range(n, &b, &e);
+    uint32_t *row = whist + w * kSortBins;
     uint64_t *sk = k, *dk = k2;
     uint32_t *sv = v, *dv = v2;
     for (int p = 0; p < kSortPasses; ++p) {
         const int sh = kSortBits * p;
-        for (int d = 0; d < kSortBins; ++d) hist[d * kForestThreads + t] = 0;
-        for (uint32_t i = b; i < e; ++i) hist[((sk[i] >> sh) & (kSortBins - 1)) * kForestThreads + t]++;
+        for (uint32_t d = lane; d < (uint32_t)kSortBins; d += 64) row[d] = 0;
+        for (uint32_t g = b; g < e; g += 64) {
+            const uint32_t i = g + lane;
+            const bool valid = i < e;
+            const uint32_t dig = valid ? (uint32_t)(sk[i] >> sh) & (kSortBins - 1) : 0u;
+            const uint64_t peers = digit_peers(dig, valid);
+            if (valid && (peers & below) == 0) row[dig] += (uint32_t)__popcll(peers);
+        }
         __syncthreads();
-        // per digit: scan over the slices (wave shuffles), wave totals to LDS
-        uint32_t excl[kSortBins];
-#pragma unroll
-        for (int d = 0; d < kSortBins; ++d) {
-            const uint32_t c = hist[d * kForestThreads + t];
-            uint32_t inc = c;
+        // offsets: digit-major exclusive scan over (digit, wave); threads
+        // 0..255 (waves 0..3) take one digit each
+        uint32_t dig_excl = 0;
+        if (t < (uint32_t)kSortBins) {
+            uint32_t run = 0;
+            for (int ww = 0; ww < kForestWaves; ++ww) {
+                const uint32_t c = whist[ww * kSortBins + t];
+                whist[ww * kSortBins + t] = run;
+                run += c;
+            }
+            uint32_t inc = run;
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
                 const uint32_t o = __shfl_up(inc, off, 64);
                 if (lane >= (uint32_t)off) inc += o;
             }
-            excl[d] = inc - c;
-            if (lane == 63) wsum[d * kForestWaves + w] = inc;
+            dig_excl = inc - run;
+            if (lane == 63) wsum[w] = inc;
         }
         __syncthreads();
-        // wave offsets within each digit, then digit bases
-        if (t < kSortBins) {
-            uint32_t run = 0;
-            for (int ww = 0; ww < kForestWaves; ++ww) {
-                const uint32_t c = wsum[t * kForestWaves + ww];
-                wsum[t * kForestWaves + ww] = run;
-                run += c;
+        if (t < (uint32_t)kSortBins) {
+            for (uint32_t ww = 0; ww < w; ++ww) dig_excl += wsum[ww];
+            for (int ww = 0; ww < kForestWaves; ++ww) whist[ww * kSortBins + t] += dig_excl;
+        }
+        __syncthreads();
+        for (uint32_t g = b; g < e; g += 64) {
+            const uint32_t i = g + lane;
+            const bool valid = i < e;
+            uint64_t key = 0;
+            uint32_t val = 0;
+            if (valid) {
+                key = sk[i];
+                val = sv[i];
             }
-            dbase[t] = run;
-        }
-        __syncthreads();
-        if (t == 0) {
-            uint32_t run = 0;
-            for (int d = 0; d < kSortBins; ++d) {
-                const uint32_t c = dbase[d];
-                dbase[d] = run;
-                run += c;
+            const uint32_t dig = (uint32_t)(key >> sh) & (kSortBins - 1);
+            const uint64_t peers = digit_peers(dig, valid);
+            // every lane reads its digit's offset before the leader advances it
+            const uint32_t base = row[dig];
+            if (valid) {
+                const uint32_t pos = base + (uint32_t)__popcll(peers & below);
+                dk[pos] = key;
+                dv[pos] = val;
+                if ((peers & below) == 0) row[dig] = base + (uint32_t)__popcll(peers);
             }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int d = 0; d < kSortBins; ++d)
-            hist[d * kForestThreads + t] = dbase[d] + wsum[d * kForestWaves + w] + excl[d];
-        __syncthreads();
-        for (uint32_t i = b; i < e; ++i) {
-            const uint64_t key = sk[i];
-            const uint32_t pos = hist[((key >> sh) & (kSortBins - 1)) * kForestThreads + t]++;
-            dk[pos] = key;
-            dv[pos] = sv[i];
         }
         __syncthreads();
         uint64_t *tk = sk;
@@ -792,22 +826,16 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
         sv = dv;
         dv = tv;
     }
-    if (sk != k) {
-        for (uint32_t i = t; i < n; i += kForestThreads) {
-            k[i] = sk[i];
-            v[i] = sv[i];
-        }
-        __syncthreads();
-    }
 }
 
 __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order,
     const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
     const double *__restrict__ gm, double G, ForestBufs fb, uint32_t *items, uint32_t *status) {
-    __shared__ uint32_t hist[kSortBins * kForestThreads];
-    __shared__ uint32_t wsum[kSortBins * kForestWaves + 1];
-    __shared__ uint32_t dbase[kSortBins];
+    __shared__ uint32_t whist[kSortBins * kForestWaves];
+    __shared__ uint32_t wsum[kForestWaves + 1];
+    __shared__ uint32_t lvl_n[kMaxDepth + 2];  // cells per depth, then their start offsets
+    __shared__ uint32_t lvl_c[kMaxDepth + 1];  // fill cursors
     __shared__ double red[kForestWaves][6];
     __shared__ double cube[4];  // root cell {min x, min y, min z, edge}
     const uint32_t s = order[blockIdx.x];
@@ -894,7 +922,7 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
             sidx[i] = i;
         }
         __syncthreads();
-        wg_radix_sort(skey, sidx, skey_alt, sidx_alt, n, hist, wsum, dbase);
+        wg_radix_sort(skey, sidx, skey_alt, sidx_alt, n, whist, wsum);
     }
     // 4./5. lcp, deep runs, cell numbering
     for (uint32_t i = t; i + 1 < n; i += kForestThreads)
@@ -907,21 +935,46 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const uint32_t I = min(I_all, icap);
     // 6. topology
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
-    for (uint32_t j = t; j < I; j += kForestThreads) {
-        nchild[j] = 0;
-        arrive[j] = 0;
-    }
+    for (uint32_t j = t; j < I; j += kForestThreads) nchild[j] = 0;
+    if (t < (uint32_t)kMaxDepth + 2) lvl_n[t] = 0;
     __syncthreads();
     const KeyView kv{skey, klo};
     for (uint32_t i = t; i < n; i += kForestThreads)
         link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node, nchild,
                 body_parent, int_local, nullptr, 0u, int_first, n);
     __syncthreads();
-    // 7. centre of mass: every cell is local to this workgroup
-    for (uint32_t i = t; i < n; i += kForestThreads)
-        com_climb(i, icap, h, x, y, z, m, sidx, body_parent, ref, int_parent, nchild, int_local,
-                  arrive, sums, counts);
-    __syncthreads();
+    // 7. centre of mass, level by level from the deepest cells up: a cell's
+    // sums (its children in digit order, as in the one-system climb) need only
+    // deeper cells.  The cells are bucketed by depth into `lvl` (the arrival
+    // counters of the one-system climb, unused here).
+    {
+        uint32_t *lvl = arrive;
+        for (uint32_t i = t; i < n; i += kForestThreads)
+            if (m[i] == 0) raise_flag(h, kFlagZeroMass);
+        for (uint32_t k = t; k < I; k += kForestThreads)
+            atomicAdd(&lvl_n[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth)], 1u);
+        __syncthreads();
+        if (t == 0) {
+            uint32_t run = 0;
+            for (int d = 0; d <= kMaxDepth; ++d) {
+                const uint32_t c = lvl_n[d];
+                lvl_n[d] = run;
+                lvl_c[d] = run;
+                run += c;
+            }
+            lvl_n[kMaxDepth + 1] = run;
+        }
+        __syncthreads();
+        for (uint32_t k = t; k < I; k += kForestThreads)
+            lvl[atomicAdd(&lvl_c[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth)], 1u)] = k;
+        __syncthreads();
+        for (int d = kMaxDepth; d >= 0; --d) {
+            const uint32_t lb = lvl_n[d], le = lvl_n[d + 1];
+            for (uint32_t j = lb + t; j < le; j += kForestThreads)
+                sum_children(lvl[j], x, y, z, m, ref, sums, counts, h);
+            __syncthreads();
+        }
+    }
     // 8. records and traversal groups
     const double root_edge = root.e;
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
