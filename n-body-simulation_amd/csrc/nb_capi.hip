@@ -177,8 +177,7 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.int_depth, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_parent, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_node, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_nchild, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_arrive, t.icap))) return rc;
+    if ((rc = dev_alloc(c, &t.int_lvl, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_count, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_sums, t.icap))) return rc;
     if ((rc = dev_alloc(c, &t.int_local, t.icap))) return rc;
@@ -187,7 +186,7 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.group_size, n))) return rc;
     if ((rc = dev_alloc(c, &t.group_slot, n))) return rc;
     // a cell crossing a COM block boundary holds that boundary's two bodies:
-    // at most kMaxDepth such cells per boundary
+    // at most one per depth (kMaxDepth) and block
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
     if ((rc = dev_alloc(c, &t.spanning, t.spanning_cap))) return rc;
     if ((rc = dev_alloc(c, &t.ref, t.mcap))) return rc;
@@ -393,7 +392,7 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         TreeBufs &t = c->tree;
         void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
-                        t.int_nchild, t.int_arrive, t.int_count, t.int_sums, t.int_local,
+                        t.int_lvl, t.int_count, t.int_sums, t.int_local,
                         t.int_first, t.groups, t.group_size, t.group_slot, t.spanning, t.ref,
                         t.cm, t.edge, t.rec,
                         t.radix_hist, t.scan_part};

@@ -268,8 +268,8 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &f.group_slot, fo)) || (rc = ealloc(e, &f.lcp, fo)) ||
         (rc = ealloc(e, &f.groups, fo)) || (rc = ealloc(e, &f.int_depth, io)) ||
         (rc = ealloc(e, &f.int_local, io)) || (rc = ealloc(e, &f.int_parent, io)) ||
-        (rc = ealloc(e, &f.int_node, io)) || (rc = ealloc(e, &f.int_nchild, io)) ||
-        (rc = ealloc(e, &f.int_arrive, io)) || (rc = ealloc(e, &f.int_count, io)) ||
+        (rc = ealloc(e, &f.int_node, io)) || (rc = ealloc(e, &f.int_lvl, io)) ||
+        (rc = ealloc(e, &f.int_count, io)) ||
         (rc = ealloc(e, &f.int_first, io)) || (rc = ealloc(e, &f.int_sums, io)) ||
         (rc = ealloc(e, &f.ref, no)) || (rc = ealloc(e, &f.rec, no)) ||
         (rc = ealloc(e, &e->items, nsys)) || (rc = ealloc(e, &e->item_off, nsys + 1ull)) ||

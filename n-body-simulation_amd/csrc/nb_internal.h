@@ -33,6 +33,7 @@ enum : uint32_t {
 
 // node reference encoding
 constexpr int32_t kRefEmpty = -1;  // empty leaf
+constexpr uint32_t kNoCell = 0xffffffffu;  // no internal cell (tables, parents)
 __host__ __device__ inline int32_t ref_of_body(uint32_t b) { return -2 - (int32_t)b; }
 __host__ __device__ inline uint32_t body_of_ref(int32_t r) { return (uint32_t)(-2 - r); }
 
@@ -44,8 +45,7 @@ struct alignas(128) TreeHdr {
     uint32_t max_depth;                                // deepest leaf
     uint32_t status;                                   // kFlag* bits
     uint32_t deep_runs;                                // runs of bodies sharing 21 digits
-    uint32_t spanning_count;                           // cells crossing a COM block boundary
-    uint32_t stack_peak;                               // deepest group-traversal stack seen
+    uint32_t stack_peak;                              // deepest group-traversal stack seen
     uint32_t group_count;                              // traversal groups
     uint32_t group_next;                               // traversal work-queue cursor
     unsigned long long corr_pairs;                     // learned-correction evaluations
@@ -106,15 +106,15 @@ struct TreeBufs {
     uint32_t *chain = nullptr;    // n entries: internal nodes owned by pair i, then offsets
     uint32_t *body_parent = nullptr;  // n: internal index of each sorted body's parent
     uint8_t *int_depth = nullptr;
-    uint32_t *int_parent = nullptr, *int_node = nullptr, *int_nchild = nullptr,
-             *int_arrive = nullptr, *int_count = nullptr;
+    uint32_t *int_parent = nullptr, *int_node = nullptr, *int_count = nullptr;
+    uint32_t *int_lvl = nullptr;    // centre of mass: cells bucketed by depth
     InternalSums *int_sums = nullptr;
     uint8_t *int_local = nullptr;   // 1 if the cell's bodies lie in one kComBlock block
     uint32_t *int_first = nullptr;  // first sorted position of the cell's bodies
     uint2 *groups = nullptr;        // traversal groups {first sorted position, body count}
     uint32_t *group_size = nullptr; // per sorted position: size of the group starting there
     uint32_t *group_slot = nullptr; // per sorted position: 1 if a group starts there, then slots
-    uint32_t *spanning = nullptr;   // cells crossing a block boundary
+    uint32_t *spanning = nullptr;   // [block * kMaxDepth + depth]: cell crossing the block's end
     uint32_t spanning_cap = 0;
     int32_t *ref = nullptr;
     double4 *cm = nullptr;          // FP64 records (reference-semantics traversal)
@@ -152,7 +152,7 @@ struct ForestBufs {
     uint2 *groups;
     // per internal cell (offset int_off)
     uint8_t *int_depth, *int_local;
-    uint32_t *int_parent, *int_node, *int_nchild, *int_arrive, *int_count, *int_first;
+    uint32_t *int_parent, *int_node, *int_lvl, *int_count, *int_first;
     InternalSums *int_sums;
     // per node (offset node_off)
     int32_t *ref;

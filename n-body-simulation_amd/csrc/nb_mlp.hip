@@ -555,7 +555,8 @@ int mlp_fail(nb_mlp *n, int code, const std::string &msg) {
 
 // flat 9411-parameter layout <-> padded device layout
 int flat_to_padded(int i) {
-    constexpr int fW1 = 0, fB1 = 832, fW2 = 896, fB2 = 4992, fW3 = 5056, fB3 = 9152, fW4 = 9216,
+    // flat offsets: W1 at 0, then B1, W2, B2, W3, B3, W4, B4
+    constexpr int fB1 = 832, fW2 = 896, fB2 = 4992, fW3 = 5056, fB3 = 9152, fW4 = 9216,
                   fB4 = 9408;
     if (i < fB1) return kPW1 + i;  // W1[13][64] rows keep their stride
     if (i < fW2) return kPB1 + (i - fB1);

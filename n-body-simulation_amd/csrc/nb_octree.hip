@@ -56,7 +56,6 @@ __global__ void __launch_bounds__(kThreads) k_build_begin(TreeHdr *h) {
         h->max_depth = 0;
         h->status = 0;
         h->deep_runs = 0;
-        h->spanning_count = 0;
         h->stack_peak = 0;
         h->group_count = 0;
         h->group_next = 0;
@@ -282,11 +281,33 @@ __device__ inline void chain_at(uint32_t i, const int32_t *__restrict__ lcp, uin
     chain[i] = cn > cp ? (uint32_t)(cn - cp) : 0u;
 }
 
+// h->max_depth = depth of the deepest leaf = 1 + the deepest cell that two
+// sorted neighbours share (max lcp), for n >= 2.  One workgroup maximum; the
+// device-scope RMW only when it raises the stored value, so the build does
+// not serialise on one address.
+__device__ inline void raise_max_depth(TreeHdr *h, uint32_t d) {
+    if (d > __hip_atomic_load(&h->max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&h->max_depth, d);
+}
+
 __global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ lcp, uint32_t n,
-                                                    uint32_t *chain) {
+                                                    uint32_t *chain, TreeHdr *h) {
+    __shared__ int wmax[kThreads / 64];
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i + 1 >= n) return;
-    chain_at(i, lcp, chain);
+    int v = -1;
+    if (i + 1 < n) {
+        chain_at(i, lcp, chain);
+        v = lcp[i];
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63u) == 0) wmax[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmax[0];
+        for (int w = 1; w < kThreads / 64; ++w) b = max(b, wmax[w]);
+        if (b >= 0) raise_max_depth(h, (uint32_t)b + 1);
+    }
 }
 
 // ---- 6. topology --------------------------------------------------------------
@@ -328,31 +349,29 @@ struct KeyView {
 };
 
 __global__ void __launch_bounds__(kThreads) k_fill(TreeHdr *h, uint32_t icap, int32_t *ref,
-                                                   uint32_t *nchild, uint32_t *arrive) {
+                                                   uint32_t *spanning, uint32_t spanning_cap) {
     const uint32_t I = min(h->internal_count, icap);
     const uint64_t nodes = 1 + 8ull * I;
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride)
         ref[j] = kRefEmpty;
-    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < I; j += stride) {
-        nchild[j] = 0;
-        arrive[j] = 0;
-    }
+    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < spanning_cap; j += stride)
+        spanning[j] = kNoCell;
 }
 
 // Internal cells owned by pair i and the leaf of sorted body i.  A cell is
 // "local" when its bodies stay inside one com_block-body block of the sorted
-// order; the others are listed in `spanning`.
+// order; the others go to the table spanning[block * kMaxDepth + depth].
+// No atomics: every cell, node slot and table entry has exactly one writer.
 __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h, KeyView kv,
                                const uint32_t *__restrict__ sidx, const int32_t *__restrict__ lcp,
                                const uint32_t *__restrict__ off, int32_t *ref, uint8_t *int_depth,
-                               uint32_t *int_parent, uint32_t *int_node, uint32_t *nchild,
-                               uint32_t *body_parent, uint8_t *int_local, uint32_t *spanning,
-                               uint32_t spanning_cap, uint32_t *int_first, uint32_t com_block) {
-    if (n == 1) {  // the root is a leaf holding the only body
+                               uint32_t *int_parent, uint32_t *int_node, uint32_t *body_parent,
+                               uint8_t *int_local, uint32_t *spanning, uint32_t spanning_cap,
+                               uint32_t *int_first, uint32_t com_block) {
+    if (n == 1) {  // the root is a leaf holding the only body (max_depth stays 0)
         ref[0] = ref_of_body(sidx[0]);
         body_parent[0] = 0xffffffffu;
-        atomicMax(&h->max_depth, 0u);
         return;
     }
     const int cp = i > 0 ? lcp[i - 1] : -1;
@@ -376,7 +395,10 @@ __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h
         const bool local = !(next_block < n && kv.same(next_block, i, l));
         int_local[k] = local ? 1 : 0;
         if (!local) {
-            const uint32_t slot = atomicAdd(&h->spanning_count, 1u);
+            // the block of body i owns at most one crossing cell per depth:
+            // the depth-l cell holding the block's last and the next block's
+            // first body
+            const uint64_t slot = (uint64_t)(i / com_block) * kMaxDepth + (uint32_t)l;
             if (slot < spanning_cap) spanning[slot] = k;
             else raise_flag(h, kFlagNodeOverflow);
         }
@@ -395,7 +417,6 @@ __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h
         int_node[k] = node;
         int_parent[k] = pk;
         ref[node] = (int32_t)k;
-        atomicAdd(&nchild[pk], 1u);
     }
     // the body's leaf: one level below the deepest cell it shares with a neighbour
     const int D = cp > cn ? cp : cn;
@@ -408,8 +429,7 @@ __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h
     const uint32_t leaf = 1u + 8u * pk + kv.digit(i, D);
     ref[leaf] = ref_of_body(sidx[i]);
     body_parent[i] = pk;
-    atomicAdd(&nchild[pk], 1u);
-    atomicMax(&h->max_depth, (uint32_t)(D + 1));
+    // the deepest leaf, 1 + max lcp, is taken from the lcp array (max_depth_of)
 }
 
 __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, TreeHdr *h,
@@ -418,13 +438,13 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
                                                    const uint32_t *__restrict__ off,
                                                    int32_t *ref, uint8_t *int_depth,
                                                    uint32_t *int_parent, uint32_t *int_node,
-                                                   uint32_t *nchild, uint32_t *body_parent,
-                                                   uint8_t *int_local, uint32_t *spanning,
-                                                   uint32_t spanning_cap, uint32_t *int_first) {
+                                                   uint32_t *body_parent, uint8_t *int_local,
+                                                   uint32_t *spanning, uint32_t spanning_cap,
+                                                   uint32_t *int_first) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    link_at(i, n, icap, h, kv, sidx, lcp, off, ref, int_depth, int_parent, int_node, nchild,
-            body_parent, int_local, spanning, spanning_cap, int_first, kComBlock);
+    link_at(i, n, icap, h, kv, sidx, lcp, off, ref, int_depth, int_parent, int_node, body_parent,
+            int_local, spanning, spanning_cap, int_first, kComBlock);
 }
 
 // ---- 7. centre of mass --------------------------------------------------------
@@ -463,84 +483,92 @@ __device__ inline void sum_children(uint32_t pk, const double *__restrict__ x,
     if (sm == 0) raise_flag(h, kFlagZeroMass);
 }
 
-__device__ inline uint32_t arrive_workgroup(uint32_t *p) {
-    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Pass 1: one workgroup per kComBlock consecutive (sorted) bodies.  A thread
-// starts at each body's leaf and climbs while the parent cell lies inside the
-// block; the last child to arrive at a cell sums it.  All arrivals at such a
-// cell come from this workgroup, so workgroup-scope atomics suffice (no L2
-// write-back or invalidation across the chip's XCDs).
+// Centre of mass, level by level from the deepest cells up: a cell's sums
+// (its children in digit order) need only deeper cells, so every cell of one
+// depth can be summed at once, with a barrier between depths.
 constexpr int kComThreads = 1024;
 
-// Climb from sorted body i's leaf while the cells are local to this workgroup.
-__device__ inline void com_climb(uint32_t i, uint32_t icap, TreeHdr *h,
-                                 const double *__restrict__ x, const double *__restrict__ y,
-                                 const double *__restrict__ z, const double *__restrict__ m,
-                                 const uint32_t *__restrict__ sidx,
-                                 const uint32_t *__restrict__ body_parent,
-                                 const int32_t *__restrict__ ref,
-                                 const uint32_t *__restrict__ int_parent,
-                                 const uint32_t *__restrict__ nchild,
-                                 const uint8_t *__restrict__ int_local, uint32_t *arrive,
-                                 InternalSums *sums, uint32_t *counts) {
-    if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
-    uint32_t pk = body_parent[i];
-    if (pk == 0xffffffffu) return;  // single body (root leaf) or overflow
-    // a leaf has at most kMaxDepth ancestors; the bound guards a corrupt chain
-    for (int level = 0; level <= kMaxDepth; ++level) {
-        if (!int_local[pk]) break;  // crosses a block boundary: pass 2
-        const uint32_t seen = arrive_workgroup(&arrive[pk]);
-        // The child counts were built with device-scope atomics (in L2); when
-        // the same kernel produced them (ensemble build), a plain load could
-        // hit a stale L1 line, so read them the way they were written.
-        const uint32_t want = __hip_atomic_load(&nchild[pk], __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_AGENT);
-        if (seen + 1 != want) break;  // a sibling finishes this cell
-        sum_children(pk, x, y, z, m, ref, sums, counts, h);
-        if (pk == 0) break;
-        pk = int_parent[pk];
-        if (pk >= icap) {
-            raise_flag(h, kFlagNodeOverflow);
-            break;
+// Cells [kb, ke) bucketed by depth into lvl[kb ..) (cells with keep(k) only),
+// then summed deepest depth first.  cnt/cur: kMaxDepth + 1 and kMaxDepth
+// LDS words.  Every thread of the workgroup calls it.
+template <typename Keep>
+__device__ inline void com_levels(uint32_t kb, uint32_t ke, Keep keep,
+                                  const uint8_t *__restrict__ int_depth, uint32_t *lvl,
+                                  uint32_t *cnt, uint32_t *cur, const double *__restrict__ x,
+                                  const double *__restrict__ y, const double *__restrict__ z,
+                                  const double *__restrict__ m, const int32_t *__restrict__ ref,
+                                  InternalSums *sums, uint32_t *counts, TreeHdr *h) {
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += nt) cnt[d] = 0;
+    __syncthreads();
+    for (uint32_t k = kb + t; k < ke; k += nt)
+        if (keep(k)) atomicAdd(&cnt[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kMaxDepth; ++d) {
+            const uint32_t c = cnt[d];
+            cnt[d] = run;
+            cur[d] = run;
+            run += c;
         }
+        cnt[kMaxDepth] = run;
+    }
+    __syncthreads();
+    for (uint32_t k = kb + t; k < ke; k += nt)
+        if (keep(k))
+            lvl[kb + atomicAdd(&cur[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u)] = k;
+    __syncthreads();
+    for (int d = kMaxDepth - 1; d >= 0; --d) {
+        const uint32_t lb = cnt[d], le = cnt[d + 1];
+        if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
+        for (uint32_t j = lb + t; j < le; j += nt)
+            sum_children(lvl[kb + j], x, y, z, m, ref, sums, counts, h);
+        __syncthreads();
     }
 }
 
+// Pass 1: one workgroup per kComBlock consecutive (sorted) bodies sums the
+// cells that lie inside its block.  Pair i owns cells [off[i], off[i+1]), so
+// the block's cells are one contiguous range; lvl (one word per cell) holds
+// the depth buckets.
 __global__ void __launch_bounds__(kComThreads) k_com_local(
     uint32_t n, uint32_t icap, TreeHdr *h, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const uint32_t *__restrict__ sidx, const uint32_t *__restrict__ body_parent,
-    const int32_t *__restrict__ ref, const uint32_t *__restrict__ int_parent,
-    const uint32_t *__restrict__ nchild, const uint8_t *__restrict__ int_local,
-    uint32_t *arrive, InternalSums *sums, uint32_t *counts) {
+    const uint32_t *__restrict__ sidx, const int32_t *__restrict__ ref,
+    const uint32_t *__restrict__ off, const uint8_t *__restrict__ int_depth,
+    const uint8_t *__restrict__ int_local, uint32_t *lvl, InternalSums *sums, uint32_t *counts) {
+    __shared__ uint32_t cnt[kMaxDepth + 1], cur[kMaxDepth];
     const uint32_t s = blockIdx.x * kComBlock;
     const uint32_t e = min(n, s + kComBlock);
     for (uint32_t i = s + threadIdx.x; i < e; i += kComThreads)
-        com_climb(i, icap, h, x, y, z, m, sidx, body_parent, ref, int_parent, nchild, int_local,
-                  arrive, sums, counts);
+        if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
+    if (n < 2) return;  // a lone body is the root leaf: no internal cell
+    const uint32_t I = min(h->internal_count, icap);
+    const uint32_t kb = s < n - 1 ? min(off[s], I) : I;
+    const uint32_t ke = e < n - 1 ? min(off[e], I) : I;
+    com_levels(kb, ke, [&](uint32_t k) { return int_local[k] != 0; }, int_depth, lvl, cnt, cur,
+               x, y, z, m, ref, sums, counts, h);
 }
 
-// Pass 2: one workgroup finishes the cells that cross block boundaries,
-// deepest level first, with a barrier between levels.
+// Pass 2: one workgroup sums the cells that cross block boundaries, the
+// table spanning[block * kMaxDepth + depth], deepest depth first.
 __global__ void __launch_bounds__(kComThreads) k_com_spanning(
-    uint32_t spanning_cap, TreeHdr *h, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
-    const int32_t *__restrict__ ref, const uint8_t *__restrict__ int_depth,
+    uint32_t nblocks, TreeHdr *h, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, const int32_t *__restrict__ ref,
     const uint32_t *__restrict__ spanning, InternalSums *sums, uint32_t *counts) {
-    const uint32_t M = min(h->spanning_count, spanning_cap);
-    if (M == 0) return;
-    __shared__ uint32_t deepest;
-    if (threadIdx.x == 0) deepest = 0;
+    __shared__ uint32_t live[kMaxDepth];
+    const uint32_t t = threadIdx.x;
+    if (t < (uint32_t)kMaxDepth) live[t] = 0;
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < M; j += kComThreads)
-        atomicMax(&deepest, (uint32_t)int_depth[spanning[j]]);
+    for (uint32_t j = t; j < nblocks * (uint32_t)kMaxDepth; j += kComThreads)
+        if (spanning[j] != kNoCell) live[j % kMaxDepth] = 1;
     __syncthreads();
-    for (int level = (int)deepest; level >= 0; --level) {
-        for (uint32_t j = threadIdx.x; j < M; j += kComThreads) {
-            const uint32_t k = spanning[j];
-            if (int_depth[k] == level) sum_children(k, x, y, z, m, ref, sums, counts, h);
+    for (int d = kMaxDepth - 1; d >= 0; --d) {
+        if (!live[d]) continue;  // the same for every thread: LDS, read after a barrier
+        for (uint32_t b = t; b < nblocks; b += kComThreads) {
+            const uint32_t k = spanning[(uint64_t)b * kMaxDepth + d];
+            if (k != kNoCell) sum_children(k, x, y, z, m, ref, sums, counts, h);
         }
         __syncthreads();
     }
@@ -834,8 +862,9 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const double *__restrict__ gm, double G, ForestBufs fb, uint32_t *items, uint32_t *status) {
     __shared__ uint32_t whist[kSortBins * kForestWaves];
     __shared__ uint32_t wsum[kForestWaves + 1];
-    __shared__ uint32_t lvl_n[kMaxDepth + 2];  // cells per depth, then their start offsets
-    __shared__ uint32_t lvl_c[kMaxDepth + 1];  // fill cursors
+    __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
+    __shared__ uint32_t lvl_c[kMaxDepth];      // fill cursors
+    __shared__ int lcp_max;
     __shared__ double red[kForestWaves][6];
     __shared__ double cube[4];  // root cell {min x, min y, min z, edge}
     const uint32_t s = order[blockIdx.x];
@@ -853,8 +882,8 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     uint2 *groups = fb.groups + fo;
     uint8_t *int_depth = fb.int_depth + S.int_off, *int_local = fb.int_local + S.int_off;
     uint32_t *int_parent = fb.int_parent + S.int_off, *int_node = fb.int_node + S.int_off,
-             *nchild = fb.int_nchild + S.int_off, *arrive = fb.int_arrive + S.int_off,
-             *counts = fb.int_count + S.int_off, *int_first = fb.int_first + S.int_off;
+             *lvl = fb.int_lvl + S.int_off, *counts = fb.int_count + S.int_off,
+             *int_first = fb.int_first + S.int_off;
     InternalSums *sums = fb.int_sums + S.int_off;
     int32_t *ref = fb.ref + S.node_off;
     NodeF32 *rec = fb.rec + S.node_off;
@@ -865,11 +894,11 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
         h->max_depth = 0;
         h->status = 0;
         h->deep_runs = 0;
-        h->spanning_count = 0;
         h->stack_peak = 0;
         h->group_count = 0;
         h->group_next = 0;
         h->corr_pairs = 0;
+        lcp_max = -1;
     }
     __syncthreads();  // the reset precedes every flag raised below
     // 1. AABB including the origin (every partial starts at 0.0), then cubify
@@ -928,53 +957,34 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     for (uint32_t i = t; i + 1 < n; i += kForestThreads)
         lcp_at(i, x, y, z, n, root, h, skey, sidx, klo, lcp);
     __syncthreads();
-    for (uint32_t i = t; i + 1 < n; i += kForestThreads) chain_at(i, lcp, chain);
+    {
+        int lmax = -1;  // deepest cell shared by two neighbours
+        for (uint32_t i = t; i + 1 < n; i += kForestThreads) {
+            chain_at(i, lcp, chain);
+            lmax = max(lmax, lcp[i]);
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
+        if ((t & 63u) == 0) atomicMax(&lcp_max, lmax);
+    }
     __syncthreads();
+    if (t == 0 && lcp_max >= 0) h->max_depth = (uint32_t)lcp_max + 1;  // the deepest leaf
     const uint32_t I_all = n > 1 ? wg_scan_array(chain, n - 1, wsum) : 0u;
     if (t == 0) h->internal_count = I_all;
     const uint32_t I = min(I_all, icap);
     // 6. topology
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
-    for (uint32_t j = t; j < I; j += kForestThreads) nchild[j] = 0;
-    if (t < (uint32_t)kMaxDepth + 2) lvl_n[t] = 0;
     __syncthreads();
     const KeyView kv{skey, klo};
     for (uint32_t i = t; i < n; i += kForestThreads)
-        link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node, nchild,
+        link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node,
                 body_parent, int_local, nullptr, 0u, int_first, n);
     __syncthreads();
-    // 7. centre of mass, level by level from the deepest cells up: a cell's
-    // sums (its children in digit order, as in the one-system climb) need only
-    // deeper cells.  The cells are bucketed by depth into `lvl` (the arrival
-    // counters of the one-system climb, unused here).
-    {
-        uint32_t *lvl = arrive;
-        for (uint32_t i = t; i < n; i += kForestThreads)
-            if (m[i] == 0) raise_flag(h, kFlagZeroMass);
-        for (uint32_t k = t; k < I; k += kForestThreads)
-            atomicAdd(&lvl_n[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth)], 1u);
-        __syncthreads();
-        if (t == 0) {
-            uint32_t run = 0;
-            for (int d = 0; d <= kMaxDepth; ++d) {
-                const uint32_t c = lvl_n[d];
-                lvl_n[d] = run;
-                lvl_c[d] = run;
-                run += c;
-            }
-            lvl_n[kMaxDepth + 1] = run;
-        }
-        __syncthreads();
-        for (uint32_t k = t; k < I; k += kForestThreads)
-            lvl[atomicAdd(&lvl_c[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth)], 1u)] = k;
-        __syncthreads();
-        for (int d = kMaxDepth; d >= 0; --d) {
-            const uint32_t lb = lvl_n[d], le = lvl_n[d + 1];
-            for (uint32_t j = lb + t; j < le; j += kForestThreads)
-                sum_children(lvl[j], x, y, z, m, ref, sums, counts, h);
-            __syncthreads();
-        }
-    }
+    // 7. centre of mass, level by level: every cell is local to this workgroup
+    for (uint32_t i = t; i < n; i += kForestThreads)
+        if (m[i] == 0) raise_flag(h, kFlagZeroMass);
+    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c, x, y, z, m,
+               ref, sums, counts, h);
     // 8. records and traversal groups
     const double root_edge = root.e;
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
@@ -1037,31 +1047,30 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
         hipLaunchKernelGGL(k_lcp, grid, dim3(kThreads), 0, s, x, y, z, n, t.hdr, t.skey, t.sidx,
                            t.klo, t.lcp);
         hipLaunchKernelGGL(k_chain, grid, dim3(kThreads), 0, s, (const int32_t *)t.lcp, n,
-                           t.chain);
+                           t.chain, t.hdr);
         scan_exclusive_u32(t.chain, t.chain, n - 1, &t.hdr->internal_count, t.scan_part,
                            t.scan_part_cap, s);
     }
     const uint32_t fill_blocks = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(8ull * n, kThreads)));
     hipLaunchKernelGGL(k_fill, dim3(fill_blocks), dim3(kThreads), 0, s, t.hdr, t.icap, t.ref,
-                       t.int_nchild, t.int_arrive);
+                       t.spanning, t.spanning_cap);
     KeyView kv{t.skey, t.klo};
     hipLaunchKernelGGL(k_link, grid, dim3(kThreads), 0, s, n, t.icap, t.hdr, kv,
                        (const uint32_t *)t.sidx, (const int32_t *)t.lcp,
                        (const uint32_t *)t.chain, t.ref, t.int_depth, t.int_parent, t.int_node,
-                       t.int_nchild, t.body_parent, t.int_local, t.spanning, t.spanning_cap,
-                       t.int_first);
+                       t.body_parent, t.int_local, t.spanning, t.spanning_cap, t.int_first);
 }
 
 void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
                 const double *m, hipStream_t s) {
-    hipLaunchKernelGGL(k_com_local, dim3(ceil_div(t.n, kComBlock)), dim3(kComThreads), 0, s, t.n,
-                       t.icap, t.hdr, x, y, z, m, (const uint32_t *)t.sidx,
-                       (const uint32_t *)t.body_parent, (const int32_t *)t.ref,
-                       (const uint32_t *)t.int_parent, (const uint32_t *)t.int_nchild,
-                       (const uint8_t *)t.int_local, t.int_arrive, t.int_sums, t.int_count);
-    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, t.spanning_cap, t.hdr, x,
-                       y, z, m, (const int32_t *)t.ref, (const uint8_t *)t.int_depth,
-                       (const uint32_t *)t.spanning, t.int_sums, t.int_count);
+    const uint32_t nblocks = ceil_div(t.n, kComBlock);
+    hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComThreads), 0, s, t.n, t.icap, t.hdr, x,
+                       y, z, m, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
+                       (const uint32_t *)t.chain, (const uint8_t *)t.int_depth,
+                       (const uint8_t *)t.int_local, t.int_lvl, t.int_sums, t.int_count);
+    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, nblocks, t.hdr, x, y, z,
+                       m, (const int32_t *)t.ref, (const uint32_t *)t.spanning, t.int_sums,
+                       t.int_count);
 }
 
 void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
