@@ -620,6 +620,7 @@ int nb_phase_ms(const nb_ctx *c, double *ms) {
 int nb_tree_info(nb_ctx *c, uint64_t *node_count, uint32_t *internal_count, uint32_t *max_depth,
                  double aabb[7]) {
     if (!c) return NB_E_INVALID_ARG;
+    if (c->tree_valid) nb::launch_max_depth(c->tree, c->stream);  // computed on demand
     NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
                              c->stream));
     NB_HIP(c, hipStreamSynchronize(c->stream));

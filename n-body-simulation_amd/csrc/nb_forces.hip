@@ -335,6 +335,29 @@ struct GroupTree {
     uint32_t *inter;
 };
 
+// Pair forces of list entries p[0], p[LPB], p[2 LPB], ... (t entries, in that
+// order) on the point (rx, ry, rz).  LPB and the uniform trip count make the
+// LDS offsets immediates and let four reads be in flight ahead of the maths.
+template <int LPB>
+__device__ __forceinline__ void flush_share(const float4 *p, uint32_t t, float rx, float ry,
+                                            float rz, float eps2, float &fx, float &fy,
+                                            float &fz) {
+    uint32_t k = 0;
+    for (; k + 4 <= t; k += 4) {
+        const float4 a = p[(k + 0) * LPB], b = p[(k + 1) * LPB], c = p[(k + 2) * LPB],
+                     d = p[(k + 3) * LPB];
+        pair_f32(a, rx, ry, rz, eps2, fx, fy, fz);
+        pair_f32(b, rx, ry, rz, eps2, fx, fy, fz);
+        pair_f32(c, rx, ry, rz, eps2, fx, fy, fz);
+        pair_f32(d, rx, ry, rz, eps2, fx, fy, fz);
+    }
+    for (; k < t; ++k) pair_f32(p[k * LPB], rx, ry, rz, eps2, fx, fy, fz);
+}
+
+// A list entry that adds exactly zero for any eps^2: its distance squared
+// overflows to +inf, so rsqrt gives 0 and the force 0 * offset = 0.
+__device__ inline float4 far_entry() { return make_float4(1e30f, 0.f, 0.f, 0.f); }
+
 // LDS of one wave's walk.
 struct GroupLds {
     uint32_t *stk;
@@ -426,9 +449,27 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
             bcorr[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
 
+        // Entries lst[0 .. cnt) (cnt <= kGroup): lane share q takes q, q+lpb,
+        // ...  The list is padded with zero-force entries to a multiple of
+        // lpb, so every lane runs the same (uniform) trip count.
         auto flush = [&](uint32_t cnt) {
+            const uint32_t cp = (cnt + lpb - 1) / lpb * lpb;
+            if (cp > cnt) {
+                if (lane < cp - cnt) lst[cnt + lane] = far_entry();
+                __syncthreads();
+            }
+            const uint32_t trips = cp / lpb;
+            const float4 *p = lst + q;
             float fx = 0.f, fy = 0.f, fz = 0.f;
-            for (uint32_t e = q; e < cnt; e += lpb) pair_f32(lst[e], rx, ry, rz, eps2, fx, fy, fz);
+            switch (lpb) {
+            case 1: flush_share<1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            case 2: flush_share<2>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            case 4: flush_share<4>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            case 8: flush_share<8>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            case 16: flush_share<16>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            case 32: flush_share<32>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            default: flush_share<64>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            }
             acc_x += (double)fx;
             acc_y += (double)fy;
             acc_z += (double)fz;

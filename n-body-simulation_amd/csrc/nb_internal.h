@@ -181,6 +181,8 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
                      hipStream_t s);
 void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
                 const double *m, hipStream_t s);
+// hdr->max_depth of the last build (not maintained by the build itself)
+void launch_max_depth(const TreeBufs &t, hipStream_t s);
 // f64: FP64 records (cm, edge) for the reference-semantics traversal;
 // f32: packed FP32 records and traversal groups for the group traversal.
 void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,

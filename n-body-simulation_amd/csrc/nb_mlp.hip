@@ -496,6 +496,7 @@ __global__ void __launch_bounds__(64) k_pair_targets(
 void launch_quadrupoles(const TreeBufs &t, const double *x, const double *y, const double *z,
                         const double *m, double *quad, MlpCell *mcell, hipStream_t s) {
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(t.icap, kThreads)));
+    launch_max_depth(t, s);  // k_quad_level skips the levels below the deepest leaf
     for (int level = kMaxDepth - 1; level >= 0; --level)
         hipLaunchKernelGGL(k_quad_level, dim3(blocks), dim3(kThreads), 0, s, level, t.icap,
                            (const TreeHdr *)t.hdr, x, y, z, m, (const int32_t *)t.ref,

@@ -281,24 +281,25 @@ __device__ inline void chain_at(uint32_t i, const int32_t *__restrict__ lcp, uin
     chain[i] = cn > cp ? (uint32_t)(cn - cp) : 0u;
 }
 
-// h->max_depth = depth of the deepest leaf = 1 + the deepest cell that two
-// sorted neighbours share (max lcp), for n >= 2.  One workgroup maximum; the
-// device-scope RMW only when it raises the stored value, so the build does
-// not serialise on one address.
-__device__ inline void raise_max_depth(TreeHdr *h, uint32_t d) {
-    if (d > __hip_atomic_load(&h->max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(&h->max_depth, d);
+__global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ lcp, uint32_t n,
+                                                    uint32_t *chain) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i + 1 >= n) return;
+    chain_at(i, lcp, chain);
 }
 
-__global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ lcp, uint32_t n,
-                                                    uint32_t *chain, TreeHdr *h) {
+// h->max_depth = depth of the deepest leaf = 1 + the deepest cell that two
+// sorted neighbours share (max lcp), for n >= 2; 0 for one body.  Run on
+// demand (tree queries, cell moments), not in every build: a grid-stride
+// maximum per workgroup, and the device-scope RMW only when it raises the
+// stored value.
+constexpr int kDepthBlocks = 256;
+__global__ void __launch_bounds__(kThreads) k_max_depth(const int32_t *__restrict__ lcp, uint32_t n,
+                                                        TreeHdr *h) {
     __shared__ int wmax[kThreads / 64];
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     int v = -1;
-    if (i + 1 < n) {
-        chain_at(i, lcp, chain);
-        v = lcp[i];
-    }
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i + 1 < n; i += gridDim.x * kThreads)
+        v = max(v, lcp[i]);
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63u) == 0) wmax[threadIdx.x >> 6] = v;
@@ -306,7 +307,10 @@ __global__ void __launch_bounds__(kThreads) k_chain(const int32_t *__restrict__ 
     if (threadIdx.x == 0) {
         int b = wmax[0];
         for (int w = 1; w < kThreads / 64; ++w) b = max(b, wmax[w]);
-        if (b >= 0) raise_max_depth(h, (uint32_t)b + 1);
+        const uint32_t d = (uint32_t)(b + 1);
+        if (b >= 0 &&
+            d > __hip_atomic_load(&h->max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&h->max_depth, d);
     }
 }
 
@@ -1047,7 +1051,7 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
         hipLaunchKernelGGL(k_lcp, grid, dim3(kThreads), 0, s, x, y, z, n, t.hdr, t.skey, t.sidx,
                            t.klo, t.lcp);
         hipLaunchKernelGGL(k_chain, grid, dim3(kThreads), 0, s, (const int32_t *)t.lcp, n,
-                           t.chain, t.hdr);
+                           t.chain);
         scan_exclusive_u32(t.chain, t.chain, n - 1, &t.hdr->internal_count, t.scan_part,
                            t.scan_part_cap, s);
     }
@@ -1059,6 +1063,13 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
                        (const uint32_t *)t.sidx, (const int32_t *)t.lcp,
                        (const uint32_t *)t.chain, t.ref, t.int_depth, t.int_parent, t.int_node,
                        t.body_parent, t.int_local, t.spanning, t.spanning_cap, t.int_first);
+}
+
+void launch_max_depth(const TreeBufs &t, hipStream_t s) {
+    if (t.n < 2) return;
+    const uint32_t blocks = std::min<uint32_t>(kDepthBlocks, ceil_div(t.n, kThreads));
+    hipLaunchKernelGGL(k_max_depth, dim3(blocks), dim3(kThreads), 0, s, (const int32_t *)t.lcp,
+                       t.n, t.hdr);
 }
 
 void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
