@@ -185,6 +185,7 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.groups, n))) return rc;
     if ((rc = dev_alloc(c, &t.group_size, n))) return rc;
     if ((rc = dev_alloc(c, &t.group_slot, n))) return rc;
+    if ((rc = dev_alloc(c, &t.body4, n))) return rc;
     // a cell crossing a COM block boundary holds that boundary's two bodies:
     // at most one per depth (kMaxDepth) and block
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
@@ -214,15 +215,15 @@ int build_tree(nb_ctx *c) {
     nb::launch_aabb(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_AABB);
     mark_begin(c, NB_PHASE_KEYS_SORT);
-    nb::launch_keys(t, c->x, c->y, c->z, c->stream);
+    nb::launch_keys(t, c->x, c->y, c->z, c->m, c->stream);
     mark_end(c, NB_PHASE_KEYS_SORT);
     mark_begin(c, NB_PHASE_TOPOLOGY);
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
-    nb::launch_com(t, c->x, c->y, c->z, c->m, c->stream);
+    nb::launch_com(t, c->stream);
     const bool fp64 = c->p.precision == NB_PREC_FP64;
-    nb::launch_node_records(t, c->x, c->y, c->z, c->m, c->p.G, fp64, !fp64, c->stream);
+    nb::launch_node_records(t, c->p.G, fp64, !fp64, c->stream);
     c->quad_valid = false;
     if (c->corr && !fp64) {
         nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);
@@ -393,7 +394,8 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
                         t.int_lvl, t.int_count, t.int_sums, t.int_local,
-                        t.int_first, t.groups, t.group_size, t.group_slot, t.spanning, t.ref,
+                        t.int_first, t.groups, t.group_size, t.group_slot, t.body4, t.spanning,
+                        t.ref,
                         t.cm, t.edge, t.rec,
                         t.radix_hist, t.scan_part};
         (void)hipStreamSynchronize(c->stream);

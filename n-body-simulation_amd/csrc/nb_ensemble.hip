@@ -266,7 +266,8 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &f.sidx_alt, fo)) || (rc = ealloc(e, &f.chain, fo)) ||
         (rc = ealloc(e, &f.body_parent, fo)) || (rc = ealloc(e, &f.group_size, fo)) ||
         (rc = ealloc(e, &f.group_slot, fo)) || (rc = ealloc(e, &f.lcp, fo)) ||
-        (rc = ealloc(e, &f.groups, fo)) || (rc = ealloc(e, &f.int_depth, io)) ||
+        (rc = ealloc(e, &f.groups, fo)) || (rc = ealloc(e, &f.body4, fo)) ||
+        (rc = ealloc(e, &f.int_depth, io)) ||
         (rc = ealloc(e, &f.int_local, io)) || (rc = ealloc(e, &f.int_parent, io)) ||
         (rc = ealloc(e, &f.int_node, io)) || (rc = ealloc(e, &f.int_lvl, io)) ||
         (rc = ealloc(e, &f.int_count, io)) ||

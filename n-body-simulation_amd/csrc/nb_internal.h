@@ -89,9 +89,10 @@ struct NodeF32 {
 };
 
 // Bodies per block of the first centre-of-mass pass: cells whose body range
-// stays inside one block are summed there with workgroup-scope atomics; the
-// few cells crossing a block boundary are finished by one workgroup.
-constexpr uint32_t kComBlock = 4096;
+// stays inside one block are summed there, level by level in LDS; the few
+// cells crossing a block boundary are finished by one workgroup.  About 0.48
+// internal cells per body, so a block's cells fit the LDS stage.
+constexpr uint32_t kComBlock = 2048;
 
 // Device buffers of one octree.
 struct TreeBufs {
@@ -115,6 +116,7 @@ struct TreeBufs {
     uint32_t *group_size = nullptr; // per sorted position: size of the group starting there
     uint32_t *group_slot = nullptr; // per sorted position: 1 if a group starts there, then slots
     uint32_t *spanning = nullptr;   // [block * kMaxDepth + depth]: cell crossing the block's end
+    double4 *body4 = nullptr;       // {x, y, z, m} per body, input order: one load per body
     uint32_t spanning_cap = 0;
     int32_t *ref = nullptr;
     double4 *cm = nullptr;          // FP64 records (reference-semantics traversal)
@@ -150,6 +152,7 @@ struct ForestBufs {
     uint32_t *sidx, *sidx_alt, *chain, *body_parent, *group_size, *group_slot;
     int32_t *lcp;
     uint2 *groups;
+    double4 *body4;  // {x, y, z, m} in the system's input order
     // per internal cell (offset int_off)
     uint8_t *int_depth, *int_local;
     uint32_t *int_parent, *int_node, *int_lvl, *int_count, *int_first;
@@ -175,18 +178,17 @@ uint32_t radix_blocks_for(uint32_t n);
 // nb_octree.hip
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
                  hipStream_t s);
+// keys, sort, and t.body4 = {x, y, z, m} of every body (input order)
 void launch_keys(const TreeBufs &t, const double *x, const double *y, const double *z,
-                 hipStream_t s);
+                 const double *m, hipStream_t s);
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,
                      hipStream_t s);
-void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
-                const double *m, hipStream_t s);
+void launch_com(const TreeBufs &t, hipStream_t s);
 // hdr->max_depth of the last build (not maintained by the build itself)
 void launch_max_depth(const TreeBufs &t, hipStream_t s);
 // f64: FP64 records (cm, edge) for the reference-semantics traversal;
 // f32: packed FP32 records and traversal groups for the group traversal.
-void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
-                         const double *m, double G, bool f64, bool f32, hipStream_t s);
+void launch_node_records(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s);
 
 // Octree of every Barnes-Hut system of an ensemble, one workgroup per system
 // (order: the BH systems' indices, largest first); items[s] receives the

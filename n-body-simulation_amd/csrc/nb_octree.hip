@@ -192,16 +192,22 @@ __device__ inline uint64_t key_from(Cell &c, double px, double py, double pz) {
     return key;
 }
 
+// Keys, identity permutation, and the packed body record {x, y, z, m} that
+// the centre-of-mass and node-record passes gather (one 32-byte load).
 __global__ void __launch_bounds__(kThreads) k_keys(const double *__restrict__ x,
                                                    const double *__restrict__ y,
-                                                   const double *__restrict__ z, uint32_t n,
+                                                   const double *__restrict__ z,
+                                                   const double *__restrict__ m, uint32_t n,
                                                    const TreeHdr *__restrict__ h,
-                                                   uint64_t *skey, uint32_t *sidx) {
+                                                   uint64_t *skey, uint32_t *sidx,
+                                                   double4 *body4) {
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     Cell c{h->minx, h->miny, h->minz, h->edge};
-    skey[i] = key_from(c, x[i], y[i], z[i]);
+    const double px = x[i], py = y[i], pz = z[i];
+    skey[i] = key_from(c, px, py, pz);
     sidx[i] = i;
+    body4[i] = make_double4(px, py, pz, m[i]);
 }
 
 // ---- 4./5. LCP and deep runs --------------------------------------------------
@@ -453,38 +459,60 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
 
 // ---- 7. centre of mass --------------------------------------------------------
 // Sum of the 8 children of internal cell pk in digit order 0..7 from 0.0
-// (BarnesHutOctree.cpp:299-311).  Empty children add +0.0, which changes no
-// running sum that starts at +0.0, so they are skipped.
-__device__ inline void sum_children(uint32_t pk, const double *__restrict__ x,
-                                    const double *__restrict__ y, const double *__restrict__ z,
-                                    const double *__restrict__ m,
-                                    const int32_t *__restrict__ ref, InternalSums *sums,
-                                    uint32_t *counts, TreeHdr *h) {
+// (BarnesHutOctree.cpp:299-311): a body child adds (x m, y m, z m, m), a cell
+// child its sums.  Empty children add +0.0, which changes no running sum that
+// starts at +0.0, so they are skipped.  All eight children are loaded first,
+// so their loads are in flight together; the additions keep digit order.
+__device__ inline void add_children(const int32_t (&r)[8], const double4 (&v)[8],
+                                    const uint32_t (&c)[8], InternalSums &out, uint32_t &cnt) {
     double sx = 0, sy = 0, sz = 0, sm = 0;
-    uint32_t cnt = 0;
-    const uint64_t base = 1ull + 8ull * pk;
+    cnt = 0;
+#pragma unroll
     for (int d = 0; d < 8; ++d) {
-        const int32_t r = ref[base + d];
-        if (r >= 0) {
-            const InternalSums s = sums[r];
-            sx += s.x;
-            sy += s.y;
-            sz += s.z;
-            sm += s.m;
-            cnt += counts[r];
-        } else if (r != kRefEmpty) {
-            const uint32_t b = body_of_ref(r);
-            const double mb = m[b];
-            sx += x[b] * mb;
-            sy += y[b] * mb;
-            sz += z[b] * mb;
-            sm += mb;
-            cnt += 1;
+        if (r[d] != kRefEmpty) {
+            sx += v[d].x;
+            sy += v[d].y;
+            sz += v[d].z;
+            sm += v[d].w;
+            cnt += c[d];
         }
     }
-    sums[pk] = InternalSums{sx, sy, sz, sm};
+    out = InternalSums{sx, sy, sz, sm};
+}
+
+// the term of a body child: (x m, y m, z m, m)
+__device__ inline double4 body_term(const double4 p) {
+    return make_double4(p.x * p.w, p.y * p.w, p.z * p.w, p.w);
+}
+
+__device__ inline void sum_children(uint32_t pk, const double4 *__restrict__ body4,
+                                    const int32_t *__restrict__ ref, InternalSums *sums,
+                                    uint32_t *counts, TreeHdr *h) {
+    const uint64_t base = 1ull + 8ull * pk;
+    int32_t r[8];
+    double4 v[8];
+    uint32_t c[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) r[d] = ref[base + d];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        v[d] = make_double4(0.0, 0.0, 0.0, 0.0);
+        c[d] = 0;
+        if (r[d] >= 0) {
+            const InternalSums s = sums[r[d]];
+            v[d] = make_double4(s.x, s.y, s.z, s.m);
+            c[d] = counts[r[d]];
+        } else if (r[d] != kRefEmpty) {
+            v[d] = body_term(body4[body_of_ref(r[d])]);
+            c[d] = 1;
+        }
+    }
+    InternalSums s;
+    uint32_t cnt;
+    add_children(r, v, c, s, cnt);
+    sums[pk] = s;
     counts[pk] = cnt;
-    if (sm == 0) raise_flag(h, kFlagZeroMass);
+    if (s.m == 0) raise_flag(h, kFlagZeroMass);
 }
 
 // Centre of mass, level by level from the deepest cells up: a cell's sums
@@ -498,10 +526,10 @@ constexpr int kComThreads = 1024;
 template <typename Keep>
 __device__ inline void com_levels(uint32_t kb, uint32_t ke, Keep keep,
                                   const uint8_t *__restrict__ int_depth, uint32_t *lvl,
-                                  uint32_t *cnt, uint32_t *cur, const double *__restrict__ x,
-                                  const double *__restrict__ y, const double *__restrict__ z,
-                                  const double *__restrict__ m, const int32_t *__restrict__ ref,
-                                  InternalSums *sums, uint32_t *counts, TreeHdr *h) {
+                                  uint32_t *cnt, uint32_t *cur,
+                                  const double4 *__restrict__ body4,
+                                  const int32_t *__restrict__ ref, InternalSums *sums,
+                                  uint32_t *counts, TreeHdr *h) {
     const uint32_t t = threadIdx.x, nt = blockDim.x;
     for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += nt) cnt[d] = 0;
     __syncthreads();
@@ -527,40 +555,119 @@ __device__ inline void com_levels(uint32_t kb, uint32_t ke, Keep keep,
         const uint32_t lb = cnt[d], le = cnt[d + 1];
         if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
         for (uint32_t j = lb + t; j < le; j += nt)
-            sum_children(lvl[kb + j], x, y, z, m, ref, sums, counts, h);
+            sum_children(lvl[kb + j], body4, ref, sums, counts, h);
         __syncthreads();
     }
 }
 
 // Pass 1: one workgroup per kComBlock consecutive (sorted) bodies sums the
 // cells that lie inside its block.  Pair i owns cells [off[i], off[i+1]), so
-// the block's cells are one contiguous range; lvl (one word per cell) holds
-// the depth buckets.
+// the block's cells are one contiguous range [kb, ke).  Up to kComLdsCells of
+// them are summed in LDS: their child references are staged once
+// (coalesced), and a child cell's sums are read from LDS, so a level costs one
+// global round trip (the body children) instead of three.  A larger block
+// falls back to the same sums in global memory (lvl: depth buckets).
+constexpr uint32_t kComLdsCells = 1536;  // a 2048-body block holds about 1,000 cells
 __global__ void __launch_bounds__(kComThreads) k_com_local(
-    uint32_t n, uint32_t icap, TreeHdr *h, const double *__restrict__ x,
-    const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
+    uint32_t n, uint32_t icap, TreeHdr *h, const double4 *__restrict__ body4,
     const uint32_t *__restrict__ sidx, const int32_t *__restrict__ ref,
     const uint32_t *__restrict__ off, const uint8_t *__restrict__ int_depth,
     const uint8_t *__restrict__ int_local, uint32_t *lvl, InternalSums *sums, uint32_t *counts) {
     __shared__ uint32_t cnt[kMaxDepth + 1], cur[kMaxDepth];
+    __shared__ uint32_t lvl_s[kComLdsCells];
+    __shared__ int32_t ref_s[8 * kComLdsCells];
+    __shared__ double sx_s[kComLdsCells], sy_s[kComLdsCells], sz_s[kComLdsCells],
+        sm_s[kComLdsCells];
+    __shared__ uint32_t n_s[kComLdsCells];
+    const uint32_t t = threadIdx.x;
     const uint32_t s = blockIdx.x * kComBlock;
     const uint32_t e = min(n, s + kComBlock);
-    for (uint32_t i = s + threadIdx.x; i < e; i += kComThreads)
-        if (m[sidx[i]] == 0) raise_flag(h, kFlagZeroMass);
+    for (uint32_t i = s + t; i < e; i += kComThreads)
+        if (body4[sidx[i]].w == 0) raise_flag(h, kFlagZeroMass);
     if (n < 2) return;  // a lone body is the root leaf: no internal cell
     const uint32_t I = min(h->internal_count, icap);
     const uint32_t kb = s < n - 1 ? min(off[s], I) : I;
     const uint32_t ke = e < n - 1 ? min(off[e], I) : I;
-    com_levels(kb, ke, [&](uint32_t k) { return int_local[k] != 0; }, int_depth, lvl, cnt, cur,
-               x, y, z, m, ref, sums, counts, h);
+    const uint32_t nc = ke - kb;
+    if (nc > kComLdsCells) {  // the same for the whole workgroup
+        com_levels(kb, ke, [&](uint32_t k) { return int_local[k] != 0; }, int_depth, lvl, cnt,
+                   cur, body4, ref, sums, counts, h);
+        return;
+    }
+    for (uint32_t j = t; j < 8 * nc; j += kComThreads) ref_s[j] = ref[1ull + 8ull * kb + j];
+    for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += kComThreads) cnt[d] = 0;
+    __syncthreads();
+    for (uint32_t c = t; c < nc; c += kComThreads)
+        if (int_local[kb + c])
+            atomicAdd(&cnt[min((uint32_t)int_depth[kb + c], (uint32_t)kMaxDepth - 1)], 1u);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kMaxDepth; ++d) {
+            const uint32_t c = cnt[d];
+            cnt[d] = run;
+            cur[d] = run;
+            run += c;
+        }
+        cnt[kMaxDepth] = run;
+    }
+    __syncthreads();
+    for (uint32_t c = t; c < nc; c += kComThreads)
+        if (int_local[kb + c])
+            lvl_s[atomicAdd(&cur[min((uint32_t)int_depth[kb + c], (uint32_t)kMaxDepth - 1)], 1u)] = c;
+    __syncthreads();
+    for (int d = kMaxDepth - 1; d >= 0; --d) {
+        const uint32_t lb = cnt[d], le = cnt[d + 1];
+        if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
+        for (uint32_t j = lb + t; j < le; j += kComThreads) {
+            const uint32_t c = lvl_s[j];
+            // the children's data first, then the sums in digit order (as sum_children)
+            int32_t r[8];
+            double4 v[8];
+            uint32_t cc[8];
+#pragma unroll
+            for (int dd = 0; dd < 8; ++dd) r[dd] = ref_s[8 * c + dd];
+#pragma unroll
+            for (int dd = 0; dd < 8; ++dd) {
+                v[dd] = make_double4(0.0, 0.0, 0.0, 0.0);
+                cc[dd] = 0;
+                if (r[dd] >= 0) {
+                    const uint32_t rl = (uint32_t)r[dd] - kb;
+                    if (rl < nc) {
+                        v[dd] = make_double4(sx_s[rl], sy_s[rl], sz_s[rl], sm_s[rl]);
+                        cc[dd] = n_s[rl];
+                    } else {  // not in this block: cannot happen for a local cell
+                        const InternalSums q = sums[r[dd]];
+                        v[dd] = make_double4(q.x, q.y, q.z, q.m);
+                        cc[dd] = counts[r[dd]];
+                    }
+                } else if (r[dd] != kRefEmpty) {
+                    v[dd] = body_term(body4[body_of_ref(r[dd])]);
+                    cc[dd] = 1;
+                }
+            }
+            InternalSums q;
+            uint32_t cn;
+            add_children(r, v, cc, q, cn);
+            sx_s[c] = q.x;
+            sy_s[c] = q.y;
+            sz_s[c] = q.z;
+            sm_s[c] = q.m;
+            n_s[c] = cn;
+            sums[kb + c] = q;
+            counts[kb + c] = cn;
+            if (q.m == 0) raise_flag(h, kFlagZeroMass);
+        }
+        __syncthreads();
+    }
 }
 
 // Pass 2: one workgroup sums the cells that cross block boundaries, the
 // table spanning[block * kMaxDepth + depth], deepest depth first.
 __global__ void __launch_bounds__(kComThreads) k_com_spanning(
-    uint32_t nblocks, TreeHdr *h, const double *__restrict__ x, const double *__restrict__ y,
-    const double *__restrict__ z, const double *__restrict__ m, const int32_t *__restrict__ ref,
-    const uint32_t *__restrict__ spanning, InternalSums *sums, uint32_t *counts) {
+    uint32_t nblocks, TreeHdr *h, const double4 *__restrict__ body4,
+    const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning, InternalSums *sums,
+    uint32_t *counts) {
     __shared__ uint32_t live[kMaxDepth];
     const uint32_t t = threadIdx.x;
     if (t < (uint32_t)kMaxDepth) live[t] = 0;
@@ -572,7 +679,7 @@ __global__ void __launch_bounds__(kComThreads) k_com_spanning(
         if (!live[d]) continue;  // the same for every thread: LDS, read after a barrier
         for (uint32_t b = t; b < nblocks; b += kComThreads) {
             const uint32_t k = spanning[(uint64_t)b * kMaxDepth + d];
-            if (k != kNoCell) sum_children(k, x, y, z, m, ref, sums, counts, h);
+            if (k != kNoCell) sum_children(k, body4, ref, sums, counts, h);
         }
         __syncthreads();
     }
@@ -643,9 +750,8 @@ __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
 }
 
 // ---- 8. node records -----------------------------------------------------------
-__device__ inline void record_at(uint64_t j, double root_edge, const double *__restrict__ x,
-                                 const double *__restrict__ y, const double *__restrict__ z,
-                                 const double *__restrict__ m, const int32_t *__restrict__ ref,
+__device__ inline void record_at(uint64_t j, double root_edge, const double4 *__restrict__ body4,
+                                 const int32_t *__restrict__ ref,
                                  const InternalSums *__restrict__ sums,
                                  const uint8_t *__restrict__ int_depth, double G, int write_f64,
                                  int write_f32, double4 *cm, double *edge, NodeF32 *rec) {
@@ -659,9 +765,9 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double *__r
         c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
         e = ldexp(root_edge, -(int)int_depth[r]);
     } else if (r != kRefEmpty) {
-        const uint32_t b = body_of_ref(r);
-        const double mb = m[b];
-        c = make_double4((x[b] * mb) / mb, (y[b] * mb) / mb, (z[b] * mb) / mb, mb);
+        const double4 p = body4[body_of_ref(r)];
+        const double mb = p.w;
+        c = make_double4((p.x * mb) / mb, (p.y * mb) / mb, (p.z * mb) / mb, mb);
     }
     if (write_f64) {
         cm[j] = c;
@@ -683,10 +789,7 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double *__r
 
 __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict__ h,
                                                       uint32_t icap,
-                                                      const double *__restrict__ x,
-                                                      const double *__restrict__ y,
-                                                      const double *__restrict__ z,
-                                                      const double *__restrict__ m,
+                                                      const double4 *__restrict__ body4,
                                                       const int32_t *__restrict__ ref,
                                                       const InternalSums *__restrict__ sums,
                                                       const uint8_t *__restrict__ int_depth,
@@ -697,8 +800,8 @@ __global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict_
     const double root_edge = h->edge;
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
     for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride)
-        record_at(j, root_edge, x, y, z, m, ref, sums, int_depth, G, write_f64, write_f32, cm,
-                  edge, rec);
+        record_at(j, root_edge, body4, ref, sums, int_depth, G, write_f64, write_f32, cm, edge,
+                  rec);
 }
 
 // ---- ensemble: one workgroup builds one system's octree ----------------------------
@@ -884,6 +987,7 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
              *group_slot = fb.group_slot + fo;
     int32_t *lcp = fb.lcp + fo;
     uint2 *groups = fb.groups + fo;
+    double4 *body4 = fb.body4 + fo;
     uint8_t *int_depth = fb.int_depth + S.int_off, *int_local = fb.int_local + S.int_off;
     uint32_t *int_parent = fb.int_parent + S.int_off, *int_node = fb.int_node + S.int_off,
              *lvl = fb.int_lvl + S.int_off, *counts = fb.int_count + S.int_off,
@@ -951,8 +1055,10 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     {
         for (uint32_t i = t; i < n; i += kForestThreads) {
             Cell c = root;
-            skey[i] = key_from(c, x[i], y[i], z[i]);
+            const double px = x[i], py = y[i], pz = z[i];
+            skey[i] = key_from(c, px, py, pz);
             sidx[i] = i;
+            body4[i] = make_double4(px, py, pz, m[i]);
         }
         __syncthreads();
         wg_radix_sort(skey, sidx, skey_alt, sidx_alt, n, whist, wsum);
@@ -987,12 +1093,12 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     // 7. centre of mass, level by level: every cell is local to this workgroup
     for (uint32_t i = t; i < n; i += kForestThreads)
         if (m[i] == 0) raise_flag(h, kFlagZeroMass);
-    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c, x, y, z, m,
-               ref, sums, counts, h);
+    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c, body4, ref, sums,
+               counts, h);
     // 8. records and traversal groups
     const double root_edge = root.e;
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
-        record_at(j, root_edge, x, y, z, m, ref, sums, int_depth, G, 0, 1, nullptr, nullptr, rec);
+        record_at(j, root_edge, body4, ref, sums, int_depth, G, 0, 1, nullptr, nullptr, rec);
     for (uint32_t i = t; i < n; i += kForestThreads)
         group_mark_body(i, n, counts, body_parent, group_size, group_slot);
     __syncthreads();
@@ -1036,9 +1142,9 @@ void launch_aabb(const TreeBufs &t, const double *x, const double *y, const doub
 }
 
 void launch_keys(const TreeBufs &t, const double *x, const double *y, const double *z,
-                 hipStream_t s) {
-    hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, t.n,
-                       (const TreeHdr *)t.hdr, t.skey, t.sidx);
+                 const double *m, hipStream_t s) {
+    hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, m,
+                       t.n, (const TreeHdr *)t.hdr, t.skey, t.sidx, t.body4);
     radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels, t.radix_hist,
                        t.radix_blocks, t.scan_part, t.scan_part_cap, s);
 }
@@ -1072,23 +1178,21 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s) {
                        t.n, t.hdr);
 }
 
-void launch_com(const TreeBufs &t, const double *x, const double *y, const double *z,
-                const double *m, hipStream_t s) {
+void launch_com(const TreeBufs &t, hipStream_t s) {
     const uint32_t nblocks = ceil_div(t.n, kComBlock);
-    hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComThreads), 0, s, t.n, t.icap, t.hdr, x,
-                       y, z, m, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
+    hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComThreads), 0, s, t.n, t.icap, t.hdr,
+                       (const double4 *)t.body4, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
                        (const uint32_t *)t.chain, (const uint8_t *)t.int_depth,
                        (const uint8_t *)t.int_local, t.int_lvl, t.int_sums, t.int_count);
-    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, nblocks, t.hdr, x, y, z,
-                       m, (const int32_t *)t.ref, (const uint32_t *)t.spanning, t.int_sums,
-                       t.int_count);
+    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, nblocks, t.hdr,
+                       (const double4 *)t.body4, (const int32_t *)t.ref,
+                       (const uint32_t *)t.spanning, t.int_sums, t.int_count);
 }
 
-void launch_node_records(const TreeBufs &t, const double *x, const double *y, const double *z,
-                         const double *m, double G, bool f64, bool f32, hipStream_t s) {
+void launch_node_records(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) {
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(8192, ceil_div(8ull * t.n, kThreads)));
     hipLaunchKernelGGL(k_records, dim3(blocks), dim3(kThreads), 0, s, (const TreeHdr *)t.hdr,
-                       t.icap, x, y, z, m, (const int32_t *)t.ref,
+                       t.icap, (const double4 *)t.body4, (const int32_t *)t.ref,
                        (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, G,
                        (int)f64, (int)f32, t.cm, t.edge, t.rec);
     if (f32) {

@@ -127,32 +127,72 @@ __global__ void __launch_bounds__(kThreads) k_radix_hist(const uint64_t *__restr
     hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Stable scatter.  The tile is processed in kRadixItems rounds of 256 keys in
-// index order; inside a round a key's rank is (keys of the same digit in
-// earlier rounds) + (in earlier waves) + (in lower lanes of its own wave).
+// Stable scatter of one tile of kRadixTile keys.  The tile is loaded at once
+// (kRadixItems rounds of 256 keys in index order) and ranked round by round:
+// a key's place among its digit's keys in the tile is (same digit in earlier
+// rounds) + (in earlier waves) + (in lower lanes of its own wave).  The tile
+// is then reordered in LDS by digit and written out digit run by digit run,
+// so consecutive threads write consecutive addresses.
 __global__ void __launch_bounds__(kThreads) k_radix_scatter(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals, uint32_t n, int shift,
     const uint32_t *__restrict__ offs, uint32_t nb) {
-    __shared__ uint32_t running[256];
+    __shared__ uint32_t running[256];  // tile histogram, then the per-digit cursor
+    __shared__ uint32_t dbase[256];    // first tile position of digit d
+    __shared__ uint32_t gbase[256];    // output position of the tile's first digit-d key
+    __shared__ uint32_t wsum[kWaves];
     __shared__ uint32_t wcnt[2][kWaves][256];
+    __shared__ uint64_t lk[kRadixTile];
+    __shared__ uint32_t lv[kRadixTile];
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    running[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
+    const uint64_t tile = (uint64_t)blockIdx.x * kRadixTile;
+    const uint64_t rest = (uint64_t)n - tile;  // > 0: one block per started tile
+    const uint32_t tn = rest < (uint64_t)kRadixTile ? (uint32_t)rest : (uint32_t)kRadixTile;
+    uint64_t kk[kRadixItems];
+    uint32_t vv[kRadixItems], dg[kRadixItems];
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        const uint32_t j = (uint32_t)r * kThreads + tid;
+        kk[r] = j < tn ? keys[tile + j] : ~0ull;
+        vv[r] = j < tn ? vals[tile + j] : 0u;
+    }
+    running[tid] = 0;
+    gbase[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
         for (int k = 0; k < kWaves; ++k) wcnt[b][k][tid] = 0;
     __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRadixItems; ++r) {
+        dg[r] = (uint32_t)(kk[r] >> shift) & 255u;
+        if ((uint32_t)r * kThreads + tid < tn) atomicAdd(&running[dg[r]], 1u);
+    }
+    __syncthreads();
+    {
+        // exclusive scan of the tile histogram: thread tid owns digit tid
+        const uint32_t c = running[tid];
+        uint32_t inc = c;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(inc, off, 64);
+            if (lane >= (uint32_t)off) inc += o;
+        }
+        if (lane == 63) wsum[w] = inc;
+        __syncthreads();
+        uint32_t ex = inc - c;
+        for (uint32_t k = 0; k < w; ++k) ex += wsum[k];
+        dbase[tid] = ex;
+        running[tid] = ex;
+    }
+    __syncthreads();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
-    const uint64_t tile = (uint64_t)blockIdx.x * kRadixTile;
+#pragma unroll
     for (int r = 0; r < kRadixItems; ++r) {
         const int buf = r & 1;
-        const uint64_t i = tile + (uint64_t)r * kThreads + tid;
-        const bool valid = i < n;
-        const uint64_t key = valid ? keys[i] : ~0ull;
-        const uint32_t val = valid ? vals[i] : 0u;
-        const uint32_t d = valid ? ((uint32_t)(key >> shift) & 255u) : 255u;
-        uint64_t peers = ~0ull;
+        const bool valid = (uint32_t)r * kThreads + tid < tn;
+        const uint32_t d = valid ? dg[r] : 255u;
+        uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
             const bool bit = (d >> b) & 1u;
@@ -160,10 +200,10 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(
             peers &= bit ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (rank == 0) wcnt[buf][w][d] = (uint32_t)__popcll(peers);
+        if (valid && rank == 0) wcnt[buf][w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
         {
-            // thread tid owns digit tid: turn the per-wave counts into offsets
+            // thread tid owns digit tid: turn the per-wave counts into positions
             uint32_t run = running[tid];
 #pragma unroll
             for (int k = 0; k < kWaves; ++k) {
@@ -176,10 +216,18 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = wcnt[buf][w][d] + rank;
-            okeys[pos] = key;
-            ovals[pos] = val;
+            const uint32_t p = wcnt[buf][w][d] + rank;
+            lk[p] = kk[r];
+            lv[p] = vv[r];
         }
+    }
+    __syncthreads();
+    for (uint32_t j = tid; j < tn; j += kThreads) {
+        const uint64_t key = lk[j];
+        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t pos = gbase[d] + (j - dbase[d]);
+        okeys[pos] = key;
+        ovals[pos] = lv[j];
     }
 }
 
