@@ -24,74 +24,105 @@ namespace {
 __device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status, f); }
 
 // ---------------------------------------------------------------------------
-// FP64 direct sum, reference order.  One lane per body; sources staged in LDS
-// 64 at a time; the j loop is sequential per body as in the reference.
-constexpr int kDirectF64Tile = 64;
+// FP64 direct sum, reference order.  The sum over j = 0..n-1 of a target must
+// run in that order to be bit-exact, but only its three additions per pair
+// are sequential; the pair terms m_j (r_ij den_ij), with their IEEE division
+// and square root, are not.  A workgroup owns 64 targets (the lanes of wave
+// 0).  Sources are staged in LDS kF64Stage at a time; for each chunk of
+// kF64Chunk sources all eight waves compute the terms into one of two LDS
+// tiles while wave 0 adds the previous tile in source order.
+constexpr int kDirectF64Tile = 64;                  // targets per workgroup
+constexpr int kF64Threads = 512;                    // waves computing pair terms
+constexpr int kF64Chunk = 32;                       // sources per term tile
+constexpr int kF64PerWave = kF64Chunk / (kF64Threads / 64);
+constexpr int kF64Stage = 256;                      // sources per LDS stage
+static_assert(kF64Stage % kF64Chunk == 0, "whole chunks per stage");
 
+// acc (valid in wave 0) = sum over j in order of m_j (r_ij den_ij) for the
+// lane's target at (px, py, pz)
 __device__ inline void direct_f64_body(uint32_t n, const double *__restrict__ x,
                                        const double *__restrict__ y,
                                        const double *__restrict__ z,
-                                       const double *__restrict__ m, double eps2, bool valid,
-                                       double px, double py, double pz, double *acc) {
-    __shared__ double sx[kDirectF64Tile], sy[kDirectF64Tile], sz[kDirectF64Tile],
-        sm[kDirectF64Tile];
+                                       const double *__restrict__ m, double eps2, double px,
+                                       double py, double pz, double *acc) {
+    __shared__ double4 src[kF64Stage];
+    __shared__ double tx[2][kF64Chunk][kDirectF64Tile], ty[2][kF64Chunk][kDirectF64Tile],
+        tz[2][kF64Chunk][kDirectF64Tile];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     double acc_x = 0, acc_y = 0, acc_z = 0;
-    for (uint32_t k = 0; k < n; k += kDirectF64Tile) {
-        const uint32_t j = k + threadIdx.x;
-        if (j < n) {
-            sx[threadIdx.x] = x[j];
-            sy[threadIdx.x] = y[j];
-            sz[threadIdx.x] = z[j];
-            sm[threadIdx.x] = m[j];
+    uint32_t pending = 0;  // sources in the tile wave 0 still has to add
+    int buf = 0;
+    // wave 0: add tile `b` (cnt sources) in source order
+    auto add_tile = [&](int b, uint32_t cnt) {
+        for (uint32_t jj = 0; jj < cnt; ++jj) {
+            acc_x += tx[b][jj][lane];
+            acc_y += ty[b][jj][lane];
+            acc_z += tz[b][jj][lane];
         }
+    };
+    for (uint32_t k0 = 0; k0 < n; k0 += kF64Stage) {
+        const uint32_t staged = min((uint32_t)kF64Stage, n - k0);
+        __syncthreads();  // the previous stage's terms are computed
+        for (uint32_t j = threadIdx.x; j < staged; j += kF64Threads)
+            src[j] = make_double4(x[k0 + j], y[k0 + j], z[k0 + j], m[k0 + j]);
         __syncthreads();
-        const uint32_t cnt = min((uint32_t)kDirectF64Tile, n - k);
-        if (valid) {
-            for (uint32_t jj = 0; jj < cnt; ++jj) {
-                const double r_x = sx[jj] - px;
-                const double r_y = sy[jj] - py;
-                const double r_z = sz[jj] - pz;
-                double denominator = (r_x * r_x) + (r_y * r_y) + (r_z * r_z) + eps2;
-                denominator = denominator * denominator * denominator;
-                denominator = 1.0 / sqrt(denominator);
-                acc_x += sm[jj] * (r_x * denominator);
-                acc_y += sm[jj] * (r_y * denominator);
-                acc_z += sm[jj] * (r_z * denominator);
+        for (uint32_t c = 0; c < staged; c += kF64Chunk) {
+            const uint32_t cnt = min((uint32_t)kF64Chunk, staged - c);
+#pragma unroll
+            for (int u = 0; u < kF64PerWave; ++u) {
+                const uint32_t jj = w * kF64PerWave + u;  // the same for the whole wave
+                if (jj < cnt) {
+                    const double4 sj = src[c + jj];
+                    const double r_x = sj.x - px;
+                    const double r_y = sj.y - py;
+                    const double r_z = sj.z - pz;
+                    double denominator = (r_x * r_x) + (r_y * r_y) + (r_z * r_z) + eps2;
+                    denominator = denominator * denominator * denominator;
+                    denominator = 1.0 / sqrt(denominator);
+                    tx[buf][jj][lane] = sj.w * (r_x * denominator);
+                    ty[buf][jj][lane] = sj.w * (r_y * denominator);
+                    tz[buf][jj][lane] = sj.w * (r_z * denominator);
+                }
             }
+            // wave 0 adds the previous tile (written before the last barrier)
+            if (w == 0 && pending) add_tile(buf ^ 1, pending);
+            __syncthreads();
+            pending = cnt;
+            buf ^= 1;
         }
-        __syncthreads();
     }
+    if (w == 0 && pending) add_tile(buf ^ 1, pending);
     acc[0] = acc_x;
     acc[1] = acc_y;
     acc[2] = acc_z;
 }
 
-__global__ void __launch_bounds__(kDirectF64Tile) k_direct_f64(
+__global__ void __launch_bounds__(kF64Threads) k_direct_f64(
     uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
     double *ax, double *ay, double *az) {
-    const uint32_t i = blockIdx.x * kDirectF64Tile + threadIdx.x;
+    const uint32_t i = blockIdx.x * kDirectF64Tile + (threadIdx.x & 63u);
     const bool valid = i < n;
     const uint32_t ii = valid ? i : 0;
     double acc[3];
-    direct_f64_body(n, x, y, z, m, eps2, valid, x[ii], y[ii], z[ii], acc);
-    if (valid) {
+    direct_f64_body(n, x, y, z, m, eps2, x[ii], y[ii], z[ii], acc);
+    if (valid && threadIdx.x < 64u) {
         ax[i] = acc[0] * G;
         ay[i] = acc[1] * G;
         az[i] = acc[2] * G;
     }
 }
 
-__global__ void __launch_bounds__(kDirectF64Tile) k_direct_targets_f64(
+__global__ void __launch_bounds__(kF64Threads) k_direct_targets_f64(
     uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
     uint32_t nt, const uint32_t *__restrict__ targets, double *ax, double *ay, double *az) {
-    const uint32_t t = blockIdx.x * kDirectF64Tile + threadIdx.x;
+    const uint32_t t = blockIdx.x * kDirectF64Tile + (threadIdx.x & 63u);
     const bool valid = t < nt;
     const uint32_t i = valid ? targets[t] : 0;
     double acc[3];
-    direct_f64_body(n, x, y, z, m, eps2, valid, x[i], y[i], z[i], acc);
-    if (valid) {
+    direct_f64_body(n, x, y, z, m, eps2, x[i], y[i], z[i], acc);
+    if (valid && threadIdx.x < 64u) {
         ax[t] = acc[0] * G;
         ay[t] = acc[1] * G;
         az[t] = acc[2] * G;
@@ -284,7 +315,8 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 // Positions are relative to the group centre; the records' hi/lo split keeps
 // the FP32 differences exact to the last FP32 bit.  Waves take groups from an
 // atomic work queue.
-constexpr int kGroup = 64;          // lanes per wavefront = bodies per group at most
+constexpr int kGroup = 64;          // lanes per wavefront (a group has up to 2 x kGroup bodies)
+static_assert(kGroupMax <= 2 * kGroup, "a lane holds at most two bodies of its group");
 constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 6 | depth)
 constexpr int kListCap = 2 * kGroup;
 constexpr int kGroupBlocks = 8192;  // persistent waves
@@ -336,22 +368,28 @@ struct GroupTree {
 };
 
 // Pair forces of list entries p[0], p[LPB], p[2 LPB], ... (t entries, in that
-// order) on the point (rx, ry, rz).  LPB and the uniform trip count make the
-// LDS offsets immediates and let four reads be in flight ahead of the maths.
-template <int LPB>
-__device__ __forceinline__ void flush_share(const float4 *p, uint32_t t, float rx, float ry,
-                                            float rz, float eps2, float &fx, float &fy,
-                                            float &fz) {
+// order) on the NB points (rx, ry, rz)[b].  LPB and the uniform trip count make
+// the LDS offsets immediates and let four reads be in flight ahead of the
+// maths; with NB = 2 every entry read feeds two pairs.
+template <int LPB, int NB>
+__device__ __forceinline__ void flush_share(const float4 *p, uint32_t t, const float (&rx)[NB],
+                                            const float (&ry)[NB], const float (&rz)[NB],
+                                            float eps2, float (&fx)[NB], float (&fy)[NB],
+                                            float (&fz)[NB]) {
     uint32_t k = 0;
     for (; k + 4 <= t; k += 4) {
-        const float4 a = p[(k + 0) * LPB], b = p[(k + 1) * LPB], c = p[(k + 2) * LPB],
-                     d = p[(k + 3) * LPB];
-        pair_f32(a, rx, ry, rz, eps2, fx, fy, fz);
-        pair_f32(b, rx, ry, rz, eps2, fx, fy, fz);
-        pair_f32(c, rx, ry, rz, eps2, fx, fy, fz);
-        pair_f32(d, rx, ry, rz, eps2, fx, fy, fz);
+        const float4 e[4] = {p[(k + 0) * LPB], p[(k + 1) * LPB], p[(k + 2) * LPB],
+                             p[(k + 3) * LPB]};
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) pair_f32(e[u], rx[b], ry[b], rz[b], eps2, fx[b], fy[b], fz[b]);
     }
-    for (; k < t; ++k) pair_f32(p[k * LPB], rx, ry, rz, eps2, fx, fy, fz);
+    for (; k < t; ++k) {
+        const float4 e = p[k * LPB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) pair_f32(e, rx[b], ry[b], rz[b], eps2, fx[b], fy[b], fz[b]);
+    }
 }
 
 // A list entry that adds exactly zero for any eps^2: its distance squared
@@ -368,13 +406,17 @@ struct GroupLds {
     float4 *bcorr;   // correction: learned correction per body
 };
 
-// Forces on the bodies of one group grp = {first sorted position, count}
-// (checked by the caller).  *ncorr_out accumulates network evaluations.
-template <bool kCorr>
-__device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float eps2,
-                                           const CorrArgs &corr, const MlpForward &w,
-                                           const GroupLds &L, uint32_t &peak, bool &overflow,
-                                           uint32_t &ncorr_out) {
+// Forces on the s bodies of one group starting at sorted position g0 (both
+// wave-uniform, checked by the caller).  NB = 1 (s <= 64): s2 = s rounded up
+// to a power of two, and the 64 / s2 lanes of body `sub` share its list.
+// NB = 2 (64 < s <= 128): a lane holds bodies lane and lane + 64 and takes
+// the whole list for both.  *ncorr_out accumulates network evaluations.
+template <bool kCorr, int NB>
+__device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, uint32_t s,
+                                             float eps2, const CorrArgs &corr,
+                                             const MlpForward &w, const GroupLds &L,
+                                             uint32_t &peak, bool &overflow,
+                                             uint32_t &ncorr_out) {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = lanes_below();
     const uint32_t *__restrict__ sidx = T.sidx;
@@ -389,18 +431,36 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
     float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
     uint32_t *cl_k = L.cl_k;
     {
-        const uint32_t g0 = wave_uniform(grp.x);
-        const uint32_t s = wave_uniform(grp.y);
-        const uint32_t s2 = s <= 1 ? 1u : 1u << (32 - __clz(s - 1));
+        const uint32_t s2 = NB == 2 ? (uint32_t)kGroup
+                                    : (s <= 1 ? 1u : 1u << (32 - __clz(s - 1)));
         const uint32_t sub = lane & (s2 - 1);  // which body of the group
         const uint32_t q = lane / s2;          // which share of the list
         const uint32_t lpb = kGroup / s2;      // lanes per body
-        const bool valid = sub < s;
-        const uint32_t bi = sidx[g0 + (valid ? sub : 0)];
-        const double px = x[bi], py = y[bi], pz = z[bi];
+        bool valid[NB];
+        uint32_t bi[NB];
+        double px[NB], py[NB], pz[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t k = sub + (uint32_t)b * kGroup;
+            valid[b] = k < s;
+            // an unused slot repeats a body of the group (its box stays the group's)
+            bi[b] = sidx[g0 + (valid[b] ? k : (b == 0 ? 0u : sub))];
+            px[b] = x[bi[b]];
+            py[b] = y[bi[b]];
+            pz[b] = z[bi[b]];
+        }
 
         // bounding box (FP64, exact min/max), centre and half extent
-        double mnx = px, mny = py, mnz = pz, mxx = px, mxy = py, mxz = pz;
+        double mnx = px[0], mny = py[0], mnz = pz[0], mxx = px[0], mxy = py[0], mxz = pz[0];
+#pragma unroll
+        for (int b = 1; b < NB; ++b) {
+            mnx = fmin(mnx, px[b]);
+            mny = fmin(mny, py[b]);
+            mnz = fmin(mnz, pz[b]);
+            mxx = fmax(mxx, px[b]);
+            mxy = fmax(mxy, py[b]);
+            mxz = fmax(mxz, pz[b]);
+        }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
             mnx = fmin(mnx, __shfl_xor(mnx, off, 64));
@@ -422,10 +482,11 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
         // produce its leaf's list entry (record hi/lo of (x*m)/m, k_records),
         // so the self pair has a distance of exactly 0 and adds nothing; with
         // eps^2 = 1e-22 any rounding residual would be a huge spurious force.
-        float rx, ry, rz;
-        {
-            const double mb = m[bi];
-            const double c3[3] = {(px * mb) / mb, (py * mb) / mb, (pz * mb) / mb};
+        float rx[NB], ry[NB], rz[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const double mb = m[bi[b]];
+            const double c3[3] = {(px[b] * mb) / mb, (py[b] * mb) / mb, (pz[b] * mb) / mb};
             const float gh[3] = {gxh, gyh, gzh}, gl[3] = {gxl, gyl, gzl};
             float r3[3];
 #pragma unroll
@@ -434,19 +495,25 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
                 const float lo = (float)(c3[c] - (double)hi);
                 r3[c] = (hi - gh[c]) + (lo - gl[c]);
             }
-            rx = r3[0];
-            ry = r3[1];
-            rz = r3[2];
+            rx[b] = r3[0];
+            ry[b] = r3[1];
+            rz[b] = r3[2];
         }
 
-        double acc_x = 0, acc_y = 0, acc_z = 0;
+        double acc_x[NB], acc_y[NB], acc_z[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc_x[b] = acc_y[b] = acc_z[b] = 0.0;
         uint32_t n_inter = 0;
         uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
         uint32_t cn = 0;          // wave-uniform size of the correction's cell list
         uint32_t ncorr = 0;       // network evaluations of this group
         if constexpr (kCorr) {
-            if (valid && q == 0) boff[sub] = make_float4(rx, ry, rz, 0.f);
-            bcorr[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const uint32_t k = sub + (uint32_t)b * kGroup;
+                if (valid[b] && q == 0) boff[k] = make_float4(rx[b], ry[b], rz[b], 0.f);
+                bcorr[lane + (uint32_t)b * kGroup] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
 
         // Entries lst[0 .. cnt) (cnt <= kGroup): lane share q takes q, q+lpb,
@@ -460,19 +527,28 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
             }
             const uint32_t trips = cp / lpb;
             const float4 *p = lst + q;
-            float fx = 0.f, fy = 0.f, fz = 0.f;
-            switch (lpb) {
-            case 1: flush_share<1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            case 2: flush_share<2>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            case 4: flush_share<4>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            case 8: flush_share<8>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            case 16: flush_share<16>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            case 32: flush_share<32>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-            default: flush_share<64>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+            float fx[NB], fy[NB], fz[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) fx[b] = fy[b] = fz[b] = 0.f;
+            if constexpr (NB == 2) {
+                flush_share<1, 2>(p, trips, rx, ry, rz, eps2, fx, fy, fz);
+            } else {
+                switch (lpb) {
+                case 1: flush_share<1, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 2: flush_share<2, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 4: flush_share<4, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 8: flush_share<8, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 16: flush_share<16, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 32: flush_share<32, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                default: flush_share<64, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                }
             }
-            acc_x += (double)fx;
-            acc_y += (double)fy;
-            acc_z += (double)fz;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                acc_x[b] += (double)fx[b];
+                acc_y[b] += (double)fy[b];
+                acc_z[b] += (double)fz[b];
+            }
             n_inter += cnt;
         };
 
@@ -640,33 +716,54 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
         __syncthreads();
         // add up the list shares of each body (fixed order: deterministic)
         for (uint32_t off = s2; off < (uint32_t)kGroup; off <<= 1) {
-            acc_x += __shfl_xor(acc_x, off, 64);
-            acc_y += __shfl_xor(acc_y, off, 64);
-            acc_z += __shfl_xor(acc_z, off, 64);
+            acc_x[0] += __shfl_xor(acc_x[0], off, 64);
+            acc_y[0] += __shfl_xor(acc_y[0], off, 64);
+            acc_z[0] += __shfl_xor(acc_z[0], off, 64);
         }
         ncorr_out += ncorr;
         // Stored also after an overflow (the caller reports it and the values
         // are then undefined): a predicate holding `overflow` would let the
         // compiler route the storing lanes past the caller's exit test, a
         // per-lane split of the wave's loop.
-        if (valid && q == 0) {
-            if constexpr (kCorr) {
-                const float4 c = bcorr[sub];
-                acc_x += (double)c.x;
-                acc_y += (double)c.y;
-                acc_z += (double)c.z;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            if (valid[b] && q == 0) {
+                if constexpr (kCorr) {
+                    const float4 c = bcorr[sub + (uint32_t)b * kGroup];
+                    acc_x[b] += (double)c.x;
+                    acc_y[b] += (double)c.y;
+                    acc_z[b] += (double)c.z;
+                }
+                ax[bi[b]] = acc_x[b];
+                ay[bi[b]] = acc_y[b];
+                az[bi[b]] = acc_z[b];
+                if (inter) inter[bi[b]] = n_inter;
             }
-            ax[bi] = acc_x;
-            ay[bi] = acc_y;
-            az[bi] = acc_z;
-            if (inter) inter[bi] = n_inter;
         }
     }
 }
 
+// Forces on the bodies of one group grp = {first sorted position, count}
+// (checked by the caller): one body per lane up to 64 bodies, two beyond.
+template <bool kCorr>
+__device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float eps2,
+                                           const CorrArgs &corr, const MlpForward &w,
+                                           const GroupLds &L, uint32_t &peak, bool &overflow,
+                                           uint32_t &ncorr_out) {
+    const uint32_t g0 = wave_uniform(grp.x);
+    const uint32_t s = wave_uniform(grp.y);
+    if constexpr (kGroupMax > (uint32_t)kGroup) {
+        if (s > (uint32_t)kGroup) {
+            group_walk_n<kCorr, 2>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out);
+            return;
+        }
+    }
+    group_walk_n<kCorr, 1>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out);
+}
+
 // a group {first, count} that a walk may trust
 __device__ inline bool group_ok(uint2 grp, uint32_t n) {
-    return grp.y != 0 && grp.y <= (uint32_t)kGroup && grp.x < n && grp.y <= n - grp.x;
+    return grp.y != 0 && grp.y <= kGroupMax && grp.x < n && grp.y <= n - grp.x;
 }
 
 template <bool kCorr>
@@ -680,8 +777,8 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     __shared__ float4 lst[kListCap];
     __shared__ float4 cl_e[kCorr ? kCellCap : 1];
     __shared__ uint32_t cl_k[kCorr ? kCellCap : 1];
-    __shared__ float4 boff[kCorr ? kGroup : 1];
-    __shared__ float4 bcorr[kCorr ? kGroup : 1];
+    __shared__ float4 boff[kCorr ? kGroupMax : 1];
+    __shared__ float4 bcorr[kCorr ? kGroupMax : 1];
     const GroupLds L{stk, lst, cl_e, cl_k, boff, bcorr};
     const uint32_t lane = threadIdx.x;
     const uint32_t ngroups = min(h->group_count, n);
@@ -833,8 +930,8 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
                        const double *m, double G, double eps2, double *ax, double *ay,
                        double *az, hipStream_t s) {
-    hipLaunchKernelGGL(k_direct_f64, dim3(ceil_div(n, kDirectF64Tile)), dim3(kDirectF64Tile), 0,
-                       s, n, x, y, z, m, G, eps2, ax, ay, az);
+    hipLaunchKernelGGL(k_direct_f64, dim3(ceil_div(n, kDirectF64Tile)), dim3(kF64Threads), 0, s,
+                       n, x, y, z, m, G, eps2, ax, ay, az);
 }
 
 void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, const double *z,
@@ -843,8 +940,7 @@ void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, con
                                hipStream_t s) {
     if (nt == 0) return;
     hipLaunchKernelGGL(k_direct_targets_f64, dim3(ceil_div(nt, kDirectF64Tile)),
-                       dim3(kDirectF64Tile), 0, s, n, x, y, z, m, G, eps2, nt, targets, ax, ay,
-                       az);
+                       dim3(kF64Threads), 0, s, n, x, y, z, m, G, eps2, nt, targets, ax, ay, az);
 }
 
 uint32_t direct_f32c_splits(uint32_t n) {

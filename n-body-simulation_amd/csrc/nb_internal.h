@@ -53,6 +53,10 @@ struct alignas(128) TreeHdr {
 
 // Bodies per traversal group: a group is an octree cell holding at most
 // kGroupMax bodies whose parent holds more (or a lone body under such a parent).
+// A 64-lane wave walks one group.  The walk also supports up to 128 bodies
+// (two per lane); measured on a 1M-body disk, 128 made the traversal 7 %
+// slower (18 % more interactions from the larger boxes), 32 made it 18 %
+// slower (twice the walks), so 64.
 constexpr uint32_t kGroupMax = 64;
 
 struct InternalSums {
