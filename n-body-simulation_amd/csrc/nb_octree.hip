@@ -814,6 +814,7 @@ constexpr int kSortBits = 8;  // radix digit width: 256 bins x 16 waves of LDS c
 constexpr int kSortBins = 1 << kSortBits;
 constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;  // 8: the result ends in place
 static_assert(kSortPasses % 2 == 0, "the sorted pairs must end in the input buffers");
+constexpr int kSortUnroll = 4;  // 64-key groups per iteration of a wave
 
 // lanes below this one, as a ballot mask
 __device__ inline uint64_t forest_lanes_below() {
@@ -900,12 +901,21 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
     for (int p = 0; p < kSortPasses; ++p) {
         const int sh = kSortBits * p;
         for (uint32_t d = lane; d < (uint32_t)kSortBins; d += 64) row[d] = 0;
-        for (uint32_t g = b; g < e; g += 64) {
-            const uint32_t i = g + lane;
-            const bool valid = i < e;
-            const uint32_t dig = valid ? (uint32_t)(sk[i] >> sh) & (kSortBins - 1) : 0u;
-            const uint64_t peers = digit_peers(dig, valid);
-            if (valid && (peers & below) == 0) row[dig] += (uint32_t)__popcll(peers);
+        // kSortUnroll groups of 64 per iteration: their loads are in flight together
+        for (uint32_t g = b; g < e; g += 64 * kSortUnroll) {
+            uint32_t dig[kSortUnroll];
+            bool val[kSortUnroll];
+#pragma unroll
+            for (int u = 0; u < kSortUnroll; ++u) {
+                const uint32_t i = g + 64u * u + lane;
+                val[u] = i < e;
+                dig[u] = val[u] ? (uint32_t)(sk[i] >> sh) & (kSortBins - 1) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kSortUnroll; ++u) {
+                const uint64_t peers = digit_peers(dig[u], val[u]);
+                if (val[u] && (peers & below) == 0) row[dig[u]] += (uint32_t)__popcll(peers);
+            }
         }
         __syncthreads();
         // offsets: digit-major exclusive scan over (digit, wave); threads
@@ -933,24 +943,30 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
             for (int ww = 0; ww < kForestWaves; ++ww) whist[ww * kSortBins + t] += dig_excl;
         }
         __syncthreads();
-        for (uint32_t g = b; g < e; g += 64) {
-            const uint32_t i = g + lane;
-            const bool valid = i < e;
-            uint64_t key = 0;
-            uint32_t val = 0;
-            if (valid) {
-                key = sk[i];
-                val = sv[i];
+        for (uint32_t g = b; g < e; g += 64 * kSortUnroll) {
+            uint64_t key[kSortUnroll];
+            uint32_t val[kSortUnroll];
+            bool ok[kSortUnroll];
+#pragma unroll
+            for (int u = 0; u < kSortUnroll; ++u) {
+                const uint32_t i = g + 64u * u + lane;
+                ok[u] = i < e;
+                key[u] = ok[u] ? sk[i] : 0ull;
+                val[u] = ok[u] ? sv[i] : 0u;
             }
-            const uint32_t dig = (uint32_t)(key >> sh) & (kSortBins - 1);
-            const uint64_t peers = digit_peers(dig, valid);
-            // every lane reads its digit's offset before the leader advances it
-            const uint32_t base = row[dig];
-            if (valid) {
-                const uint32_t pos = base + (uint32_t)__popcll(peers & below);
-                dk[pos] = key;
-                dv[pos] = val;
-                if ((peers & below) == 0) row[dig] = base + (uint32_t)__popcll(peers);
+            // the groups in order: group u's ranks follow group u-1's
+#pragma unroll
+            for (int u = 0; u < kSortUnroll; ++u) {
+                const uint32_t dig = (uint32_t)(key[u] >> sh) & (kSortBins - 1);
+                const uint64_t peers = digit_peers(dig, ok[u]);
+                // every lane reads its digit's offset before the leader advances it
+                const uint32_t base = row[dig];
+                if (ok[u]) {
+                    const uint32_t pos = base + (uint32_t)__popcll(peers & below);
+                    dk[pos] = key[u];
+                    dv[pos] = val[u];
+                    if ((peers & below) == 0) row[dig] = base + (uint32_t)__popcll(peers);
+                }
             }
         }
         __syncthreads();
