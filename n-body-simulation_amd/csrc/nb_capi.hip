@@ -186,6 +186,7 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.group_size, n))) return rc;
     if ((rc = dev_alloc(c, &t.group_slot, n))) return rc;
     if ((rc = dev_alloc(c, &t.body4, n))) return rc;
+    if ((rc = dev_alloc(c, &t.gq, TreeBufs::kGroupQueueWords))) return rc;
     // a cell crossing a COM block boundary holds that boundary's two bodies:
     // at most one per depth (kMaxDepth) and block
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
@@ -394,7 +395,8 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
                         t.int_lvl, t.int_count, t.int_sums, t.int_local,
-                        t.int_first, t.groups, t.group_size, t.group_slot, t.body4, t.spanning,
+                        t.int_first, t.groups, t.group_size, t.group_slot, t.body4, t.gq,
+                        t.spanning,
                         t.ref,
                         t.cm, t.edge, t.rec,
                         t.radix_hist, t.scan_part};

@@ -318,21 +318,38 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 constexpr int kGroup = 64;          // lanes per wavefront (a group has up to 2 x kGroup bodies)
 static_assert(kGroupMax <= 2 * kGroup, "a lane holds at most two bodies of its group");
 constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 6 | depth)
-constexpr int kListCap = 2 * kGroup;
+// Child records per lane and walk step (8 kPop cells).  More loads in flight
+// per wave cost registers: on a 1M-body disk kPop 2 and 4 made the traversal
+// 7 % and 17 % slower than 1 (occupancy 6 -> 4 waves per SIMD).
+constexpr int kPop = 1;
+constexpr int kListCap = kGroup * (kPop + 1);
 constexpr int kGroupBlocks = 8192;  // persistent waves
+constexpr uint32_t kXcds = 8;       // XCDs of an MI355X; workgroups are dealt round-robin
+constexpr uint32_t kQueueStride = 32;  // one 128-byte line per group-queue counter
+static_assert(kXcds * kQueueStride <= TreeBufs::kGroupQueueWords, "queue counters fit TreeBufs::gq");
 
 // Every traversal starts with fresh queue cursors (a kernel, not a memset,
 // so the sequence can be captured in a graph).
-__global__ void k_group_queue_reset(TreeHdr *h) {
+__global__ void k_group_queue_reset(TreeHdr *h, uint32_t *gq) {
     if (threadIdx.x == 0) {
         h->group_next = 0;
         h->corr_pairs = 0;
     }
+    if (threadIdx.x < kXcds) gq[threadIdx.x * kQueueStride] = 0;
 }
 
-__device__ inline uint64_t lanes_below() {
-    const uint32_t lane = threadIdx.x & 63u;
-    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+// The device-scope maximum with a read first: the read-modify-write (which
+// serialises at one memory channel when every wave does it) only when it
+// raises the value.
+__device__ inline void raise_max(uint32_t *p, uint32_t v) {
+    if (v > __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMax(p, v);
+}
+
+// Set bits of a wave-uniform mask in the lanes below this one (v_mbcnt: two
+// VALU instructions, where a per-lane mask and popcount take four).
+__device__ inline uint32_t count_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
 // A value every lane of the wave holds alike, made visibly uniform (SGPR).
@@ -351,7 +368,7 @@ __device__ inline uint32_t lane0_uniform(uint32_t v) { return wave_uniform(__shf
 // group (one 16-sample matrix-core tile per body) and adds the corrections of
 // a body in a fixed order.
 constexpr int kCorrTile = 16;
-constexpr int kCellCap = kCorrTile + kGroup;
+constexpr int kCellCap = kCorrTile + kGroup * kPop;
 
 // One octree as the group walk sees it (body indices of sidx and of the node
 // references index x, y, z, m, ax, ay, az, inter).
@@ -418,7 +435,6 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                                              uint32_t &peak, bool &overflow,
                                              uint32_t &ncorr_out) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t below = lanes_below();
     const uint32_t *__restrict__ sidx = T.sidx;
     const double *__restrict__ x = T.x, *__restrict__ y = T.y, *__restrict__ z = T.z,
                  *__restrict__ m = T.m;
@@ -516,17 +532,17 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             }
         }
 
-        // Entries lst[0 .. cnt) (cnt <= kGroup): lane share q takes q, q+lpb,
-        // ...  The list is padded with zero-force entries to a multiple of
-        // lpb, so every lane runs the same (uniform) trip count.
-        auto flush = [&](uint32_t cnt) {
+        // Entries lst[base .. base + cnt) (cnt <= kGroup): lane share q takes
+        // q, q+lpb, ...  The entries are padded with zero-force entries to a
+        // multiple of lpb, so every lane runs the same (uniform) trip count.
+        auto flush = [&](uint32_t base, uint32_t cnt) {
             const uint32_t cp = (cnt + lpb - 1) / lpb * lpb;
             if (cp > cnt) {
-                if (lane < cp - cnt) lst[cnt + lane] = far_entry();
+                if (lane < cp - cnt) lst[base + cnt + lane] = far_entry();
                 __syncthreads();
             }
             const uint32_t trips = cp / lpb;
-            const float4 *p = lst + q;
+            const float4 *p = lst + base + q;
             float fx[NB], fy[NB], fz[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) fx[b] = fy[b] = fz[b] = 0.f;
@@ -588,20 +604,24 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             }
         };
 
-        // 0 skip, 1 list, 2 open; d2 = squared distance to the group box (cells)
-        auto classify = [&](const NodeF32 &r, int depth, float4 &entry, float &d2) -> int {
-            d2 = 0.f;
-            if (r.ref == kRefEmpty) return 0;
+        // A node goes to the list (accepted cell, or body leaf: exact pair, the
+        // self pair adds 0) or is opened (cell failing the MAC); an empty slot
+        // does neither.  d2 = squared distance to the group box.  Computed for
+        // every lane without branches: an empty slot's values are unused.
+        auto classify = [&](const NodeF32 &r, int depth, float4 &entry, float &d2,
+                            bool &to_list, bool &to_open) {
             const float dx = (r.x - gxh) + (r.lx - gxl);
             const float dy = (r.y - gyh) + (r.ly - gyl);
             const float dz = (r.z - gzh) + (r.lz - gzl);
             entry = make_float4(dx, dy, dz, r.gm);
-            if (r.ref < 0) return 1;  // body leaf: exact pair (the self pair adds 0)
             const float ex = fmaxf(fabsf(dx) - hx, 0.f);
             const float ey = fmaxf(fabsf(dy) - hy, 0.f);
             const float ez = fmaxf(fabsf(dz) - hz, 0.f);
             d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-            return d2 > ldexpf(ro2_root, -2 * depth) ? 1 : 2;
+            const bool far_enough = d2 > ldexpf(ro2_root, -2 * depth);
+            const bool cell = r.ref >= 0;
+            to_list = r.ref != kRefEmpty && (!cell || far_enough);
+            to_open = cell && !far_enough;
         };
 
         {
@@ -609,7 +629,9 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             float d2r;
             const NodeF32 root = rec[0];
             // every lane evaluates the root alike
-            const int cls = (int)wave_uniform((uint32_t)classify(root, 0, e, d2r));
+            bool rl, ro;
+            classify(root, 0, e, d2r, rl, ro);
+            const int cls = (int)wave_uniform(rl ? 1u : (ro ? 2u : 0u));
             if (cls == 1) {
                 if (lane == 0) lst[0] = e;
                 ln = 1;
@@ -629,17 +651,24 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             __syncthreads();
         }
 
-        // pop up to 8 cells and load the record of this lane's child
-        NodeF32 r;
-        int depth = 0;
+        // Pop up to 8 kPop cells and load this lane's kPop child records
+        // (child lane & 7 of the cells lane >> 3, + 8, + 16, ...): the kPop
+        // loads of a lane are in flight together, so a walk waits on memory
+        // once per kPop batches of cells instead of once per batch.
+        NodeF32 r[kPop];
+        int depth[kPop];
         auto pop_and_load = [&]() -> uint32_t {
-            const uint32_t take = sp < 8u ? sp : 8u;
+            const uint32_t take = sp < 8u * kPop ? sp : 8u * kPop;
             sp -= take;
-            const bool act = (lane >> 3) < take;
-            const uint32_t entry = act ? stk[sp + (lane >> 3)] : 0u;
-            depth = (int)(entry & 63u) + 1;
-            if (act) r = rec[1ull + 8ull * (entry >> 6) + (lane & 7u)];
-            else r.ref = kRefEmpty;
+#pragma unroll
+            for (int k = 0; k < kPop; ++k) {
+                const uint32_t c = (lane >> 3) + 8u * (uint32_t)k;
+                const bool act = c < take;
+                const uint32_t entry = act ? stk[sp + c] : 0u;
+                depth[k] = (int)(entry & 63u) + 1;
+                if (act) r[k] = rec[1ull + 8ull * (entry >> 6) + (lane & 7u)];
+                else r[k].ref = kRefEmpty;
+            }
             __syncthreads();
             return take;
         };
@@ -654,29 +683,33 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 overflow = true;
                 break;
             }
-            float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-            float d2 = 0.f;
-            const int cls = classify(r, depth, e, d2);
-            const uint64_t bl = __ballot(cls == 1);
-            const uint64_t bo = __ballot(cls == 2);
-            if (cls == 1) lst[ln + __popcll(bl & below)] = e;
-            if (cls == 2) {
-                const uint32_t p = sp + __popcll(bo & below);
-                if (p < kStackCap) stk[p] = ((uint32_t)r.ref << 6) | (uint32_t)depth;
-            }
-            if constexpr (kCorr) {
-                const bool cc = cls == 1 && r.ref >= 0 &&
-                                ldexpf(e2_root, -2 * depth) > corr.tau2 * d2;
-                const uint64_t bc = __ballot(cc);
-                if (cc) {
-                    const uint32_t p = cn + __popcll(bc & below);
-                    cl_e[p] = e;
-                    cl_k[p] = (uint32_t)r.ref;
+#pragma unroll
+            for (int k = 0; k < kPop; ++k) {
+                float4 e;
+                float d2;
+                bool to_list, to_open;
+                classify(r[k], depth[k], e, d2, to_list, to_open);
+                const uint64_t bl = __ballot(to_list);
+                const uint64_t bo = __ballot(to_open);
+                if (to_list) lst[ln + count_below(bl)] = e;
+                if (to_open) {
+                    const uint32_t p = sp + count_below(bo);
+                    if (p < kStackCap) stk[p] = ((uint32_t)r[k].ref << 6) | (uint32_t)depth[k];
                 }
-                cn += __popcll(bc);  // < kCorrTile before, so at most kCellCap - 1 after
+                if constexpr (kCorr) {
+                    const bool cc = to_list && r[k].ref >= 0 &&
+                                    ldexpf(e2_root, -2 * depth[k]) > corr.tau2 * d2;
+                    const uint64_t bc = __ballot(cc);
+                    if (cc) {
+                        const uint32_t p = cn + count_below(bc);
+                        cl_e[p] = e;
+                        cl_k[p] = (uint32_t)r[k].ref;
+                    }
+                    cn += __popcll(bc);  // < kCorrTile before: at most kCellCap - 1 after
+                }
+                ln += __popcll(bl);  // < kGroup before: at most kListCap - 1 after
+                sp += __popcll(bo);
             }
-            ln += __popcll(bl);
-            sp += __popcll(bo);
             peak = sp > peak ? sp : peak;
             if (sp > kStackCap) {
                 overflow = true;
@@ -686,10 +719,11 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             // issue the next loads; the list evaluation below hides their latency
             take = sp > 0 ? pop_and_load() : 0u;
             if (ln >= kGroup) {
-                flush(kGroup);
+                uint32_t done = 0;
+                for (; ln - done >= (uint32_t)kGroup; done += kGroup) flush(done, kGroup);
                 __syncthreads();
-                const uint32_t rest = ln - kGroup;
-                if (lane < rest) lst[lane] = lst[kGroup + lane];
+                const uint32_t rest = ln - done;  // < kGroup
+                if (lane < rest) lst[lane] = lst[done + lane];
                 ln = rest;
                 __syncthreads();
             }
@@ -709,7 +743,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 }
             }
         }
-        if (ln > 0) flush(ln);
+        if (ln > 0) flush(0, ln);
         if constexpr (kCorr) {
             if (cn > 0) corr_flush(0, cn);
         }
@@ -772,7 +806,7 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
     float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
-    TreeHdr *h, CorrArgs corr) {
+    TreeHdr *h, uint32_t *gq, CorrArgs corr) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
     __shared__ float4 cl_e[kCorr ? kCellCap : 1];
@@ -791,19 +825,33 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t peak = 0, ncorr = 0;
     bool overflow = false;
 
-    // One queue over the octree-ordered group list: waves running at the same
-    // time walk neighbouring groups, whose interaction lists overlap, so the
-    // records they read are shared in L2.  (Splitting the list into one chunk
-    // per XCD cut HBM fetches 3x but lengthened the kernel by 40%: the tail
-    // of the slowest chunk dominates.)
+    // Groups come in octree order: waves running at the same time walk
+    // neighbouring groups, whose interaction lists overlap, so the records
+    // they read are shared in L2.
+    //
+    // The list is cut into kXcds contiguous chunks, one per XCD: workgroup b
+    // runs on XCD b mod kXcds, so the waves of one XCD walk one region of the
+    // tree and its records stay in that XCD's L2.  A wave whose chunk is empty
+    // goes on with the next chunks, so the XCDs finish together.  The chunk
+    // counters sit in separate cache lines.
+    uint32_t c = blockIdx.x % kXcds, visited = 0;
     for (;;) {
+        const uint32_t lo = (uint32_t)((uint64_t)ngroups * c / kXcds);
+        const uint32_t hi = (uint32_t)((uint64_t)ngroups * (c + 1) / kXcds);
         uint32_t slot = 0;
-        if (lane == 0)
-            slot = __hip_atomic_fetch_add(&h->group_next, 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t g = lane0_uniform(slot);
-        if (g >= ngroups) break;
-        const uint2 grp = groups[g];
+        if (lane == 0) {
+            uint32_t *q = gq + c * kQueueStride;
+            slot = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (slot < hi - lo)  // no read-modify-write once the chunk is done
+                slot = __hip_atomic_fetch_add(q, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        const uint32_t k = lane0_uniform(slot);
+        if (k >= hi - lo) {
+            if (++visited == kXcds) break;
+            c = c + 1 == kXcds ? 0u : c + 1;
+            continue;
+        }
+        const uint2 grp = groups[lo + k];
         if (!group_ok(grp, n)) {
             overflow = true;  // corrupt group table: report instead of reading out of bounds
             break;
@@ -815,7 +863,7 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
         if (lane == 0 && ncorr) atomicAdd(&h->corr_pairs, (unsigned long long)ncorr);
     }
     if (overflow) raise_flag(h, kFlagStack);
-    if (lane == 0) atomicMax(&h->stack_peak, peak);
+    if (lane == 0) raise_max(&h->stack_peak, peak);
 }
 
 // ---- ensemble of independent systems -------------------------------------------
@@ -979,11 +1027,11 @@ void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const 
                      const double *m, double eps2, double theta, double *ax, double *ay,
                      double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
-    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr);
+    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
     hipLaunchKernelGGL(k_bh_group<false>, dim3(blocks), dim3(kGroup), 0, s, t.n,
                        (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
-                       interactions, t.hdr, CorrArgs{});
+                       interactions, t.hdr, t.gq, CorrArgs{});
 }
 
 void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, const double *z,
@@ -991,11 +1039,11 @@ void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, c
                           double *ax, double *ay, double *az, uint32_t *interactions,
                           hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
-    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr);
+    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
     hipLaunchKernelGGL(k_bh_group<true>, dim3(blocks), dim3(kGroup), 0, s, t.n,
                        (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
                        (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
-                       interactions, t.hdr, corr);
+                       interactions, t.hdr, t.gq, corr);
 }
 
 void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items,

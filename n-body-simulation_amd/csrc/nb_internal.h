@@ -121,6 +121,8 @@ struct TreeBufs {
     uint32_t *group_slot = nullptr; // per sorted position: 1 if a group starts there, then slots
     uint32_t *spanning = nullptr;   // [block * kMaxDepth + depth]: cell crossing the block's end
     double4 *body4 = nullptr;       // {x, y, z, m} per body, input order: one load per body
+    uint32_t *gq = nullptr;         // traversal queue counters, one per XCD, 128 bytes apart
+    static constexpr uint32_t kGroupQueueWords = 8 * 32;
     uint32_t spanning_cap = 0;
     int32_t *ref = nullptr;
     double4 *cm = nullptr;          // FP64 records (reference-semantics traversal)
