@@ -40,13 +40,16 @@ MLP_PLAIN_THETA = 0.4
 # star + disk (dt = 1 h) and Plummer (dt = 1e-3); direct sum below the
 # autotuned threshold, Barnes-Hut theta = 0.5 above it.
 ENSEMBLE = dict(nsys=512, seed=0x4E424F44590004, lo_log2=10, hi_log2=16, theta=0.5,
-                err_systems=32, err_targets=64, cpu_systems=12)
+                err_systems=32, err_targets=64)
+# seconds of CPU loop time the reference CPU path is timed for
+CPU_SAMPLE_S = 10.0
 
 # HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this script scaled by a device copy of known size
-# (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0).
+# (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0):
+# 550,081 KiB x 2 read + 134,862 KiB written.
 TRAFFIC_BYTES = {
-    "bh_disk_1m": 3.33e9,
+    "bh_disk_1m": 1.265e9,
 }
 
 WORKLOADS = {
@@ -88,7 +91,14 @@ def cpu_baseline(d, algorithm, theta, dt):
     import oracle as orc
     n = d["m"].size
     threads = orc.get_threads()
-    iters = max(1, (4 << 20) // n) if algorithm == "BarnesHut" else 4
+    if algorithm == "BarnesHut":
+        iters = max(1, (4 << 20) // n)  # about 10 s at 1 M bodies, one iteration at 4 M
+    else:
+        # a 2-iteration probe sizes the run to about CPU_SAMPLE_S seconds
+        probe = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"],
+                             dt, 2 * dt)
+        per_iter = probe["loop_ms"] / max(1, probe["steps"])
+        iters = int(min(1000, max(4, CPU_SAMPLE_S * 1e3 / max(per_iter, 1e-3))))
     log(f"cpu baseline: {algorithm} n={n}, {iters} loop iterations on {threads} threads ...")
     r = orc.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"], dt,
                      iters * dt, theta=theta)
@@ -125,6 +135,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS) + ["ensemble_512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--theta-sweep", default="",
+                    help="comma-separated opening angles: also time the Barnes-Hut workload at "
+                         "each (same input), reported under config.theta_sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -196,6 +209,19 @@ def main():
             speedup_vs_plain=round(value / pvalue, 4))
         log(f"plain theta={MLP_PLAIN_THETA}: {pvalue:.4g} body-steps/s, err {perr:.3e}; "
             f"corrected: {value:.4g}, err {err:.3e}")
+    if args.theta_sweep and algorithm == "BarnesHut" and not mlp:
+        sweep = []
+        for th in (float(v) for v in args.theta_sweep.split(",")):
+            ctx.set_params(theta=th)
+            ssecs, sph, serr, _, sinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
+            sms = sph["Acceleration Kernel Time"] / args.steps
+            sflop = FLOP_PER_INTERACTION * float(sinter.sum())
+            sweep.append({"theta": th, "value": round(n * args.steps / ssecs, 1),
+                          "median_rel_err": serr, "mean_interactions": float(sinter.mean()),
+                          "force_ms": round(sms, 4),
+                          "frac": round(sflop / (sms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)})
+            log(f"theta {th}: {sweep[-1]['value']:.4g} body-steps/s, err {serr:.3e}")
+        out["config"]["theta_sweep"] = sweep
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, algorithm, theta, dt)
@@ -282,25 +308,29 @@ def run_ensemble(args):
 
 def ensemble_cpu_baseline(sizes, st, dts, uses_bh, off):
     """The reference CPU path (oracle restatement) runs one system at a time:
-    one loop iteration of a sample of the systems, Barnes-Hut or naive as the
-    tuned split chose, timed and scaled to body-steps/s."""
+    one loop iteration per system, Barnes-Hut or naive as the tuned split
+    chose, over systems in random order until about CPU_SAMPLE_S seconds of
+    loop time, scaled to body-steps/s."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
     c = ENSEMBLE
-    pick = np.random.default_rng(2).choice(c["nsys"], c["cpu_systems"], replace=False)
-    bodies, ms = 0, 0.0
-    for s in pick:
+    order = np.random.default_rng(2).permutation(c["nsys"])
+    bodies, ms, used = 0, 0.0, 0
+    for s in order:
         sl = slice(off[s], off[s + 1])
         algo = "BarnesHut" if uses_bh[s] else "naive"
         r = orc.simulate(algo, st["x"][sl], st["y"][sl], st["z"][sl], st["vx"][sl], st["vy"][sl],
                          st["vz"][sl], st["m"][sl], dts[s], dts[s], theta=c["theta"])
         bodies += int(sizes[s]) * r["steps"]
         ms += r["loop_ms"]
+        used += 1
+        if ms >= CPU_SAMPLE_S * 1e3:
+            break
     return {"value": bodies / (ms * 1e-3), "unit": "body-steps/s", "cores": orc.get_threads(),
             "kind": "port",
-            "sample": f"1 loop iteration of {c['cpu_systems']} of the systems ({bodies} body-steps), "
-                      f"FP64, naive below the tuned threshold and Barnes-Hut above; "
-                      f"{ms / 1e3:.1f} s"}
+            "sample": f"1 loop iteration of each of {used} systems in random order "
+                      f"({bodies} body-steps), FP64, naive below the tuned threshold and "
+                      f"Barnes-Hut above; {ms / 1e3:.1f} s"}
 
 
 def timed_run(ctx, d, algorithm, dt, steps, warmup):
