@@ -130,6 +130,8 @@ int status_to_code(nb_ctx *c, uint32_t st) {
     if (st & nb::kFlagZeroMass)
         return fail(c, NB_E_ZERO_MASS, "a non-empty octree cell has zero mass");
     if (st & nb::kFlagStack) return fail(c, NB_E_STACK_OVERFLOW, "traversal stack exhausted");
+    if (st & nb::kFlagSortWait)
+        return fail(c, NB_E_HIP, "radix sort: a look-back wait did not end");
     return NB_OK;
 }
 
@@ -195,12 +197,16 @@ int alloc_tree(nb_ctx *c) {
     if ((rc = dev_alloc(c, &t.cm, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.edge, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.rec, t.mcap))) return rc;
-    t.radix_blocks = nb::radix_blocks_for(n);
-    if ((rc = dev_alloc(c, &t.radix_hist, 256ull * t.radix_blocks))) return rc;
-    t.scan_part_cap = std::max(nb::scan_scratch_words(256u * t.radix_blocks),
-                               nb::scan_scratch_words(n));
+    const uint32_t ctl_words = nb::radix_ctl_words(n);
+    if ((rc = dev_alloc(c, &t.radix_ctl, ctl_words))) return rc;
+    t.scan_part_cap = nb::scan_scratch_words(n);
     if ((rc = dev_alloc(c, &t.scan_part, t.scan_part_cap))) return rc;
     NB_HIP(c, hipMemsetAsync(t.hdr, 0, sizeof(TreeHdr), c->stream));
+    NB_HIP(c, hipMemsetAsync(t.radix_ctl, 0, 4ull * ctl_words, c->stream));
+    // sort scratch starts as valid data (body index 0), so that nothing the
+    // sort could leave behind is out of range
+    NB_HIP(c, hipMemsetAsync(t.skey_alt, 0, 8ull * n, c->stream));
+    NB_HIP(c, hipMemsetAsync(t.sidx_alt, 0, 4ull * n, c->stream));
     return NB_OK;
 }
 
@@ -399,7 +405,7 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
                         t.spanning,
                         t.ref,
                         t.cm, t.edge, t.rec,
-                        t.radix_hist, t.scan_part};
+                        t.radix_ctl, t.scan_part};
         (void)hipStreamSynchronize(c->stream);
         for (void *b : bufs) dev_free(c, b);
         dev_free(c, c->quad);

@@ -29,6 +29,7 @@ enum : uint32_t {
     kFlagZeroMass = 4u,
     kFlagStack = 8u,
     kFlagAabb = 16u,
+    kFlagSortWait = 32u,  // radix sort look-back safety net (never expected)
 };
 
 // node reference encoding
@@ -130,8 +131,7 @@ struct TreeBufs {
     NodeF32 *rec = nullptr;         // FP32 records (group traversal)
     // scan / sort scratch
     uint32_t *scan_part = nullptr;
-    uint32_t *radix_hist = nullptr;
-    uint32_t radix_blocks = 0;
+    uint32_t *radix_ctl = nullptr;  // radix sort control words (owned by the sort)
     uint32_t scan_part_cap = 0;
 };
 
@@ -174,12 +174,15 @@ struct ForestBufs {
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total_dev,
                         uint32_t *scratch, uint32_t scratch_cap, hipStream_t s);
 uint32_t scan_scratch_words(uint32_t n);
-// Stable LSD radix sort of (key, val) pairs on key bits [0, bits).  Result in
-// keys/vals (the alt buffers are scratch).
+// Stable LSD radix sort of (key, val) pairs on key bits [0, bits), bits <= 64,
+// n < 2^30.  Result in keys/vals (the alt buffers are scratch).  ctl holds
+// radix_ctl_words(n) words, zeroed at allocation and then owned by the sort
+// (one ctl per stream: sorts sharing it must not overlap); a failed look-back
+// wait raises kFlagSortWait in *flags.
 void radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                        uint32_t n, int bits, uint32_t *hist, uint32_t hist_blocks,
-                        uint32_t *scan_scratch, uint32_t scan_cap, hipStream_t s);
+                        uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s);
 uint32_t radix_blocks_for(uint32_t n);
+uint32_t radix_ctl_words(uint32_t n);
 
 // nb_octree.hip
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,

@@ -1161,8 +1161,8 @@ void launch_keys(const TreeBufs &t, const double *x, const double *y, const doub
                  const double *m, hipStream_t s) {
     hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, m,
                        t.n, (const TreeHdr *)t.hdr, t.skey, t.sidx, t.body4);
-    radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels, t.radix_hist,
-                       t.radix_blocks, t.scan_part, t.scan_part_cap, s);
+    radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels, t.radix_ctl,
+                       &t.hdr->status, s);
 }
 
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,

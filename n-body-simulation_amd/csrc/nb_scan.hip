@@ -3,7 +3,8 @@
 // Both are written for 64-lane wavefronts: prefix sums use __shfl_up inside a
 // wave and LDS across the 4 waves of a 256-thread block; the radix sort ranks
 // keys stably with an 8-ballot "match any" on the 8-bit digit, so no
-// per-thread histogram is needed.  The octree build uses the scan for node
+// per-thread histogram is needed, and takes one launch per digit (plus one
+// counting launch per sort).  The octree build uses the scan for node
 // numbering and the sort for the octant keys (rocPRIM/hipCUB are not used).
 #include "nb_internal.h"
 
@@ -14,8 +15,10 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;
-constexpr int kRadixItems = 16;
-constexpr int kRadixTile = kThreads * kRadixItems;
+constexpr int kRadixTile = 4096;                       // keys per radix tile
+constexpr int kPassThreads = 512;
+constexpr int kPassWaves = kPassThreads / 64;
+constexpr int kPassItems = kRadixTile / kPassThreads;  // keys per lane
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -106,92 +109,180 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t *in, uint
     }
 }
 
-// ---- radix sort -------------------------------------------------------------
+// ---- radix sort: one sweep per digit -----------------------------------------
+//
+// Each pass is one k_radix_pass launch: a block takes the next tile id from a
+// counter, publishes its tile's per-digit counts, ranks its keys in LDS, and
+// learns how many keys of each digit the earlier tiles hold from their
+// published counts (decoupled look-back: a tile that has finished its
+// look-back publishes an inclusive prefix, so later tiles stop there).  Tile
+// ids are handed out in the order blocks start, so every tile a block waits on
+// belongs to a block that is running and that publishes its counts before it
+// waits on anything: every wait ends.  The digit totals (which do not depend
+// on the order) come from k_radix_count for pass 0 and from pass p - 1 for
+// pass p, which counts the next digit of the keys it has loaded anyway.
+//
+// Control words (u32):
+//   [0, 2048)              digit totals of pass p at p * 256
+//   kCtlTiles + p          tile counter of pass p
+//   kCtlStatus + plane     two status planes of nb x 256 words, plane p & 1
+//                          for pass p; pass p clears its tile's row of the
+//                          other plane for pass p + 1
+// Pass p > 0 zeroes the totals and the tile counter of pass p - 1 (that
+// launch has ended), and pass 0 those of the last pass.  Between sorts every
+// word is therefore zero except the last pass's totals and tile counter,
+// which pass 0 clears before pass passes - 2 adds to them (allocation sets
+// everything to zero).  A status word is 0 (not yet published),
+// kStatAgg | tile count, or kStatPrefix | count in tiles [0, t].  Counts stay
+// below 2^30: octree contexts hold fewer than 2^29 bodies.
 
-// Per-block 256-bin histogram of digit (key >> shift) & 255, stored bin-major
-// (hist[d * nb + block]) so that one exclusive scan gives every block its
-// output offset per digit in stable order.
-__global__ void __launch_bounds__(kThreads) k_radix_hist(const uint64_t *__restrict__ keys,
-                                                         uint32_t n, int shift, uint32_t *hist,
-                                                         uint32_t nb) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kRadixTile;
-#pragma unroll 4
-    for (int r = 0; r < kRadixItems; ++r) {
-        const uint64_t i = base + (uint64_t)r * kThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    hist[(uint64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+constexpr int kMaxPasses = 8;
+constexpr uint32_t kCtlTiles = kMaxPasses * 256;
+constexpr uint32_t kCtlStatus = kCtlTiles + 256;  // planes start 256-byte aligned
+constexpr uint32_t kStatAgg = 1u << 30;
+constexpr uint32_t kStatPrefix = 2u << 30;
+constexpr uint32_t kStatCount = kStatAgg - 1;
+constexpr int kLookWindow = 32;               // predecessor tiles read per look-back step
+constexpr uint32_t kLookSpinLimit = 1u << 22; // a safety net: the waits end (see above)
+constexpr int kMinPasses = 4;  // the control-word rotation above needs 3 or more
+constexpr int kCountThreads = 1024;
+constexpr uint32_t kCountBlocks = 256;
+
+__device__ inline uint32_t status_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void status_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Stable scatter of one tile of kRadixTile keys.  The tile is loaded at once
-// (kRadixItems rounds of 256 keys in index order) and ranked round by round:
-// a key's place among its digit's keys in the tile is (same digit in earlier
-// rounds) + (in earlier waves) + (in lower lanes of its own wave).  The tile
-// is then reordered in LDS by digit and written out digit run by digit run,
-// so consecutive threads write consecutive addresses.
-__global__ void __launch_bounds__(kThreads) k_radix_scatter(
+// Totals of the lowest 8-bit digit of the keys: total[digit].
+__global__ void __launch_bounds__(kCountThreads) k_radix_count(const uint64_t *__restrict__ keys,
+                                                               uint32_t n,
+                                                               uint32_t *__restrict__ total) {
+    __shared__ uint32_t h[256];
+    if (threadIdx.x < 256) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t stride = (uint64_t)gridDim.x * kCountThreads;
+#pragma unroll 4
+    for (uint64_t i = (uint64_t)blockIdx.x * kCountThreads + threadIdx.x; i < n; i += stride)
+        atomicAdd(&h[(uint32_t)keys[i] & 255u], 1u);
+    __syncthreads();
+    if (threadIdx.x < 256 && h[threadIdx.x]) atomicAdd(&total[threadIdx.x], h[threadIdx.x]);
+}
+
+// Exclusive prefix over threads [0, 256) of one value per thread (thread tid
+// owns digit tid).  Every thread of the block calls it (threads from 256 on
+// pass 0 and ignore the result); ws holds 4 words and is not reused.
+__device__ inline uint32_t digit_exclusive_scan(uint32_t c, uint32_t *ws) {
+    const uint32_t inc = wave_inclusive_scan(c);
+    const uint32_t w = threadIdx.x >> 6;
+    if (w < 4 && lane_id() == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t ex = inc - c;
+    for (uint32_t k = 0; k < w && k < 4; ++k) ex += ws[k];
+    return ex;
+}
+
+// Keys of digit d in tiles [0, t), t > 0: walk back from tile t - 1,
+// kLookWindow tiles per step, adding published counts up to and including the
+// first inclusive prefix.  False if a predecessor never published (cannot
+// happen, see above; the caller then raises a flag instead of waiting forever).
+__device__ inline bool look_back(const uint32_t *stat, uint32_t t, uint32_t d, uint32_t *excl) {
+    uint32_t sum = 0, spins = 0;
+    int64_t j = (int64_t)t - 1;
+    for (;;) {
+        uint32_t v[kLookWindow];
+#pragma unroll
+        for (int k = 0; k < kLookWindow; ++k)
+            v[k] = j - k >= 0 ? status_load(stat + (uint64_t)(j - k) * 256u + d) : kStatPrefix;
+        bool pending = false;
+#pragma unroll
+        for (int k = 0; k < kLookWindow; ++k) pending |= v[k] == 0;
+        while (pending) {
+            if (++spins > kLookSpinLimit) return false;
+            __builtin_amdgcn_s_sleep(2);
+            pending = false;
+#pragma unroll
+            for (int k = 0; k < kLookWindow; ++k) {
+                if (v[k] == 0) v[k] = status_load(stat + (uint64_t)(j - k) * 256u + d);
+                pending |= v[k] == 0;
+            }
+        }
+        bool found = false;
+#pragma unroll
+        for (int k = 0; k < kLookWindow; ++k) {
+            if (!found) {
+                sum += v[k] & kStatCount;
+                found = v[k] >= kStatPrefix;
+            }
+        }
+        if (found) {
+            *excl = sum;
+            return true;
+        }
+        j -= kLookWindow;
+    }
+}
+
+// One stable pass over digit (key >> 8 pass) & 255 of a tile of kRadixTile
+// keys.  Wave w holds the tile's keys [w * 512, (w + 1) * 512) and ranks them
+// 64 at a time against its own digit counters, with no barrier between
+// rounds: a key's place among its digit's keys in the tile is (same digit in
+// earlier waves) + (in earlier rounds of its wave) + (in lower lanes of its
+// round), so the order is stable.  The tile is reordered in LDS by digit and
+// written out digit run by digit run, so consecutive threads write
+// consecutive addresses.
+__global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
-    uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals, uint32_t n, int shift,
-    const uint32_t *__restrict__ offs, uint32_t nb) {
-    __shared__ uint32_t running[256];  // tile histogram, then the per-digit cursor
-    __shared__ uint32_t dbase[256];    // first tile position of digit d
-    __shared__ uint32_t gbase[256];    // output position of the tile's first digit-d key
-    __shared__ uint32_t wsum[kWaves];
-    __shared__ uint32_t wcnt[2][kWaves][256];
+    uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals, uint32_t n, int pass, int passes,
+    uint32_t *__restrict__ ctl, uint32_t nb, uint32_t *__restrict__ flags) {
+    __shared__ uint32_t wcnt[kPassWaves][256];  // digit-d keys of wave w, then their first place
+    __shared__ uint32_t next_hist[256];          // tile histogram of the next pass's digit
+    __shared__ uint32_t dbase[256];              // first tile position of digit d
+    __shared__ uint32_t gbase[256];              // output position of the tile's first digit-d key
+    __shared__ uint32_t wsum[2][4];
     __shared__ uint64_t lk[kRadixTile];
     __shared__ uint32_t lv[kRadixTile];
+    __shared__ uint32_t tile_id, stuck;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    const uint64_t tile = (uint64_t)blockIdx.x * kRadixTile;
-    const uint64_t rest = (uint64_t)n - tile;  // > 0: one block per started tile
+    const bool owner = tid < 256;  // thread tid owns digit tid
+    const int shift = 8 * pass;
+    uint32_t *total = ctl + pass * 256;
+    uint32_t *stat = ctl + kCtlStatus + (uint64_t)(pass & 1) * 256u * nb;
+    uint32_t *stat_next = ctl + kCtlStatus + (uint64_t)((pass + 1) & 1) * 256u * nb;
+    const bool has_next = pass + 1 < passes;
+    if (tid == 0) {
+        tile_id = atomicAdd(ctl + kCtlTiles + pass, 1u);
+        stuck = 0;
+    }
+    for (uint32_t i = tid; i < kPassWaves * 256u; i += kPassThreads) (&wcnt[0][0])[i] = 0;
+    if (owner) next_hist[tid] = 0;
+    __syncthreads();
+    const uint32_t t = tile_id;
+    if (t == 0 && owner) {
+        // the previous pass (or the last pass of the previous sort) has ended
+        const int prev = pass > 0 ? pass - 1 : passes - 1;
+        ctl[prev * 256 + tid] = 0;
+        if (tid == 0) ctl[kCtlTiles + prev] = 0;
+    }
+    const uint64_t tile = (uint64_t)t * kRadixTile;
+    const uint64_t rest = (uint64_t)n - tile;  // > 0: one block per tile
     const uint32_t tn = rest < (uint64_t)kRadixTile ? (uint32_t)rest : (uint32_t)kRadixTile;
-    uint64_t kk[kRadixItems];
-    uint32_t vv[kRadixItems], dg[kRadixItems];
+    const uint32_t wbase = w * (kPassItems * 64u);
+    uint64_t kk[kPassItems];
+    uint32_t vv[kPassItems], dg[kPassItems], rk[kPassItems];
 #pragma unroll
-    for (int r = 0; r < kRadixItems; ++r) {
-        const uint32_t j = (uint32_t)r * kThreads + tid;
+    for (int r = 0; r < kPassItems; ++r) {
+        const uint32_t j = wbase + (uint32_t)r * 64u + lane;
         kk[r] = j < tn ? keys[tile + j] : ~0ull;
         vv[r] = j < tn ? vals[tile + j] : 0u;
     }
-    running[tid] = 0;
-    gbase[tid] = offs[(uint64_t)tid * nb + blockIdx.x];
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int k = 0; k < kWaves; ++k) wcnt[b][k][tid] = 0;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kRadixItems; ++r) {
-        dg[r] = (uint32_t)(kk[r] >> shift) & 255u;
-        if ((uint32_t)r * kThreads + tid < tn) atomicAdd(&running[dg[r]], 1u);
-    }
-    __syncthreads();
-    {
-        // exclusive scan of the tile histogram: thread tid owns digit tid
-        const uint32_t c = running[tid];
-        uint32_t inc = c;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t o = __shfl_up(inc, off, 64);
-            if (lane >= (uint32_t)off) inc += o;
-        }
-        if (lane == 63) wsum[w] = inc;
-        __syncthreads();
-        uint32_t ex = inc - c;
-        for (uint32_t k = 0; k < w; ++k) ex += wsum[k];
-        dbase[tid] = ex;
-        running[tid] = ex;
-    }
-    __syncthreads();
     const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64u - lane));
 #pragma unroll
-    for (int r = 0; r < kRadixItems; ++r) {
-        const int buf = r & 1;
-        const bool valid = (uint32_t)r * kThreads + tid < tn;
-        const uint32_t d = valid ? dg[r] : 255u;
+    for (int r = 0; r < kPassItems; ++r) {
+        const bool valid = wbase + (uint32_t)r * 64u + lane < tn;
+        const uint32_t d = (uint32_t)(kk[r] >> shift) & 255u;
+        dg[r] = d;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int b = 0; b < 8; ++b) {
@@ -200,34 +291,65 @@ __global__ void __launch_bounds__(kThreads) k_radix_scatter(
             peers &= bit ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
-        if (valid && rank == 0) wcnt[buf][w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {
-            // thread tid owns digit tid: turn the per-wave counts into positions
-            uint32_t run = running[tid];
+        // a wave's LDS accesses complete in order: the next round reads this write
+        const uint32_t before = wcnt[w][d];
+        rk[r] = before + rank;
+        if (valid && rank == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+        if (valid && has_next) atomicAdd(&next_hist[(uint32_t)(kk[r] >> (shift + 8)) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t c = 0, gtot = 0;
+    if (owner) {
+        // per-wave counts of digit tid become offsets among the tile's digit-tid keys
 #pragma unroll
-            for (int k = 0; k < kWaves; ++k) {
-                const uint32_t c = wcnt[buf][k][tid];
-                wcnt[buf][k][tid] = run;
-                run += c;
-                wcnt[buf ^ 1][k][tid] = 0;  // next round's counters
-            }
-            running[tid] = run;
+        for (int k = 0; k < kPassWaves; ++k) {
+            const uint32_t cnt = wcnt[k][tid];
+            wcnt[k][tid] = c;
+            c += cnt;
         }
-        __syncthreads();
-        if (valid) {
-            const uint32_t p = wcnt[buf][w][d] + rank;
+        status_store(stat + (uint64_t)t * 256u + tid, (t == 0 ? kStatPrefix : kStatAgg) | c);
+        stat_next[(uint64_t)t * 256u + tid] = 0;
+        if (has_next && next_hist[tid]) atomicAdd(total + 256 + tid, next_hist[tid]);
+        gtot = total[tid];
+    }
+    const uint32_t ex = digit_exclusive_scan(c, wsum[0]);
+    const uint32_t goff = digit_exclusive_scan(gtot, wsum[1]);
+    if (owner) {
+        dbase[tid] = ex;
+#pragma unroll
+        for (int k = 0; k < kPassWaves; ++k) wcnt[k][tid] += ex;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kPassItems; ++r) {
+        if (wbase + (uint32_t)r * 64u + lane < tn) {
+            const uint32_t p = wcnt[w][dg[r]] + rk[r];
             lk[p] = kk[r];
             lv[p] = vv[r];
         }
     }
+    if (owner) {
+        uint32_t excl = 0;
+        if (t > 0) {
+            if (!look_back(stat, t, tid, &excl)) stuck = 1;
+            // published even when stuck, so that later tiles do not wait as well
+            status_store(stat + (uint64_t)t * 256u + tid, kStatPrefix | (excl + c));
+        }
+        gbase[tid] = goff + excl;
+    }
     __syncthreads();
-    for (uint32_t j = tid; j < tn; j += kThreads) {
+    if (stuck) {
+        if (tid == 0) atomicOr(flags, kFlagSortWait);
+        return;
+    }
+    for (uint32_t j = tid; j < tn; j += kPassThreads) {
         const uint64_t key = lk[j];
         const uint32_t d = (uint32_t)(key >> shift) & 255u;
         const uint32_t pos = gbase[d] + (j - dbase[d]);
-        okeys[pos] = key;
-        ovals[pos] = lv[j];
+        if (pos < n) {
+            okeys[pos] = key;
+            ovals[pos] = lv[j];
+        }
     }
 }
 
@@ -248,22 +370,21 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
 
 uint32_t radix_blocks_for(uint32_t n) { return ceil_div(n ? n : 1, kRadixTile); }
 
+uint32_t radix_ctl_words(uint32_t n) { return kCtlStatus + 2u * 256u * radix_blocks_for(n); }
+
 void radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                        uint32_t n, int bits, uint32_t *hist, uint32_t hist_blocks,
-                        uint32_t *scan_scratch, uint32_t scan_cap, hipStream_t s) {
-    if (n <= 1) return;
+                        uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s) {
+    if (n <= 1 || bits <= 0 || bits > 8 * kMaxPasses || n >= kStatAgg) return;
     const uint32_t nb = radix_blocks_for(n);
-    if (nb > hist_blocks) return;
-    int passes = (bits + 7) / 8;
+    int passes = std::max((bits + 7) / 8, kMinPasses);
     if (passes & 1) ++passes;  // even pass count: the result lands back in keys/vals
+    hipLaunchKernelGGL(k_radix_count, dim3(std::min(nb, kCountBlocks)), dim3(kCountThreads), 0, s,
+                       (const uint64_t *)keys, n, ctl);
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kThreads), 0, s, ka, n, shift, hist, nb);
-        scan_exclusive_u32(hist, hist, 256u * nb, nullptr, scan_scratch, scan_cap, s);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kThreads), 0, s, ka, va, kb, vb, n,
-                           shift, (const uint32_t *)hist, nb);
+        hipLaunchKernelGGL(k_radix_pass, dim3(nb), dim3(kPassThreads), 0, s, (const uint64_t *)ka,
+                           (const uint32_t *)va, kb, vb, n, p, passes, ctl, nb, flags);
         uint64_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }

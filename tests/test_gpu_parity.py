@@ -120,6 +120,46 @@ def test_tree_reference_fixture(gpu, oracle):
     assert_canonical_equal(can, t.canonical(), "fixture")
 
 
+def test_tree_parity_repeated_builds(gpu, oracle):
+    # The radix sort keeps control words (digit totals, tile counters, look-back
+    # status) in the context between builds; three builds of different
+    # positions on one context, 18 sort tiles each, must each match the oracle.
+    n = 70001
+    with gpu.Context(n) as c:
+        for seed in (11, 12, 13):
+            d = clustered(n, seed) if seed != 12 else uniform_cube(n, seed)
+            c.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+            c.bh_build()
+            sorted_gpu = c.tree_sorted()
+            can = c.tree_canonical()
+            t = oracle.Tree(n)
+            t.build(d["x"], d["y"], d["z"], d["m"])
+            np.testing.assert_array_equal(sorted_gpu, t.export()["sorted"], err_msg=f"seed {seed}")
+            assert_canonical_equal(can, t.canonical(), f"seed {seed}")
+
+
+def test_tree_sort_more_tiles_than_resident(gpu, oracle):
+    # 733 sort tiles of 4096 keys (the last one ragged): more tiles than the
+    # device holds at once, so some look-back waits name tiles whose blocks
+    # start later.  Two builds, positions moved in between.
+    n = 3000017
+    d = plummer(gpu, n)
+    with ctx_for(gpu, d, precision=gpu.PREC_FP32C) as c:
+        for step in range(2):
+            if step:
+                rng = np.random.default_rng(5)
+                for k in ("x", "y", "z"):
+                    d[k] = d[k] + rng.normal(0.0, 1e-3, n)
+                c.set_positions(d["x"], d["y"], d["z"])
+            c.bh_build()
+            sorted_gpu = c.tree_sorted()
+            info = c.tree_info()
+            t = oracle.Tree(n)
+            t.build(d["x"], d["y"], d["z"], d["m"])
+            np.testing.assert_array_equal(sorted_gpu, t.export()["sorted"], err_msg=f"build {step}")
+            assert info["node_count"] == t.node_count
+
+
 @pytest.mark.parametrize("kind,n", [("cube", 1), ("cube", 2), ("cube", 9), ("cube", 1000),
                                     ("cluster", 5000), ("disk", 20000), ("plummer", 20000),
                                     ("cube", 70001)])
