@@ -228,9 +228,8 @@ int build_tree(nb_ctx *c) {
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
-    nb::launch_com(t, c->stream);
     const bool fp64 = c->p.precision == NB_PREC_FP64;
-    nb::launch_node_records(t, c->p.G, fp64, !fp64, c->stream);
+    nb::launch_com(t, c->p.G, fp64, !fp64, c->stream);
     c->quad_valid = false;
     if (c->corr && !fp64) {
         nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);

@@ -94,10 +94,15 @@ struct NodeF32 {
 };
 
 // Bodies per block of the first centre-of-mass pass: cells whose body range
-// stays inside one block are summed there, level by level in LDS; the few
-// cells crossing a block boundary are finished by one workgroup.  About 0.48
-// internal cells per body, so a block's cells fit the LDS stage.
-constexpr uint32_t kComBlock = 2048;
+// stays inside one block are summed there, level by level in LDS (a disk or
+// a Plummer sphere has about 0.48 internal cells per body, so a block's cells
+// fit the LDS stage, and two blocks share a CU); the few cells crossing a
+// block boundary are finished by the spanning passes.
+constexpr uint32_t kComBlock = 1024;
+// Blocks per super-block: cells crossing block boundaries are summed by one
+// workgroup per super-block, those crossing super-block boundaries by one more.
+constexpr uint32_t kComSuperBlocks = 64;
+constexpr uint32_t kComSuper = kComBlock * kComSuperBlocks;
 
 // Device buffers of one octree.
 struct TreeBufs {
@@ -113,9 +118,10 @@ struct TreeBufs {
     uint32_t *body_parent = nullptr;  // n: internal index of each sorted body's parent
     uint8_t *int_depth = nullptr;
     uint32_t *int_parent = nullptr, *int_node = nullptr, *int_count = nullptr;
-    uint32_t *int_lvl = nullptr;    // centre of mass: cells bucketed by depth
+    uint32_t *int_lvl = nullptr;    // centre of mass: depth buckets, or LDS ids of spanning cells
     InternalSums *int_sums = nullptr;
-    uint8_t *int_local = nullptr;   // 1 if the cell's bodies lie in one kComBlock block
+    uint8_t *int_local = nullptr;   // 1: bodies in one kComBlock block; 2: crosses a kComSuper
+                                    // boundary; 0: crosses block boundaries only
     uint32_t *int_first = nullptr;  // first sorted position of the cell's bodies
     uint2 *groups = nullptr;        // traversal groups {first sorted position, body count}
     uint32_t *group_size = nullptr; // per sorted position: size of the group starting there
@@ -192,12 +198,12 @@ void launch_keys(const TreeBufs &t, const double *x, const double *y, const doub
                  const double *m, hipStream_t s);
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,
                      hipStream_t s);
-void launch_com(const TreeBufs &t, hipStream_t s);
+// Centre of mass with the node records and traversal groups.  f64: FP64
+// records (cm, edge) for the reference-semantics traversal; f32: packed FP32
+// records and traversal groups for the group traversal.
+void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s);
 // hdr->max_depth of the last build (not maintained by the build itself)
 void launch_max_depth(const TreeBufs &t, hipStream_t s);
-// f64: FP64 records (cm, edge) for the reference-semantics traversal;
-// f32: packed FP32 records and traversal groups for the group traversal.
-void launch_node_records(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s);
 
 // Octree of every Barnes-Hut system of an ensemble, one workgroup per system
 // (order: the BH systems' indices, largest first); items[s] receives the

@@ -370,15 +370,17 @@ __global__ void __launch_bounds__(kThreads) k_fill(TreeHdr *h, uint32_t icap, in
 }
 
 // Internal cells owned by pair i and the leaf of sorted body i.  A cell is
-// "local" when its bodies stay inside one com_block-body block of the sorted
-// order; the others go to the table spanning[block * kMaxDepth + depth].
+// "local" (int_local 1) when its bodies stay inside one com_block-body block
+// of the sorted order; the others go to the table
+// spanning[block * kMaxDepth + depth], with int_local 2 if they also cross a
+// boundary of the com_super-body super-blocks and 0 if not.
 // No atomics: every cell, node slot and table entry has exactly one writer.
 __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h, KeyView kv,
                                const uint32_t *__restrict__ sidx, const int32_t *__restrict__ lcp,
                                const uint32_t *__restrict__ off, int32_t *ref, uint8_t *int_depth,
                                uint32_t *int_parent, uint32_t *int_node, uint32_t *body_parent,
                                uint8_t *int_local, uint32_t *spanning, uint32_t spanning_cap,
-                               uint32_t *int_first, uint32_t com_block) {
+                               uint32_t *int_first, uint32_t com_block, uint32_t com_super) {
     if (n == 1) {  // the root is a leaf holding the only body (max_depth stays 0)
         ref[0] = ref_of_body(sidx[0]);
         body_parent[0] = 0xffffffffu;
@@ -403,8 +405,12 @@ __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h
         // that block's first body shares its l-digit prefix with body i.
         const uint32_t next_block = (i / com_block + 1) * com_block;
         const bool local = !(next_block < n && kv.same(next_block, i, l));
-        int_local[k] = local ? 1 : 0;
-        if (!local) {
+        if (local) {
+            int_local[k] = 1;
+        } else {
+            // the same test at the next super-block boundary
+            const uint32_t next_super = (i / com_super + 1) * com_super;
+            int_local[k] = next_super < n && kv.same(next_super, i, l) ? 2 : 0;
             // the block of body i owns at most one crossing cell per depth:
             // the depth-l cell holding the block's last and the next block's
             // first body
@@ -454,7 +460,7 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
     const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     link_at(i, n, icap, h, kv, sidx, lcp, off, ref, int_depth, int_parent, int_node, body_parent,
-            int_local, spanning, spanning_cap, int_first, kComBlock);
+            int_local, spanning, spanning_cap, int_first, kComBlock, kComSuper);
 }
 
 // ---- 7. centre of mass --------------------------------------------------------
@@ -515,21 +521,127 @@ __device__ inline void sum_children(uint32_t pk, const double4 *__restrict__ bod
     if (s.m == 0) raise_flag(h, kFlagZeroMass);
 }
 
+// Traversal records and groups, written by the centre-of-mass kernels: a cell,
+// once summed, writes the records of its 8 child slots (their data is at
+// hand) and, if it holds more than kGroupMax bodies, marks the traversal
+// groups among its children (a child cell with at most kGroupMax bodies, or a
+// body).  The kernel that sums the root writes the root's record, and its
+// group if the root holds at most kGroupMax bodies.
+struct RecOut {
+    double G;
+    int f64, f32;  // FP64 records (cm, edge); FP32 records and traversal groups
+    double4 *cm;
+    double *edge;
+    NodeF32 *rec;
+    uint32_t *group_size, *group_slot;
+};
+
+// Record centre of a child from its term: a cell's sums, or a body's
+// (x m, y m, z m, m).  The reference divides numerator by mass at every visit
+// (BarnesHutAlgorithm.cpp:351-353); the quotient is the same.
+__device__ inline double4 term_com(const double4 v) {
+    return make_double4(v.x / v.w, v.y / v.w, v.z / v.w, v.w);
+}
+
+__device__ inline void put_record(const RecOut &o, uint64_t j, int32_t r, const double4 c,
+                                  double e) {
+    if (o.f64) {
+        o.cm[j] = c;
+        o.edge[j] = e;
+    }
+    if (o.f32) {
+        NodeF32 q;
+        q.x = (float)c.x;
+        q.y = (float)c.y;
+        q.z = (float)c.z;
+        q.gm = (float)(o.G * c.w);
+        q.ref = r;
+        q.lx = (float)(c.x - (double)q.x);
+        q.ly = (float)(c.y - (double)q.y);
+        q.lz = (float)(c.z - (double)q.z);
+        o.rec[j] = q;
+    }
+}
+
+// a traversal group of `size` bodies starts at sorted position p
+__device__ inline void put_group(const RecOut &o, uint32_t p, uint32_t size) {
+    if (o.f32) {
+        o.group_size[p] = size;
+        o.group_slot[p] = 1u;
+    }
+}
+
+// Records and groups of the children of cell pk (depth `depth`, first sorted
+// position `first`), given their refs, terms and body counts; the root's own
+// record and group when pk is the root.
+__device__ inline void cell_outputs(const RecOut &o, uint32_t pk, uint32_t depth, uint32_t first,
+                                    double root_edge, const int32_t (&r)[8],
+                                    const double4 (&v)[8], const uint32_t (&c)[8],
+                                    const double4 s, uint32_t cnt) {
+    const uint64_t base = 1ull + 8ull * pk;
+    const double ce = ldexp(root_edge, -(int)(depth + 1));
+    uint32_t pos = first;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        if (r[d] == kRefEmpty) {
+            put_record(o, base + d, kRefEmpty, make_double4(0.0, 0.0, 0.0, 0.0), 0.0);
+            continue;
+        }
+        put_record(o, base + d, r[d], term_com(v[d]), r[d] >= 0 ? ce : 0.0);
+        if (cnt > kGroupMax && c[d] <= kGroupMax) put_group(o, pos, c[d]);
+        pos += c[d];
+    }
+    if (depth == 0) {
+        put_record(o, 0, (int32_t)pk, term_com(s), root_edge);
+        if (cnt <= kGroupMax) put_group(o, first, cnt);
+    }
+}
+
+// The global-memory path: sum_children, then the children's records and groups.
+__device__ inline void sum_children_out(uint32_t pk, uint32_t depth, uint32_t first,
+                                        double root_edge, const double4 *__restrict__ body4,
+                                        const int32_t *__restrict__ ref, InternalSums *sums,
+                                        uint32_t *counts, TreeHdr *h, const RecOut &o) {
+    const uint64_t base = 1ull + 8ull * pk;
+    int32_t r[8];
+    double4 v[8];
+    uint32_t c[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) r[d] = ref[base + d];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        v[d] = make_double4(0.0, 0.0, 0.0, 0.0);
+        c[d] = 0;
+        if (r[d] >= 0) {
+            const InternalSums q = sums[r[d]];
+            v[d] = make_double4(q.x, q.y, q.z, q.m);
+            c[d] = counts[r[d]];
+        } else if (r[d] != kRefEmpty) {
+            v[d] = body_term(body4[body_of_ref(r[d])]);
+            c[d] = 1;
+        }
+    }
+    InternalSums q;
+    uint32_t cnt;
+    add_children(r, v, c, q, cnt);
+    sums[pk] = q;
+    counts[pk] = cnt;
+    if (q.m == 0) raise_flag(h, kFlagZeroMass);
+    cell_outputs(o, pk, depth, first, root_edge, r, v, c, make_double4(q.x, q.y, q.z, q.m), cnt);
+}
+
 // Centre of mass, level by level from the deepest cells up: a cell's sums
 // (its children in digit order) need only deeper cells, so every cell of one
 // depth can be summed at once, with a barrier between depths.
 constexpr int kComThreads = 1024;
 
 // Cells [kb, ke) bucketed by depth into lvl[kb ..) (cells with keep(k) only),
-// then summed deepest depth first.  cnt/cur: kMaxDepth + 1 and kMaxDepth
-// LDS words.  Every thread of the workgroup calls it.
-template <typename Keep>
+// then sum(k) for each, deepest depth first.  cnt/cur: kMaxDepth + 1 and
+// kMaxDepth LDS words.  Every thread of the workgroup calls it.
+template <typename Keep, typename Sum>
 __device__ inline void com_levels(uint32_t kb, uint32_t ke, Keep keep,
                                   const uint8_t *__restrict__ int_depth, uint32_t *lvl,
-                                  uint32_t *cnt, uint32_t *cur,
-                                  const double4 *__restrict__ body4,
-                                  const int32_t *__restrict__ ref, InternalSums *sums,
-                                  uint32_t *counts, TreeHdr *h) {
+                                  uint32_t *cnt, uint32_t *cur, Sum sum) {
     const uint32_t t = threadIdx.x, nt = blockDim.x;
     for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += nt) cnt[d] = 0;
     __syncthreads();
@@ -554,53 +666,296 @@ __device__ inline void com_levels(uint32_t kb, uint32_t ke, Keep keep,
     for (int d = kMaxDepth - 1; d >= 0; --d) {
         const uint32_t lb = cnt[d], le = cnt[d + 1];
         if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
-        for (uint32_t j = lb + t; j < le; j += nt)
-            sum_children(lvl[kb + j], body4, ref, sums, counts, h);
+        for (uint32_t j = lb + t; j < le; j += nt) sum(lvl[kb + j]);
         __syncthreads();
     }
 }
 
 // Pass 1: one workgroup per kComBlock consecutive (sorted) bodies sums the
 // cells that lie inside its block.  Pair i owns cells [off[i], off[i+1]), so
-// the block's cells are one contiguous range [kb, ke).  Up to kComLdsCells of
-// them are summed in LDS: their child references are staged once
-// (coalesced), and a child cell's sums are read from LDS, so a level costs one
-// global round trip (the body children) instead of three.  A larger block
-// falls back to the same sums in global memory (lvl: depth buckets).
-constexpr uint32_t kComLdsCells = 1536;  // a 2048-body block holds about 1,000 cells
-__global__ void __launch_bounds__(kComThreads) k_com_local(
+// the block's cells are one contiguous range [kb, ke).  Up to kComLdsCells
+// cells are summed entirely in LDS.  Staged once (all loads in flight
+// together): the block's bodies and their input indices in sorted order, and
+// per cell its depth, first position and a 16-bit code per child slot
+// (empty, body, or the child cell's place in the block).  The children of a
+// cell cover consecutive sorted ranges in digit order, so a body child is the
+// body at the running position: the cell's first position plus the counts of
+// its earlier children; the level pass stores that position in the body's
+// code.  The levels touch LDS only (a barrier also waits for global stores).
+// A last pass writes the sums, the node records of the cells' child slots and
+// the group marks, consecutive threads on consecutive slots.  A larger block
+// falls back to the same work in global memory.  The block also clears the
+// group marks of its positions before any is set.
+constexpr uint32_t kComLdsCells = 576;   // about 490 per block; 537 the most seen (disk, Plummer)
+constexpr int kComLocalThreads = 512;
+constexpr int kComBodiesPerThread = kComBlock / kComLocalThreads;
+constexpr int kComRefsPerThread = 8 * kComLdsCells / kComLocalThreads;  // child slots
+static_assert(kComBlock % kComLocalThreads == 0 && (8 * kComLdsCells) % kComLocalThreads == 0,
+              "whole staging rounds");
+constexpr uint16_t kCodeEmpty = 0xffffu, kCodeBody = 0xfffeu, kCodeFar = 0xfffdu;
+constexpr uint16_t kCodeAt = 0x8000u;    // | sorted position - block start: a placed body
+static_assert(kComLdsCells < kCodeAt && kComBlock <= 0x7ffdu, "16-bit LDS codes");
+__global__ void __launch_bounds__(kComLocalThreads) k_com_local(
     uint32_t n, uint32_t icap, TreeHdr *h, const double4 *__restrict__ body4,
     const uint32_t *__restrict__ sidx, const int32_t *__restrict__ ref,
     const uint32_t *__restrict__ off, const uint8_t *__restrict__ int_depth,
-    const uint8_t *__restrict__ int_local, uint32_t *lvl, InternalSums *sums, uint32_t *counts) {
+    const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *lvl,
+    InternalSums *sums, uint32_t *counts, RecOut o) {
+    constexpr uint32_t nt = kComLocalThreads;
     __shared__ uint32_t cnt[kMaxDepth + 1], cur[kMaxDepth];
-    __shared__ uint32_t lvl_s[kComLdsCells];
-    __shared__ int32_t ref_s[8 * kComLdsCells];
-    __shared__ double sx_s[kComLdsCells], sy_s[kComLdsCells], sz_s[kComLdsCells],
-        sm_s[kComLdsCells];
-    __shared__ uint32_t n_s[kComLdsCells];
+    __shared__ unsigned long long levels;  // bit d: a local cell of depth d
+    __shared__ double4 b_s[kComBlock];     // the block's bodies {x, y, z, m}, sorted order
+    __shared__ uint32_t i_s[kComBlock];    // their input indices
+    __shared__ double4 c_s[kComLdsCells];  // cell sums {x m, y m, z m, m}
+    __shared__ uint32_t n_s[kComLdsCells]; // cell body counts
+    __shared__ uint16_t code_s[8 * kComLdsCells];
+    __shared__ uint16_t first_s[kComLdsCells];  // first sorted position - block start
+    __shared__ uint8_t depth_s[kComLdsCells];   // depth; kMaxDepth if not local
     const uint32_t t = threadIdx.x;
     const uint32_t s = blockIdx.x * kComBlock;
     const uint32_t e = min(n, s + kComBlock);
-    for (uint32_t i = s + t; i < e; i += kComThreads)
-        if (body4[sidx[i]].w == 0) raise_flag(h, kFlagZeroMass);
-    if (n < 2) return;  // a lone body is the root leaf: no internal cell
+    const double root_edge = h->edge;
+    {
+        // all index loads, then all body loads, in flight together
+        uint32_t bi[kComBodiesPerThread];
+        double4 b[kComBodiesPerThread];
+#pragma unroll
+        for (int q = 0; q < kComBodiesPerThread; ++q) {
+            const uint32_t i = s + t + q * nt;
+            bi[q] = i < e ? sidx[i] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < kComBodiesPerThread; ++q)
+            if (s + t + q * nt < e) b[q] = body4[bi[q]];
+#pragma unroll
+        for (int q = 0; q < kComBodiesPerThread; ++q) {
+            const uint32_t i = s + t + q * nt;
+            if (i >= e) continue;
+            if (b[q].w == 0) raise_flag(h, kFlagZeroMass);
+            b_s[i - s] = b[q];
+            i_s[i - s] = bi[q];
+            if (o.f32) {
+                o.group_size[i] = 0;
+                o.group_slot[i] = 0;
+            }
+        }
+    }
+    if (n < 2) {  // a lone body is the root leaf: no internal cell
+        __syncthreads();
+        if (t == 0) {
+            put_record(o, 0, ref_of_body(i_s[0]), term_com(body_term(b_s[0])), 0.0);
+            put_group(o, 0, 1);
+        }
+        return;
+    }
     const uint32_t I = min(h->internal_count, icap);
     const uint32_t kb = s < n - 1 ? min(off[s], I) : I;
     const uint32_t ke = e < n - 1 ? min(off[e], I) : I;
     const uint32_t nc = ke - kb;
+    const uint32_t nbody = e - s;
     if (nc > kComLdsCells) {  // the same for the whole workgroup
-        com_levels(kb, ke, [&](uint32_t k) { return int_local[k] != 0; }, int_depth, lvl, cnt,
-                   cur, body4, ref, sums, counts, h);
+        com_levels(kb, ke, [&](uint32_t k) { return int_local[k] == 1; }, int_depth, lvl, cnt,
+                   cur, [&](uint32_t k) {
+                       sum_children_out(k, int_depth[k], int_first[k], root_edge, body4, ref,
+                                        sums, counts, h, o);
+                   });
         return;
     }
-    for (uint32_t j = t; j < 8 * nc; j += kComThreads) ref_s[j] = ref[1ull + 8ull * kb + j];
+    if (t == 0) levels = 0;
+    for (uint32_t c = t; c < nc; c += nt) {
+        const bool local = int_local[kb + c] == 1;
+        depth_s[c] = local ? (uint8_t)min((uint32_t)int_depth[kb + c], (uint32_t)kMaxDepth - 1)
+                           : (uint8_t)kMaxDepth;
+        first_s[c] = (uint16_t)(int_first[kb + c] - s);
+    }
+    {
+        int32_t r[kComRefsPerThread];
+#pragma unroll
+        for (int q = 0; q < kComRefsPerThread; ++q) {
+            const uint32_t j = t + q * nt;
+            r[q] = j < 8 * nc ? ref[1ull + 8ull * kb + j] : kRefEmpty;
+        }
+#pragma unroll
+        for (int q = 0; q < kComRefsPerThread; ++q) {
+            const uint32_t j = t + q * nt;
+            if (j >= 8 * nc) continue;
+            uint16_t code = kCodeEmpty;
+            if (r[q] >= 0) {
+                const uint32_t rl = (uint32_t)r[q] - kb;
+                code = rl < nc ? (uint16_t)rl : kCodeFar;  // far: not for a local cell
+            } else if (r[q] != kRefEmpty) {
+                code = kCodeBody;
+            }
+            code_s[j] = code;
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = t; c < nc; c += nt)
+        if (depth_s[c] < kMaxDepth) atomicOr(&levels, 1ull << depth_s[c]);
+    __syncthreads();
+    const unsigned long long lv = levels;  // the same for every thread
+    for (int d = kMaxDepth - 1; d >= 0; --d) {
+        if (!((lv >> d) & 1ull)) continue;
+        for (uint32_t c = t; c < nc; c += nt) {
+            if (depth_s[c] != (uint32_t)d) continue;
+            double sx = 0, sy = 0, sz = 0, sm = 0;
+            uint32_t cn = 0;
+            uint32_t pos = first_s[c];
+            // digit order from 0.0, as sum_children
+#pragma unroll
+            for (int dd = 0; dd < 8; ++dd) {
+                const uint16_t code = code_s[8 * c + dd];
+                if (code == kCodeEmpty) continue;
+                double4 v;
+                uint32_t m = 1;
+                if (code < kComLdsCells) {
+                    v = c_s[code];
+                    m = n_s[code];
+                } else if (code == kCodeBody && pos < nbody) {
+                    v = body_term(b_s[pos]);
+                    code_s[8 * c + dd] = (uint16_t)(kCodeAt | pos);
+                } else {  // not for a local cell: the child from global memory
+                    const int32_t r = ref[1ull + 8ull * (kb + c) + dd];
+                    if (r >= 0) {
+                        const InternalSums g = sums[r];
+                        v = make_double4(g.x, g.y, g.z, g.m);
+                        m = counts[r];
+                    } else {
+                        v = body_term(body4[body_of_ref(r)]);
+                    }
+                }
+                sx += v.x;
+                sy += v.y;
+                sz += v.z;
+                sm += v.w;
+                cn += m;
+                pos += m;
+            }
+            c_s[c] = make_double4(sx, sy, sz, sm);
+            n_s[c] = cn;
+            if (sm == 0) raise_flag(h, kFlagZeroMass);
+        }
+        __syncthreads();
+    }
+    // sums, child records and group marks of the local cells
+    for (uint32_t c = t; c < nc; c += nt) {
+        if (depth_s[c] >= kMaxDepth) continue;
+        const double4 g = c_s[c];
+        sums[kb + c] = InternalSums{g.x, g.y, g.z, g.w};
+        counts[kb + c] = n_s[c];
+    }
+    for (uint32_t j = t; j < 8 * nc; j += nt) {
+        const uint32_t c = j >> 3;
+        const uint32_t dc = depth_s[c];
+        if (dc >= kMaxDepth) continue;
+        const uint64_t slot = 1ull + 8ull * kb + j;
+        const uint16_t code = code_s[j];
+        const uint32_t pc = n_s[c];
+        if (code == kCodeEmpty) {
+            put_record(o, slot, kRefEmpty, make_double4(0.0, 0.0, 0.0, 0.0), 0.0);
+        } else if (code < kComLdsCells) {
+            put_record(o, slot, (int32_t)(kb + code), term_com(c_s[code]),
+                       ldexp(root_edge, -(int)(dc + 1)));
+            if (pc > kGroupMax && n_s[code] <= kGroupMax) put_group(o, s + first_s[code], n_s[code]);
+        } else if ((code & kCodeAt) && code < kCodeFar) {
+            const uint32_t pos = code & ~kCodeAt;
+            put_record(o, slot, ref_of_body(i_s[pos]), term_com(body_term(b_s[pos])), 0.0);
+            if (pc > kGroupMax) put_group(o, s + pos, 1);
+        } else {  // not for a local cell: the child and its first position from global memory
+            const int32_t r = ref[slot];
+            if (r >= 0) {
+                const InternalSums g = sums[r];
+                put_record(o, slot, r, term_com(make_double4(g.x, g.y, g.z, g.m)),
+                           ldexp(root_edge, -(int)(dc + 1)));
+                const uint32_t m = counts[r];
+                if (pc > kGroupMax && m <= kGroupMax) put_group(o, int_first[r], m);
+            } else {
+                put_record(o, slot, r, term_com(body_term(body4[body_of_ref(r)])), 0.0);
+                // a body's position: its parent's first plus the counts of its earlier siblings
+                if (pc > kGroupMax) {
+                    uint32_t pos = s + first_s[c];
+                    for (uint32_t q = 8 * c; q < j; ++q) {
+                        const uint16_t cq = code_s[q];
+                        if (cq == kCodeEmpty) continue;
+                        pos += cq < kComLdsCells ? n_s[cq]
+                               : cq == kCodeFar  ? counts[ref[1ull + 8ull * kb + q]]
+                                                 : 1u;
+                    }
+                    put_group(o, pos, 1);
+                }
+            }
+        }
+        if (dc == 0 && (j & 7u) == 0) {  // the root (the block holds every body)
+            put_record(o, 0, (int32_t)(kb + c), term_com(c_s[c]), root_edge);
+            if (pc <= kGroupMax) put_group(o, s + first_s[c], pc);
+        }
+    }
+}
+
+// Pass 2: the cells crossing block boundaries (the table
+// spanning[block * kMaxDepth + depth]), in two launches.  want = 0: one
+// workgroup per super-block of kComSuperBlocks blocks sums the cells that
+// cross block boundaries inside it; want = 2: one workgroup sums the cells
+// crossing super-block boundaries.  A workgroup stages up to kSpanCells cells
+// in LDS: the term of every child (a body, or a cell summed by an earlier
+// pass) is loaded once, all loads in flight together, and its record is
+// written then; a child summed by this workgroup is named by its LDS id
+// (cid[k], written first).  The levels, deepest first, then read LDS only.
+// More cells fall back to the same sums in global memory.
+constexpr uint32_t kSpanCells = 448;
+constexpr uint32_t kChildEmpty = 0xffffffffu;
+constexpr uint32_t kChildHere = 0x80000000u;  // | LDS id of a child summed here
+__global__ void __launch_bounds__(kComThreads) k_com_span(
+    uint32_t nblocks, uint32_t rows, uint32_t want, TreeHdr *h, const double4 *__restrict__ body4,
+    const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning,
+    const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *cid,
+    InternalSums *sums, uint32_t *counts, RecOut o) {
+    __shared__ uint32_t ncell, cnt[kMaxDepth + 1], cur[kMaxDepth];
+    __shared__ uint32_t list[kSpanCells];  // cell index by LDS id
+    __shared__ uint32_t fst[kSpanCells];   // first sorted position
+    __shared__ uint8_t ldep[kSpanCells];
+    __shared__ uint16_t order[kSpanCells];  // LDS ids bucketed by depth
+    __shared__ double4 cv[8 * kSpanCells];  // child terms
+    __shared__ uint32_t cc[8 * kSpanCells]; // child body counts, kChildEmpty, or kChildHere | id
+    __shared__ double4 sv[kSpanCells];
+    __shared__ uint32_t sn[kSpanCells];
+    const uint32_t t = threadIdx.x;
+    const uint32_t r0 = blockIdx.x * rows;
+    const uint32_t r1 = min(nblocks, r0 + rows);
+    const uint64_t slot0 = (uint64_t)r0 * kMaxDepth, slot1 = (uint64_t)r1 * kMaxDepth;
+    const double root_edge = h->edge;
+    if (t == 0) ncell = 0;
     for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += kComThreads) cnt[d] = 0;
     __syncthreads();
-    for (uint32_t c = t; c < nc; c += kComThreads)
-        if (int_local[kb + c])
-            atomicAdd(&cnt[min((uint32_t)int_depth[kb + c], (uint32_t)kMaxDepth - 1)], 1u);
+    for (uint64_t j = slot0 + t; j < slot1; j += kComThreads) {
+        const uint32_t k = spanning[j];
+        if (k == kNoCell || int_local[k] != want) continue;
+        const uint32_t d = (uint32_t)(j % kMaxDepth);
+        const uint32_t id = atomicAdd(&ncell, 1u);
+        if (id < kSpanCells) {
+            list[id] = k;
+            fst[id] = int_first[k];
+            ldep[id] = (uint8_t)d;
+            cid[k] = id;
+            atomicAdd(&cnt[d], 1u);
+        }
+    }
     __syncthreads();
+    const uint32_t nk = ncell;  // the same for every thread
+    if (nk > kSpanCells) {
+        for (int d = kMaxDepth - 1; d >= 0; --d) {
+            for (uint64_t j = slot0 + (uint32_t)d + (uint64_t)t * kMaxDepth; j < slot1;
+                 j += (uint64_t)kComThreads * kMaxDepth) {
+                const uint32_t k = spanning[j];
+                if (k != kNoCell && int_local[k] == want)
+                    sum_children_out(k, (uint32_t)d, int_first[k], root_edge, body4, ref, sums,
+                                     counts, h, o);
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    if (nk == 0) return;
     if (t == 0) {
         uint32_t run = 0;
         for (int d = 0; d < kMaxDepth; ++d) {
@@ -612,76 +967,97 @@ __global__ void __launch_bounds__(kComThreads) k_com_local(
         cnt[kMaxDepth] = run;
     }
     __syncthreads();
-    for (uint32_t c = t; c < nc; c += kComThreads)
-        if (int_local[kb + c])
-            lvl_s[atomicAdd(&cur[min((uint32_t)int_depth[kb + c], (uint32_t)kMaxDepth - 1)], 1u)] = c;
+    for (uint32_t id = t; id < nk; id += kComThreads) order[atomicAdd(&cur[ldep[id]], 1u)] = (uint16_t)id;
+    for (uint32_t p = t; p < 8 * nk; p += kComThreads) {
+        const uint32_t id = p >> 3;
+        const uint32_t k = list[id];
+        const uint64_t j = 1ull + 8ull * k + (p & 7u);
+        const int32_t r = ref[j];
+        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+        uint32_t code = kChildEmpty;
+        if (r >= 0) {
+            if (int_local[r] == want) {
+                code = kChildHere | cid[r];  // summed by this workgroup, one level deeper
+            } else {
+                const InternalSums q = sums[r];
+                v = make_double4(q.x, q.y, q.z, q.m);
+                code = counts[r];
+                put_record(o, j, r, term_com(v), ldexp(root_edge, -(int)(ldep[id] + 1u)));
+            }
+        } else if (r != kRefEmpty) {
+            v = body_term(body4[body_of_ref(r)]);
+            code = 1;
+            put_record(o, j, r, term_com(v), 0.0);
+        } else {
+            put_record(o, j, kRefEmpty, v, 0.0);
+        }
+        cv[p] = v;
+        cc[p] = code;
+    }
     __syncthreads();
     for (int d = kMaxDepth - 1; d >= 0; --d) {
         const uint32_t lb = cnt[d], le = cnt[d + 1];
         if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
         for (uint32_t j = lb + t; j < le; j += kComThreads) {
-            const uint32_t c = lvl_s[j];
-            // the children's data first, then the sums in digit order (as sum_children)
-            int32_t r[8];
-            double4 v[8];
-            uint32_t cc[8];
-#pragma unroll
-            for (int dd = 0; dd < 8; ++dd) r[dd] = ref_s[8 * c + dd];
+            const uint32_t id = order[j];
+            double sx = 0, sy = 0, sz = 0, sm = 0;
+            uint32_t cn = 0;
 #pragma unroll
             for (int dd = 0; dd < 8; ++dd) {
-                v[dd] = make_double4(0.0, 0.0, 0.0, 0.0);
-                cc[dd] = 0;
-                if (r[dd] >= 0) {
-                    const uint32_t rl = (uint32_t)r[dd] - kb;
-                    if (rl < nc) {
-                        v[dd] = make_double4(sx_s[rl], sy_s[rl], sz_s[rl], sm_s[rl]);
-                        cc[dd] = n_s[rl];
-                    } else {  // not in this block: cannot happen for a local cell
-                        const InternalSums q = sums[r[dd]];
-                        v[dd] = make_double4(q.x, q.y, q.z, q.m);
-                        cc[dd] = counts[r[dd]];
-                    }
-                } else if (r[dd] != kRefEmpty) {
-                    v[dd] = body_term(body4[body_of_ref(r[dd])]);
-                    cc[dd] = 1;
+                const uint32_t code = cc[8 * id + dd];
+                if (code == kChildEmpty) continue;
+                double4 v;
+                uint32_t m;
+                if (code & kChildHere) {
+                    v = sv[code & ~kChildHere];
+                    m = sn[code & ~kChildHere];
+                } else {
+                    v = cv[8 * id + dd];
+                    m = code;
                 }
+                sx += v.x;
+                sy += v.y;
+                sz += v.z;
+                sm += v.w;
+                cn += m;
             }
-            InternalSums q;
-            uint32_t cn;
-            add_children(r, v, cc, q, cn);
-            sx_s[c] = q.x;
-            sy_s[c] = q.y;
-            sz_s[c] = q.z;
-            sm_s[c] = q.m;
-            n_s[c] = cn;
-            sums[kb + c] = q;
-            counts[kb + c] = cn;
-            if (q.m == 0) raise_flag(h, kFlagZeroMass);
+            sv[id] = make_double4(sx, sy, sz, sm);
+            sn[id] = cn;
+            if (sm == 0) raise_flag(h, kFlagZeroMass);
         }
         __syncthreads();
     }
-}
-
-// Pass 2: one workgroup sums the cells that cross block boundaries, the
-// table spanning[block * kMaxDepth + depth], deepest depth first.
-__global__ void __launch_bounds__(kComThreads) k_com_spanning(
-    uint32_t nblocks, TreeHdr *h, const double4 *__restrict__ body4,
-    const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning, InternalSums *sums,
-    uint32_t *counts) {
-    __shared__ uint32_t live[kMaxDepth];
-    const uint32_t t = threadIdx.x;
-    if (t < (uint32_t)kMaxDepth) live[t] = 0;
-    __syncthreads();
-    for (uint32_t j = t; j < nblocks * (uint32_t)kMaxDepth; j += kComThreads)
-        if (spanning[j] != kNoCell) live[j % kMaxDepth] = 1;
-    __syncthreads();
-    for (int d = kMaxDepth - 1; d >= 0; --d) {
-        if (!live[d]) continue;  // the same for every thread: LDS, read after a barrier
-        for (uint32_t b = t; b < nblocks; b += kComThreads) {
-            const uint32_t k = spanning[(uint64_t)b * kMaxDepth + d];
-            if (k != kNoCell) sum_children(k, body4, ref, sums, counts, h);
+    // global stores only now (a barrier also waits for them): sums, the
+    // records of the children summed here, group marks, the root
+    for (uint32_t id = t; id < nk; id += kComThreads) {
+        const double4 g = sv[id];
+        sums[list[id]] = InternalSums{g.x, g.y, g.z, g.w};
+        counts[list[id]] = sn[id];
+    }
+    for (uint32_t p = t; p < 8 * nk; p += kComThreads) {
+        const uint32_t id = p >> 3;
+        const uint32_t dd = p & 7u;
+        const uint32_t code = cc[p];
+        const uint32_t pc = sn[id];
+        if (code != kChildEmpty) {
+            const uint32_t m = code & kChildHere ? sn[code & ~kChildHere] : code;
+            if (code & kChildHere)
+                put_record(o, 1ull + 8ull * list[id] + dd, (int32_t)list[code & ~kChildHere],
+                           term_com(sv[code & ~kChildHere]),
+                           ldexp(root_edge, -(int)(ldep[id] + 1u)));
+            if (pc > kGroupMax && m <= kGroupMax) {
+                uint32_t pos = fst[id];
+                for (uint32_t q = 8 * id; q < p; ++q) {
+                    const uint32_t cq = cc[q];
+                    if (cq != kChildEmpty) pos += cq & kChildHere ? sn[cq & ~kChildHere] : cq;
+                }
+                put_group(o, pos, m);
+            }
         }
-        __syncthreads();
+        if (dd == 0 && ldep[id] == 0) {  // the root
+            put_record(o, 0, (int32_t)list[id], term_com(sv[id]), root_edge);
+            if (pc <= kGroupMax) put_group(o, fst[id], pc);
+        }
     }
 }
 
@@ -701,14 +1077,6 @@ __device__ inline void group_mark_body(uint32_t i, uint32_t n, const uint32_t *_
     group_slot[i] = lone ? 1u : 0u;
 }
 
-__global__ void __launch_bounds__(kThreads) k_group_marks_bodies(
-    uint32_t n, const uint32_t *__restrict__ counts, const uint32_t *__restrict__ body_parent,
-    uint32_t *group_size, uint32_t *group_slot) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    group_mark_body(i, n, counts, body_parent, group_size, group_slot);
-}
-
 // pass 2: group cells mark their first sorted position (never a lone body's)
 __device__ inline void group_mark_cell(uint64_t k, uint32_t n, const uint32_t *__restrict__ counts,
                                        const uint32_t *__restrict__ int_parent,
@@ -723,20 +1091,6 @@ __device__ inline void group_mark_cell(uint64_t k, uint32_t n, const uint32_t *_
             group_slot[f] = 1u;
         }
     }
-}
-
-__global__ void __launch_bounds__(kThreads) k_group_marks_cells(uint32_t n, uint32_t icap,
-                                                                const TreeHdr *__restrict__ h,
-                                                                const uint32_t *__restrict__ counts,
-                                                                const uint32_t *__restrict__ int_parent,
-                                                                const uint32_t *__restrict__ int_first,
-                                                                uint32_t *group_size,
-                                                                uint32_t *group_slot) {
-    const uint32_t I = min(h->internal_count, icap);
-    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    const uint64_t t0 = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    for (uint64_t k = t0; k < I; k += stride)
-        group_mark_cell(k, n, counts, int_parent, int_first, group_size, group_slot);
 }
 
 __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
@@ -785,23 +1139,6 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__
         o.lz = (float)(c.z - (double)o.z);
         rec[j] = o;
     }
-}
-
-__global__ void __launch_bounds__(kThreads) k_records(const TreeHdr *__restrict__ h,
-                                                      uint32_t icap,
-                                                      const double4 *__restrict__ body4,
-                                                      const int32_t *__restrict__ ref,
-                                                      const InternalSums *__restrict__ sums,
-                                                      const uint8_t *__restrict__ int_depth,
-                                                      double G, int write_f64, int write_f32,
-                                                      double4 *cm, double *edge, NodeF32 *rec) {
-    const uint32_t I = min(h->internal_count, icap);
-    const uint64_t nodes = 1 + 8ull * I;
-    const double root_edge = h->edge;
-    const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-    for (uint64_t j = (uint64_t)blockIdx.x * kThreads + threadIdx.x; j < nodes; j += stride)
-        record_at(j, root_edge, body4, ref, sums, int_depth, G, write_f64, write_f32, cm, edge,
-                  rec);
 }
 
 // ---- ensemble: one workgroup builds one system's octree ----------------------------
@@ -1104,13 +1441,13 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const KeyView kv{skey, klo};
     for (uint32_t i = t; i < n; i += kForestThreads)
         link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node,
-                body_parent, int_local, nullptr, 0u, int_first, n);
+                body_parent, int_local, nullptr, 0u, int_first, n, n);
     __syncthreads();
     // 7. centre of mass, level by level: every cell is local to this workgroup
     for (uint32_t i = t; i < n; i += kForestThreads)
         if (m[i] == 0) raise_flag(h, kFlagZeroMass);
-    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c, body4, ref, sums,
-               counts, h);
+    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c,
+               [&](uint32_t k) { sum_children(k, body4, ref, sums, counts, h); });
     // 8. records and traversal groups
     const double root_edge = root.e;
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
@@ -1194,37 +1531,33 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s) {
                        t.n, t.hdr);
 }
 
-void launch_com(const TreeBufs &t, hipStream_t s) {
+void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) {
     const uint32_t nblocks = ceil_div(t.n, kComBlock);
-    hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComThreads), 0, s, t.n, t.icap, t.hdr,
+    const RecOut o{G, (int)f64, (int)f32, t.cm, t.edge, t.rec, t.group_size, t.group_slot};
+    hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComLocalThreads), 0, s, t.n, t.icap, t.hdr,
                        (const double4 *)t.body4, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
                        (const uint32_t *)t.chain, (const uint8_t *)t.int_depth,
-                       (const uint8_t *)t.int_local, t.int_lvl, t.int_sums, t.int_count);
-    hipLaunchKernelGGL(k_com_spanning, dim3(1), dim3(kComThreads), 0, s, nblocks, t.hdr,
-                       (const double4 *)t.body4, (const int32_t *)t.ref,
-                       (const uint32_t *)t.spanning, t.int_sums, t.int_count);
-}
-
-void launch_node_records(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) {
-    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(8192, ceil_div(8ull * t.n, kThreads)));
-    hipLaunchKernelGGL(k_records, dim3(blocks), dim3(kThreads), 0, s, (const TreeHdr *)t.hdr,
-                       t.icap, (const double4 *)t.body4, (const int32_t *)t.ref,
-                       (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth, G,
-                       (int)f64, (int)f32, t.cm, t.edge, t.rec);
+                       (const uint8_t *)t.int_local, (const uint32_t *)t.int_first, t.int_lvl,
+                       t.int_sums, t.int_count, o);
+    if (nblocks >= 2) {
+        hipLaunchKernelGGL(k_com_span, dim3(ceil_div(nblocks, kComSuperBlocks)), dim3(kComThreads),
+                           0, s, nblocks, kComSuperBlocks, 0u, t.hdr, (const double4 *)t.body4,
+                           (const int32_t *)t.ref, (const uint32_t *)t.spanning,
+                           (const uint8_t *)t.int_local, (const uint32_t *)t.int_first, t.int_lvl,
+                           t.int_sums, t.int_count, o);
+        if (nblocks > kComSuperBlocks)
+            hipLaunchKernelGGL(k_com_span, dim3(1), dim3(kComThreads), 0, s, nblocks, nblocks, 2u,
+                               t.hdr, (const double4 *)t.body4, (const int32_t *)t.ref,
+                               (const uint32_t *)t.spanning, (const uint8_t *)t.int_local,
+                               (const uint32_t *)t.int_first, t.int_lvl, t.int_sums, t.int_count,
+                               o);
+    }
     if (f32) {
-        const dim3 grid(ceil_div(t.n, kThreads));
-        hipLaunchKernelGGL(k_group_marks_bodies, grid, dim3(kThreads), 0, s, t.n,
-                           (const uint32_t *)t.int_count, (const uint32_t *)t.body_parent,
-                           t.group_size, t.group_slot);
-        const uint32_t gb = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(t.n, kThreads)));
-        hipLaunchKernelGGL(k_group_marks_cells, dim3(gb), dim3(kThreads), 0, s, t.n, t.icap,
-                           (const TreeHdr *)t.hdr, (const uint32_t *)t.int_count,
-                           (const uint32_t *)t.int_parent, (const uint32_t *)t.int_first,
-                           t.group_size, t.group_slot);
+        // the group list: slots by an exclusive scan of the marks, then a scatter
         scan_exclusive_u32(t.group_slot, t.group_slot, t.n, &t.hdr->group_count, t.scan_part,
                            t.scan_part_cap, s);
-        hipLaunchKernelGGL(k_group_scatter, grid, dim3(kThreads), 0, s, t.n,
-                           (const uint32_t *)t.group_size, (const uint32_t *)t.group_slot,
+        hipLaunchKernelGGL(k_group_scatter, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s,
+                           t.n, (const uint32_t *)t.group_size, (const uint32_t *)t.group_slot,
                            t.groups);
     }
 }

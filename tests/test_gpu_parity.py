@@ -158,6 +158,35 @@ def test_tree_sort_more_tiles_than_resident(gpu, oracle):
             t.build(d["x"], d["y"], d["z"], d["m"])
             np.testing.assert_array_equal(sorted_gpu, t.export()["sorted"], err_msg=f"build {step}")
             assert info["node_count"] == t.node_count
+            if step == 0:
+                # centre-of-mass sums over 733 blocks of 2048 bodies and 23
+                # super-blocks of 32 blocks
+                assert_canonical_equal(c.tree_canonical(), t.canonical(), "3M plummer")
+
+
+def test_tree_parity_deep_clusters(gpu, oracle):
+    # 40 clumps of 3000 bodies, 1e-9 across (about 31 levels below the root):
+    # every clump holds a block boundary, so each brings a chain of about 31
+    # cells crossing it.  A super-block of 32 blocks then holds more such
+    # cells than one workgroup sums in LDS (the global-memory path), while
+    # the cells crossing the super-block boundary take the LDS path.
+    rng = np.random.default_rng(21)
+    k, per = 40, 3000
+    centres = rng.uniform(-1, 1, (k, 3))
+    p = (centres[:, None, :] + rng.uniform(-5e-10, 5e-10, (k, per, 3))).reshape(-1, 3)
+    n = k * per
+    d = dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+             vy=np.zeros(n), vz=np.zeros(n), m=rng.uniform(1e20, 1e22, n))
+    with ctx_for(gpu, d) as c:
+        c.bh_build()
+        info = c.tree_info()
+        can = c.tree_canonical()
+        srt = c.tree_sorted()
+    assert info["max_depth"] > 30
+    t = oracle.Tree(n)
+    t.build(d["x"], d["y"], d["z"], d["m"])
+    np.testing.assert_array_equal(srt, t.export()["sorted"])
+    assert_canonical_equal(can, t.canonical(), "deep clusters")
 
 
 @pytest.mark.parametrize("kind,n", [("cube", 1), ("cube", 2), ("cube", 9), ("cube", 1000),
