@@ -310,8 +310,9 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 // must be examined, and lane l examines child (l & 7) of cell l >> 3: one
 // 32-byte record load per lane.  Accepted cells and bodies are appended to an
 // LDS list; every 64 entries the list is evaluated against the group's bodies.
-// A group of s bodies uses 64/s2 lanes per body (s2 = s rounded up to a power
-// of two), each taking every (64/s2)-th list entry, so all 64 lanes stay busy.
+// A group of s bodies uses lpb = floor(64/s) lanes per body, each taking
+// every lpb-th list entry, so s * lpb of the 64 lanes do useful pairs (51 for
+// s = 17, where rounding s up to a power of two would leave 34).
 // Positions are relative to the group centre; the records' hi/lo split keeps
 // the FP32 differences exact to the last FP32 bit.  Waves take groups from an
 // atomic work queue.
@@ -384,6 +385,21 @@ struct GroupTree {
     uint32_t *inter;
 };
 
+// The same for a stride known only at run time (lanes per body not a power
+// of two).
+__device__ __forceinline__ void flush_share_rt(const float4 *p, uint32_t stride, uint32_t t,
+                                               float rx, float ry, float rz, float eps2, float &fx,
+                                               float &fy, float &fz) {
+    uint32_t k = 0;
+    for (; k + 4 <= t; k += 4) {
+        const float4 e[4] = {p[(k + 0) * stride], p[(k + 1) * stride], p[(k + 2) * stride],
+                             p[(k + 3) * stride]};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pair_f32(e[u], rx, ry, rz, eps2, fx, fy, fz);
+    }
+    for (; k < t; ++k) pair_f32(p[k * stride], rx, ry, rz, eps2, fx, fy, fz);
+}
+
 // Pair forces of list entries p[0], p[LPB], p[2 LPB], ... (t entries, in that
 // order) on the NB points (rx, ry, rz)[b].  LPB and the uniform trip count make
 // the LDS offsets immediates and let four reads be in flight ahead of the
@@ -424,8 +440,9 @@ struct GroupLds {
 };
 
 // Forces on the s bodies of one group starting at sorted position g0 (both
-// wave-uniform, checked by the caller).  NB = 1 (s <= 64): s2 = s rounded up
-// to a power of two, and the 64 / s2 lanes of body `sub` share its list.
+// wave-uniform, checked by the caller).  NB = 1 (s <= 64): lane l holds body
+// l % s and list share l / s; the lpb = 64 / s lanes of a body share its list,
+// and the 64 - s * lpb lanes beyond repeat share 0 and their sums are dropped.
 // NB = 2 (64 < s <= 128): a lane holds bodies lane and lane + 64 and takes
 // the whole list for both.  *ncorr_out accumulates network evaluations.
 template <bool kCorr, int NB>
@@ -447,11 +464,12 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
     float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
     uint32_t *cl_k = L.cl_k;
     {
-        const uint32_t s2 = NB == 2 ? (uint32_t)kGroup
-                                    : (s <= 1 ? 1u : 1u << (32 - __clz(s - 1)));
-        const uint32_t sub = lane & (s2 - 1);  // which body of the group
-        const uint32_t q = lane / s2;          // which share of the list
-        const uint32_t lpb = kGroup / s2;      // lanes per body
+        const uint32_t ss = NB == 2 ? (uint32_t)kGroup : s;  // body slots
+        const uint32_t sub = NB == 2 ? lane : lane % ss;      // which body of the group
+        const uint32_t q = NB == 2 ? 0u : lane / ss;          // which share of the list
+        const uint32_t lpb = (uint32_t)kGroup / ss;           // lanes per body
+        const uint32_t qs = q < lpb ? q : 0u;                 // the share this lane computes
+        const bool pow2 = (ss & (ss - 1)) == 0;
         bool valid[NB];
         uint32_t bi[NB];
         double px[NB], py[NB], pz[NB];
@@ -532,31 +550,38 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             }
         }
 
-        // Entries lst[base .. base + cnt) (cnt <= kGroup): lane share q takes
-        // q, q+lpb, ...  The entries are padded with zero-force entries to a
-        // multiple of lpb, so every lane runs the same (uniform) trip count.
+        // Entries lst[base .. base + cnt) (cnt <= kGroup): lane share qs takes
+        // qs, qs + lpb, ...  Every share runs cnt / lpb trips; then shares
+        // qs < cnt % lpb take one more entry and the others a zero-force
+        // entry (a select, so the trip count stays uniform and nothing is
+        // written to the list).  The reads stay below base + cnt + lpb, inside
+        // the list.
         auto flush = [&](uint32_t base, uint32_t cnt) {
-            const uint32_t cp = (cnt + lpb - 1) / lpb * lpb;
-            if (cp > cnt) {
-                if (lane < cp - cnt) lst[base + cnt + lane] = far_entry();
-                __syncthreads();
-            }
-            const uint32_t trips = cp / lpb;
-            const float4 *p = lst + base + q;
+            const uint32_t full = cnt / lpb;
+            const uint32_t rem = cnt - full * lpb;
+            const float4 *p = lst + base + qs;
             float fx[NB], fy[NB], fz[NB];
 #pragma unroll
             for (int b = 0; b < NB; ++b) fx[b] = fy[b] = fz[b] = 0.f;
             if constexpr (NB == 2) {
-                flush_share<1, 2>(p, trips, rx, ry, rz, eps2, fx, fy, fz);
+                flush_share<1, 2>(p, cnt, rx, ry, rz, eps2, fx, fy, fz);  // lpb = 1
             } else {
                 switch (lpb) {
-                case 1: flush_share<1, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                case 2: flush_share<2, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                case 4: flush_share<4, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                case 8: flush_share<8, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                case 16: flush_share<16, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                case 32: flush_share<32, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
-                default: flush_share<64, 1>(p, trips, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 1: flush_share<1, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 2: flush_share<2, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 4: flush_share<4, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 8: flush_share<8, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 16: flush_share<16, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 32: flush_share<32, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 64: flush_share<64, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                default:
+                    flush_share_rt(p, lpb, full, rx[0], ry[0], rz[0], eps2, fx[0], fy[0], fz[0]);
+                    break;
+                }
+                if (rem) {
+                    const float4 e = p[full * lpb];
+                    pair_f32(qs < rem ? e : far_entry(), rx[0], ry[0], rz[0], eps2, fx[0], fy[0],
+                             fz[0]);
                 }
             }
 #pragma unroll
@@ -654,20 +679,24 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
         // Pop up to 8 kPop cells and load this lane's kPop child records
         // (child lane & 7 of the cells lane >> 3, + 8, + 16, ...): the kPop
         // loads of a lane are in flight together, so a walk waits on memory
-        // once per kPop batches of cells instead of once per batch.
+        // once per kPop batches of cells instead of once per batch.  Every
+        // lane loads (a lane without a cell reads a child of cell 0 and is
+        // marked idle): a load under a branch would make the compiler wait
+        // for it at the branch's end, before the list evaluation that is
+        // meant to hide it.
         NodeF32 r[kPop];
         int depth[kPop];
+        bool live[kPop];
         auto pop_and_load = [&]() -> uint32_t {
             const uint32_t take = sp < 8u * kPop ? sp : 8u * kPop;
             sp -= take;
 #pragma unroll
             for (int k = 0; k < kPop; ++k) {
                 const uint32_t c = (lane >> 3) + 8u * (uint32_t)k;
-                const bool act = c < take;
-                const uint32_t entry = act ? stk[sp + c] : 0u;
+                live[k] = c < take;
+                const uint32_t entry = live[k] ? stk[sp + c] : 0u;
                 depth[k] = (int)(entry & 63u) + 1;
-                if (act) r[k] = rec[1ull + 8ull * (entry >> 6) + (lane & 7u)];
-                else r[k].ref = kRefEmpty;
+                r[k] = rec[1ull + 8ull * (entry >> 6) + (lane & 7u)];
             }
             __syncthreads();
             return take;
@@ -689,6 +718,8 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 float d2;
                 bool to_list, to_open;
                 classify(r[k], depth[k], e, d2, to_list, to_open);
+                to_list = to_list && live[k];
+                to_open = to_open && live[k];
                 const uint64_t bl = __ballot(to_list);
                 const uint64_t bo = __ballot(to_open);
                 if (to_list) lst[ln + count_below(bl)] = e;
@@ -749,10 +780,24 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
         }
         __syncthreads();
         // add up the list shares of each body (fixed order: deterministic)
-        for (uint32_t off = s2; off < (uint32_t)kGroup; off <<= 1) {
-            acc_x[0] += __shfl_xor(acc_x[0], off, 64);
-            acc_y[0] += __shfl_xor(acc_y[0], off, 64);
-            acc_z[0] += __shfl_xor(acc_z[0], off, 64);
+        if (pow2) {
+            for (uint32_t off = ss; off < (uint32_t)kGroup; off <<= 1) {
+                acc_x[0] += __shfl_xor(acc_x[0], off, 64);
+                acc_y[0] += __shfl_xor(acc_y[0], off, 64);
+                acc_z[0] += __shfl_xor(acc_z[0], off, 64);
+            }
+        } else {
+            // body sub's shares sit in lanes sub + j ss, j < lpb; lane sub adds them
+            double tx = acc_x[0], ty = acc_y[0], tz = acc_z[0];
+            for (uint32_t j = 1; j < lpb; ++j) {
+                const int src = (int)(sub + j * ss);
+                tx += __shfl(acc_x[0], src, 64);
+                ty += __shfl(acc_y[0], src, 64);
+                tz += __shfl(acc_z[0], src, 64);
+            }
+            acc_x[0] = tx;
+            acc_y[0] = ty;
+            acc_z[0] = tz;
         }
         ncorr_out += ncorr;
         // Stored also after an overflow (the caller reports it and the values
