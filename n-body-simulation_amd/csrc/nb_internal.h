@@ -59,6 +59,14 @@ struct alignas(128) TreeHdr {
 // slower (18 % more interactions from the larger boxes), 32 made it 18 %
 // slower (twice the walks), so 64.
 constexpr uint32_t kGroupMax = 64;
+// Sibling groups of a cell with at most kPackMax bodies are packed into walks
+// of up to kGroupMax bodies: fewer walks for somewhat larger boxes.  Measured
+// (traversal time against one group per cell): 1M-body disk -15 %, 4M-body
+// Plummer sphere -6 % at 128; limits of 96 and 192 were slower.  Packing
+// without a limit made the Plummer traversal ten times slower: siblings around
+// a dense non-group sibling make a box that contains it, and the walk must
+// resolve the dense cell to its leaves.
+constexpr uint32_t kPackMax = 2 * kGroupMax;
 
 struct InternalSums {
     double x, y, z, m;  // centre-of-mass numerators and mass

@@ -571,6 +571,35 @@ __device__ inline void put_group(const RecOut &o, uint32_t p, uint32_t size) {
     }
 }
 
+// Traversal groups among the children of a cell that holds more than
+// kGroupMax bodies; first: the cell's first sorted position, m[d]: child d's
+// body count (0 if empty).  Each child with at most kGroupMax bodies is a
+// group; in a cell of at most kPackMax bodies, runs of such children are
+// packed in digit order into groups of at most kGroupMax bodies instead
+// (siblings are consecutive in the sorted order; a child with more bodies
+// ends a run).
+__device__ inline void put_child_groups(const RecOut &o, uint32_t first, const uint32_t (&m)[8]) {
+    uint32_t total = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) total += m[d];
+    const bool pack = total <= kPackMax;
+    uint32_t pos = first, g0 = first, gs = 0;
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        if (m[d] == 0) continue;
+        if (gs && (!pack || m[d] > kGroupMax || gs + m[d] > kGroupMax)) {
+            put_group(o, g0, gs);
+            gs = 0;
+        }
+        if (m[d] <= kGroupMax) {
+            if (gs == 0) g0 = pos;
+            gs += m[d];
+        }
+        pos += m[d];
+    }
+    if (gs) put_group(o, g0, gs);
+}
+
 // Records and groups of the children of cell pk (depth `depth`, first sorted
 // position `first`), given their refs, terms and body counts; the root's own
 // record and group when pk is the root.
@@ -580,17 +609,14 @@ __device__ inline void cell_outputs(const RecOut &o, uint32_t pk, uint32_t depth
                                     const double4 s, uint32_t cnt) {
     const uint64_t base = 1ull + 8ull * pk;
     const double ce = ldexp(root_edge, -(int)(depth + 1));
-    uint32_t pos = first;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
-        if (r[d] == kRefEmpty) {
+        if (r[d] == kRefEmpty)
             put_record(o, base + d, kRefEmpty, make_double4(0.0, 0.0, 0.0, 0.0), 0.0);
-            continue;
-        }
-        put_record(o, base + d, r[d], term_com(v[d]), r[d] >= 0 ? ce : 0.0);
-        if (cnt > kGroupMax && c[d] <= kGroupMax) put_group(o, pos, c[d]);
-        pos += c[d];
+        else
+            put_record(o, base + d, r[d], term_com(v[d]), r[d] >= 0 ? ce : 0.0);
     }
+    if (cnt > kGroupMax) put_child_groups(o, first, c);
     if (depth == 0) {
         put_record(o, 0, (int32_t)pk, term_com(s), root_edge);
         if (cnt <= kGroupMax) put_group(o, first, cnt);
@@ -837,12 +863,26 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
         }
         __syncthreads();
     }
-    // sums, child records and group marks of the local cells
+    // sums, group marks and child records of the local cells
     for (uint32_t c = t; c < nc; c += nt) {
         if (depth_s[c] >= kMaxDepth) continue;
         const double4 g = c_s[c];
         sums[kb + c] = InternalSums{g.x, g.y, g.z, g.w};
         counts[kb + c] = n_s[c];
+        if (n_s[c] > kGroupMax) {
+            uint32_t m[8];
+#pragma unroll
+            for (int dd = 0; dd < 8; ++dd) {
+                const uint16_t code = code_s[8 * c + dd];
+                m[dd] = code == kCodeEmpty ? 0u : 1u;
+                if (code < kComLdsCells) {
+                    m[dd] = n_s[code];
+                } else if (code == kCodeFar) {  // not for a local cell: global memory
+                    m[dd] = counts[ref[1ull + 8ull * (kb + c) + dd]];
+                }
+            }
+            put_child_groups(o, s + first_s[c], m);
+        }
     }
     for (uint32_t j = t; j < 8 * nc; j += nt) {
         const uint32_t c = j >> 3;
@@ -856,33 +896,17 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
         } else if (code < kComLdsCells) {
             put_record(o, slot, (int32_t)(kb + code), term_com(c_s[code]),
                        ldexp(root_edge, -(int)(dc + 1)));
-            if (pc > kGroupMax && n_s[code] <= kGroupMax) put_group(o, s + first_s[code], n_s[code]);
         } else if ((code & kCodeAt) && code < kCodeFar) {
             const uint32_t pos = code & ~kCodeAt;
             put_record(o, slot, ref_of_body(i_s[pos]), term_com(body_term(b_s[pos])), 0.0);
-            if (pc > kGroupMax) put_group(o, s + pos, 1);
-        } else {  // not for a local cell: the child and its first position from global memory
+        } else {  // not for a local cell: the child from global memory
             const int32_t r = ref[slot];
             if (r >= 0) {
                 const InternalSums g = sums[r];
                 put_record(o, slot, r, term_com(make_double4(g.x, g.y, g.z, g.m)),
                            ldexp(root_edge, -(int)(dc + 1)));
-                const uint32_t m = counts[r];
-                if (pc > kGroupMax && m <= kGroupMax) put_group(o, int_first[r], m);
             } else {
                 put_record(o, slot, r, term_com(body_term(body4[body_of_ref(r)])), 0.0);
-                // a body's position: its parent's first plus the counts of its earlier siblings
-                if (pc > kGroupMax) {
-                    uint32_t pos = s + first_s[c];
-                    for (uint32_t q = 8 * c; q < j; ++q) {
-                        const uint16_t cq = code_s[q];
-                        if (cq == kCodeEmpty) continue;
-                        pos += cq < kComLdsCells ? n_s[cq]
-                               : cq == kCodeFar  ? counts[ref[1ull + 8ull * kb + q]]
-                                                 : 1u;
-                    }
-                    put_group(o, pos, 1);
-                }
             }
         }
         if (dc == 0 && (j & 7u) == 0) {  // the root (the block holds every body)
@@ -1033,27 +1057,24 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
         const double4 g = sv[id];
         sums[list[id]] = InternalSums{g.x, g.y, g.z, g.w};
         counts[list[id]] = sn[id];
+        if (sn[id] > kGroupMax) {
+            uint32_t m[8];
+#pragma unroll
+            for (int dd = 0; dd < 8; ++dd) {
+                const uint32_t code = cc[8 * id + dd];
+                m[dd] = code == kChildEmpty ? 0u : code & kChildHere ? sn[code & ~kChildHere] : code;
+            }
+            put_child_groups(o, fst[id], m);
+        }
     }
     for (uint32_t p = t; p < 8 * nk; p += kComThreads) {
         const uint32_t id = p >> 3;
         const uint32_t dd = p & 7u;
         const uint32_t code = cc[p];
         const uint32_t pc = sn[id];
-        if (code != kChildEmpty) {
-            const uint32_t m = code & kChildHere ? sn[code & ~kChildHere] : code;
-            if (code & kChildHere)
-                put_record(o, 1ull + 8ull * list[id] + dd, (int32_t)list[code & ~kChildHere],
-                           term_com(sv[code & ~kChildHere]),
-                           ldexp(root_edge, -(int)(ldep[id] + 1u)));
-            if (pc > kGroupMax && m <= kGroupMax) {
-                uint32_t pos = fst[id];
-                for (uint32_t q = 8 * id; q < p; ++q) {
-                    const uint32_t cq = cc[q];
-                    if (cq != kChildEmpty) pos += cq & kChildHere ? sn[cq & ~kChildHere] : cq;
-                }
-                put_group(o, pos, m);
-            }
-        }
+        if (code != kChildEmpty && (code & kChildHere))
+            put_record(o, 1ull + 8ull * list[id] + dd, (int32_t)list[code & ~kChildHere],
+                       term_com(sv[code & ~kChildHere]), ldexp(root_edge, -(int)(ldep[id] + 1u)));
         if (dd == 0 && ldep[id] == 0) {  // the root
             put_record(o, 0, (int32_t)list[id], term_com(sv[id]), root_edge);
             if (pc <= kGroupMax) put_group(o, fst[id], pc);
@@ -1067,30 +1088,20 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
 // spatially compact runs of the octree order.  Every sorted position starts
 // at most one group: mark the starts, scan, scatter -- the group list comes
 // out in octree order, so neighbouring groups share most of their walk.
-// pass 1: every sorted position; lone bodies start a group of one
-__device__ inline void group_mark_body(uint32_t i, uint32_t n, const uint32_t *__restrict__ counts,
-                                       const uint32_t *__restrict__ body_parent,
-                                       uint32_t *group_size, uint32_t *group_slot) {
-    const uint32_t pk = body_parent[i];
-    const bool lone = pk == 0xffffffffu ? (n == 1) : counts[pk] > kGroupMax;
-    group_size[i] = lone ? 1u : 0u;
-    group_slot[i] = lone ? 1u : 0u;
-}
-
-// pass 2: group cells mark their first sorted position (never a lone body's)
-__device__ inline void group_mark_cell(uint64_t k, uint32_t n, const uint32_t *__restrict__ counts,
-                                       const uint32_t *__restrict__ int_parent,
-                                       const uint32_t *__restrict__ int_first,
-                                       uint32_t *group_size, uint32_t *group_slot) {
-    const uint32_t c = counts[k];
-    const uint32_t pc = k == 0 ? 0xffffffffu : counts[int_parent[k]];
-    if (c <= kGroupMax && pc > kGroupMax) {
-        const uint32_t f = int_first[k];
-        if (f < n) {
-            group_size[f] = c;
-            group_slot[f] = 1u;
-        }
+// The traversal groups among the children of internal cell k, from global
+// memory (the ensemble build); the same marks as the centre-of-mass kernels.
+__device__ inline void group_mark_children(uint32_t k, const uint32_t *__restrict__ counts,
+                                           const int32_t *__restrict__ ref,
+                                           const uint32_t *__restrict__ int_first,
+                                           const RecOut &o) {
+    if (counts[k] <= kGroupMax) return;
+    uint32_t m[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const int32_t r = ref[1ull + 8ull * k + d];
+        m[d] = r == kRefEmpty ? 0u : r >= 0 ? counts[r] : 1u;
     }
+    put_child_groups(o, int_first[k], m);
 }
 
 __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
@@ -1452,11 +1463,15 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const double root_edge = root.e;
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
         record_at(j, root_edge, body4, ref, sums, int_depth, G, 0, 1, nullptr, nullptr, rec);
-    for (uint32_t i = t; i < n; i += kForestThreads)
-        group_mark_body(i, n, counts, body_parent, group_size, group_slot);
+    for (uint32_t i = t; i < n; i += kForestThreads) {
+        group_size[i] = 0;
+        group_slot[i] = 0;
+    }
     __syncthreads();
+    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot};
     for (uint32_t k = t; k < I; k += kForestThreads)
-        group_mark_cell(k, n, counts, int_parent, int_first, group_size, group_slot);
+        group_mark_children(k, counts, ref, int_first, og);
+    if (t == 0 && (n == 1 || counts[0] <= kGroupMax)) put_group(og, 0, n);  // the root
     __syncthreads();
     const uint32_t ng = wg_scan_array(group_slot, n, wsum);
     for (uint32_t i = t; i < n; i += kForestThreads) {
