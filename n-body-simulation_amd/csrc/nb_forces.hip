@@ -856,8 +856,10 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     __shared__ float4 lst[kListCap];
     __shared__ float4 cl_e[kCorr ? kCellCap : 1];
     __shared__ uint32_t cl_k[kCorr ? kCellCap : 1];
+    // bcorr is cleared lane by lane (lane + 64 b): at least one wave of entries
+    constexpr uint32_t kCorrSlots = kGroupMax > (uint32_t)kGroup ? kGroupMax : (uint32_t)kGroup;
     __shared__ float4 boff[kCorr ? kGroupMax : 1];
-    __shared__ float4 bcorr[kCorr ? kGroupMax : 1];
+    __shared__ float4 bcorr[kCorr ? kCorrSlots : 1];
     const GroupLds L{stk, lst, cl_e, cl_k, boff, bcorr};
     const uint32_t lane = threadIdx.x;
     const uint32_t ngroups = min(h->group_count, n);

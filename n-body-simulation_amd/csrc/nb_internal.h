@@ -53,20 +53,22 @@ struct alignas(128) TreeHdr {
 };
 
 // Bodies per traversal group: a group is an octree cell holding at most
-// kGroupMax bodies whose parent holds more (or a lone body under such a parent).
-// A 64-lane wave walks one group.  The walk also supports up to 128 bodies
-// (two per lane); measured on a 1M-body disk, 128 made the traversal 7 %
-// slower (18 % more interactions from the larger boxes), 32 made it 18 %
-// slower (twice the walks), so 64.
-constexpr uint32_t kGroupMax = 64;
-// Sibling groups of a cell with at most kPackMax bodies are packed into walks
-// of up to kGroupMax bodies: fewer walks for somewhat larger boxes.  Measured
-// (traversal time against one group per cell): 1M-body disk -15 %, 4M-body
-// Plummer sphere -6 % at 128; limits of 96 and 192 were slower.  Packing
-// without a limit made the Plummer traversal ten times slower: siblings around
-// a dense non-group sibling make a box that contains it, and the walk must
-// resolve the dense cell to its leaves.
-constexpr uint32_t kPackMax = 2 * kGroupMax;
+// kGroupMax bodies whose parent holds more (or a lone body under such a
+// parent), and the sibling groups of a cell with at most kPackMax bodies are
+// packed, in digit order, into groups of at most kGroupMax bodies (fewer
+// walks for somewhat larger boxes).  A 64-lane wave walks one group; the
+// walk also supports up to 128 bodies (two per lane).  Measured traversal
+// times (same box, against 64 / 128):
+//  - one group per cell, no packing: 1M-body disk +16 %, 4M-body Plummer
+//    sphere +6 %;
+//  - 64 / 96 and 64 / 192: slower on both; 96 / 192 and 128 / 256: +26 % and
+//    +22 % on the disk;
+//  - 32 / 128 (this): disk -0.4 %, Plummer -2.2 %, with 8 % fewer interactions;
+//  - packing without a limit: the Plummer traversal ten times slower, because
+//    siblings around a dense non-group sibling make a box that contains it and
+//    the walk must resolve that cell to its leaves.
+constexpr uint32_t kGroupMax = 32;
+constexpr uint32_t kPackMax = 4 * kGroupMax;
 
 struct InternalSums {
     double x, y, z, m;  // centre-of-mass numerators and mass
