@@ -193,6 +193,9 @@ int alloc_tree(nb_ctx *c) {
     // at most one per depth (kMaxDepth) and block
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
     if ((rc = dev_alloc(c, &t.spanning, t.spanning_cap))) return rc;
+    // a super-block owns at most one crossing cell per depth (the cells holding its end)
+    t.cross_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComSuper) + 1);
+    if ((rc = dev_alloc(c, &t.cross, t.cross_cap))) return rc;
     if ((rc = dev_alloc(c, &t.ref, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.cm, t.mcap))) return rc;
     if ((rc = dev_alloc(c, &t.edge, t.mcap))) return rc;
@@ -401,7 +404,7 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
                         t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
                         t.int_lvl, t.int_count, t.int_sums, t.int_local,
                         t.int_first, t.groups, t.group_size, t.group_slot, t.body4, t.gq,
-                        t.spanning,
+                        t.spanning, t.cross,
                         t.ref,
                         t.cm, t.edge, t.rec,
                         t.radix_ctl, t.scan_part};

@@ -49,6 +49,7 @@ struct alignas(128) TreeHdr {
     uint32_t stack_peak;                              // deepest group-traversal stack seen
     uint32_t group_count;                              // traversal groups
     uint32_t group_next;                               // traversal work-queue cursor
+    uint32_t cross_count;                              // cells crossing super-block boundaries
     unsigned long long corr_pairs;                     // learned-correction evaluations
 };
 
@@ -137,6 +138,8 @@ struct TreeBufs {
     uint32_t *group_size = nullptr; // per sorted position: size of the group starting there
     uint32_t *group_slot = nullptr; // per sorted position: 1 if a group starts there, then slots
     uint32_t *spanning = nullptr;   // [block * kMaxDepth + depth]: cell crossing the block's end
+    uint32_t *cross = nullptr;      // cells crossing super-block boundaries: k << 6 | depth
+    uint32_t cross_cap = 0;
     double4 *body4 = nullptr;       // {x, y, z, m} per body, input order: one load per body
     uint32_t *gq = nullptr;         // traversal queue counters, one per XCD, 128 bytes apart
     static constexpr uint32_t kGroupQueueWords = 8 * 32;

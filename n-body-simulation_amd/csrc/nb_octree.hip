@@ -59,6 +59,7 @@ __global__ void __launch_bounds__(kThreads) k_build_begin(TreeHdr *h) {
         h->stack_peak = 0;
         h->group_count = 0;
         h->group_next = 0;
+        h->cross_count = 0;
     }
 }
 
@@ -919,8 +920,9 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
 // Pass 2: the cells crossing block boundaries (the table
 // spanning[block * kMaxDepth + depth]), in two launches.  want = 0: one
 // workgroup per super-block of kComSuperBlocks blocks sums the cells that
-// cross block boundaries inside it; want = 2: one workgroup sums the cells
-// crossing super-block boundaries.  A workgroup stages up to kSpanCells cells
+// cross block boundaries inside it and lists those crossing its end (at most
+// one per depth) in `cross`; want = 2: one workgroup sums the listed cells,
+// which cross super-block boundaries.  A workgroup stages up to kSpanCells cells
 // in LDS: the term of every child (a body, or a cell summed by an earlier
 // pass) is loaded once, all loads in flight together, and its record is
 // written then; a child summed by this workgroup is named by its LDS id
@@ -933,8 +935,9 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
     uint32_t nblocks, uint32_t rows, uint32_t want, TreeHdr *h, const double4 *__restrict__ body4,
     const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning,
     const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *cid,
-    InternalSums *sums, uint32_t *counts, RecOut o) {
+    InternalSums *sums, uint32_t *counts, uint32_t *cross, uint32_t cross_cap, RecOut o) {
     __shared__ uint32_t ncell, cnt[kMaxDepth + 1], cur[kMaxDepth];
+    __shared__ uint32_t nx, xbase, xl[kMaxDepth];  // want 0: this super-block's crossing cells
     __shared__ uint32_t list[kSpanCells];  // cell index by LDS id
     __shared__ uint32_t fst[kSpanCells];   // first sorted position
     __shared__ uint8_t ldep[kSpanCells];
@@ -948,13 +951,13 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
     const uint32_t r1 = min(nblocks, r0 + rows);
     const uint64_t slot0 = (uint64_t)r0 * kMaxDepth, slot1 = (uint64_t)r1 * kMaxDepth;
     const double root_edge = h->edge;
-    if (t == 0) ncell = 0;
+    if (t == 0) {
+        ncell = 0;
+        nx = 0;
+    }
     for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += kComThreads) cnt[d] = 0;
     __syncthreads();
-    for (uint64_t j = slot0 + t; j < slot1; j += kComThreads) {
-        const uint32_t k = spanning[j];
-        if (k == kNoCell || int_local[k] != want) continue;
-        const uint32_t d = (uint32_t)(j % kMaxDepth);
+    auto take = [&](uint32_t k, uint32_t d) {
         const uint32_t id = atomicAdd(&ncell, 1u);
         if (id < kSpanCells) {
             list[id] = k;
@@ -963,17 +966,56 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
             cid[k] = id;
             atomicAdd(&cnt[d], 1u);
         }
+    };
+    // want 0 scans this super-block's table rows and keeps its crossing cells
+    // (they all hold the super-block's end: at most one per depth); want 2
+    // reads the list of those cells
+    const uint32_t ncross = want == 2 ? min(h->cross_count, cross_cap) : 0u;
+    if (want == 2) {
+        for (uint32_t i = t; i < ncross; i += kComThreads) take(cross[i] >> 6, cross[i] & 63u);
+    } else {
+        for (uint64_t j = slot0 + t; j < slot1; j += kComThreads) {
+            const uint32_t k = spanning[j];
+            if (k == kNoCell) continue;
+            const uint8_t kind = int_local[k];
+            const uint32_t d = (uint32_t)(j % kMaxDepth);
+            if (kind == 2) {
+                const uint32_t x = atomicAdd(&nx, 1u);
+                if (x < (uint32_t)kMaxDepth) xl[x] = (k << 6) | d;
+            } else if (kind == want) {
+                take(k, d);
+            }
+        }
     }
     __syncthreads();
+    if (want != 2) {
+        const uint32_t m = min(nx, (uint32_t)kMaxDepth);
+        if (t == 0) xbase = m ? atomicAdd(&h->cross_count, m) : 0u;
+        __syncthreads();
+        if (t < m) {
+            if (xbase + t < cross_cap) cross[xbase + t] = xl[t];
+            else raise_flag(h, kFlagNodeOverflow);
+        }
+        if (t == 0 && nx > (uint32_t)kMaxDepth) raise_flag(h, kFlagNodeOverflow);
+    }
     const uint32_t nk = ncell;  // the same for every thread
     if (nk > kSpanCells) {
         for (int d = kMaxDepth - 1; d >= 0; --d) {
-            for (uint64_t j = slot0 + (uint32_t)d + (uint64_t)t * kMaxDepth; j < slot1;
-                 j += (uint64_t)kComThreads * kMaxDepth) {
-                const uint32_t k = spanning[j];
-                if (k != kNoCell && int_local[k] == want)
-                    sum_children_out(k, (uint32_t)d, int_first[k], root_edge, body4, ref, sums,
-                                     counts, h, o);
+            if (want == 2) {
+                for (uint32_t i = t; i < ncross; i += kComThreads)
+                    if ((cross[i] & 63u) == (uint32_t)d) {
+                        const uint32_t k = cross[i] >> 6;
+                        sum_children_out(k, (uint32_t)d, int_first[k], root_edge, body4, ref,
+                                         sums, counts, h, o);
+                    }
+            } else {
+                for (uint64_t j = slot0 + (uint32_t)d + (uint64_t)t * kMaxDepth; j < slot1;
+                     j += (uint64_t)kComThreads * kMaxDepth) {
+                    const uint32_t k = spanning[j];
+                    if (k != kNoCell && int_local[k] == want)
+                        sum_children_out(k, (uint32_t)d, int_first[k], root_edge, body4, ref,
+                                         sums, counts, h, o);
+                }
             }
             __syncthreads();
         }
@@ -1369,6 +1411,7 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
         h->stack_peak = 0;
         h->group_count = 0;
         h->group_next = 0;
+        h->cross_count = 0;
         h->corr_pairs = 0;
         lcp_max = -1;
     }
@@ -1559,13 +1602,13 @@ void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) 
                            0, s, nblocks, kComSuperBlocks, 0u, t.hdr, (const double4 *)t.body4,
                            (const int32_t *)t.ref, (const uint32_t *)t.spanning,
                            (const uint8_t *)t.int_local, (const uint32_t *)t.int_first, t.int_lvl,
-                           t.int_sums, t.int_count, o);
+                           t.int_sums, t.int_count, t.cross, t.cross_cap, o);
         if (nblocks > kComSuperBlocks)
             hipLaunchKernelGGL(k_com_span, dim3(1), dim3(kComThreads), 0, s, nblocks, nblocks, 2u,
                                t.hdr, (const double4 *)t.body4, (const int32_t *)t.ref,
                                (const uint32_t *)t.spanning, (const uint8_t *)t.int_local,
                                (const uint32_t *)t.int_first, t.int_lvl, t.int_sums, t.int_count,
-                               o);
+                               t.cross, t.cross_cap, o);
     }
     if (f32) {
         // the group list: slots by an exclusive scan of the marks, then a scatter
