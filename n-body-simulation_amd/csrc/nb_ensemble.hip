@@ -86,7 +86,8 @@ void mark(nb_ensemble *e, int phase, bool begin) {
         const size_t idx = e->marks.size() * 2;
         while (e->events.size() < idx + 2) {
             hipEvent_t ev;
-            if (hipEventCreate(&ev) != hipSuccess) return;
+            // timing only, read after a stream synchronisation: no system-scope fence
+            if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) return;
             e->events.push_back(ev);
         }
         e->marks.push_back({phase, idx});
@@ -487,8 +488,8 @@ int nb_ensemble_autotune(nb_ensemble *e, uint32_t steps, nb_ensemble_tuning *bes
         enqueue_forces(e);  // warm-up: t = 0 forces of this configuration
         if ((rc = finish(e))) return rc;
         hipEvent_t a, b;
-        ENS_HIP(e, hipEventCreate(&a));
-        ENS_HIP(e, hipEventCreate(&b));
+        ENS_HIP(e, hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+        ENS_HIP(e, hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
         ENS_HIP(e, hipEventRecord(a, e->stream));
         enqueue_steps(e, steps);
         ENS_HIP(e, hipEventRecord(b, e->stream));
