@@ -469,6 +469,9 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
         const uint32_t q = NB == 2 ? 0u : lane / ss;          // which share of the list
         const uint32_t lpb = (uint32_t)kGroup / ss;           // lanes per body
         const uint32_t qs = q < lpb ? q : 0u;                 // the share this lane computes
+        // list entries per flush: a multiple of lpb (60 to 64), so that only
+        // the group's last flush has a remainder trip
+        const uint32_t chunk = lpb * ((uint32_t)kGroup / lpb);
         const bool pow2 = (ss & (ss - 1)) == 0;
         bool valid[NB];
         uint32_t bi[NB];
@@ -738,7 +741,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                     }
                     cn += __popcll(bc);  // < kCorrTile before: at most kCellCap - 1 after
                 }
-                ln += __popcll(bl);  // < kGroup before: at most kListCap - 1 after
+                ln += __popcll(bl);  // < chunk <= kGroup before: at most kListCap - 1 after
                 sp += __popcll(bo);
             }
             peak = sp > peak ? sp : peak;
@@ -749,11 +752,11 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             __syncthreads();
             // issue the next loads; the list evaluation below hides their latency
             take = sp > 0 ? pop_and_load() : 0u;
-            if (ln >= kGroup) {
+            if (ln >= chunk) {
                 uint32_t done = 0;
-                for (; ln - done >= (uint32_t)kGroup; done += kGroup) flush(done, kGroup);
+                for (; ln - done >= chunk; done += chunk) flush(done, chunk);
                 __syncthreads();
-                const uint32_t rest = ln - done;  // < kGroup
+                const uint32_t rest = ln - done;  // < chunk <= done: no overlap
                 if (lane < rest) lst[lane] = lst[done + lane];
                 ln = rest;
                 __syncthreads();
