@@ -433,6 +433,7 @@ __device__ inline float4 far_entry() { return make_float4(1e30f, 0.f, 0.f, 0.f);
 struct GroupLds {
     uint32_t *stk;
     float4 *lst;
+    float *thr;      // opening threshold by depth, (edge / theta)^2 = ro2_root * 4^-depth
     float4 *cl_e;    // correction: accepted cell offset from the group centre, G*M
     uint32_t *cl_k;  // correction: accepted cell's internal index
     float4 *boff;    // correction: body offsets from the group centre
@@ -462,6 +463,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
     uint32_t *inter = T.inter;
     uint32_t *stk = L.stk;
     float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
+    float *thr = L.thr;
     uint32_t *cl_k = L.cl_k;
     {
         const uint32_t ss = NB == 2 ? (uint32_t)kGroup : s;  // body slots
@@ -646,12 +648,16 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             const float ey = fmaxf(fabsf(dy) - hy, 0.f);
             const float ez = fmaxf(fabsf(dz) - hz, 0.f);
             d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-            const bool far_enough = d2 > ldexpf(ro2_root, -2 * depth);
+            const bool far_enough = d2 > thr[depth];
             const bool cell = r.ref >= 0;
             to_list = r.ref != kRefEmpty && (!cell || far_enough);
             to_open = cell && !far_enough;
         };
 
+        // this tree's thresholds by depth (depths stay below 64), the same
+        // values as ldexpf at every test, one LDS read instead of three VALU
+        thr[lane] = ldexpf(ro2_root, -2 * (int)lane);
+        __syncthreads();
         {
             float4 e;
             float d2r;
@@ -863,7 +869,8 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     constexpr uint32_t kCorrSlots = kGroupMax > (uint32_t)kGroup ? kGroupMax : (uint32_t)kGroup;
     __shared__ float4 boff[kCorr ? kGroupMax : 1];
     __shared__ float4 bcorr[kCorr ? kCorrSlots : 1];
-    const GroupLds L{stk, lst, cl_e, cl_k, boff, bcorr};
+    __shared__ float thr[kGroup];
+    const GroupLds L{stk, lst, thr, cl_e, cl_k, boff, bcorr};
     const uint32_t lane = threadIdx.x;
     const uint32_t ngroups = min(h->group_count, n);
     // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
@@ -975,7 +982,8 @@ __device__ inline void direct_tile(const double *__restrict__ x, const double *_
 __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
-    const GroupLds L{stk, lst, nullptr, nullptr, nullptr, nullptr};
+    __shared__ float thr[kGroup];
+    const GroupLds L{stk, lst, thr, nullptr, nullptr, nullptr, nullptr};
     const uint32_t lane = threadIdx.x;
     const uint32_t total = a.item_off[a.nq];
     const MlpForward w{};
