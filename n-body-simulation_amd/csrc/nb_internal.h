@@ -193,12 +193,13 @@ struct ForestBufs {
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total_dev,
                         uint32_t *scratch, uint32_t scratch_cap, hipStream_t s);
 uint32_t scan_scratch_words(uint32_t n);
-// Stable LSD radix sort of (key, val) pairs on key bits [0, bits), bits <= 64,
-// n < 2^30.  Result in keys/vals (the alt buffers are scratch).  ctl holds
+// Stable LSD radix sort of (key, val) pairs on key bits [0, bits), bits <= 63,
+// n < 2^30.  Returns true if the result is in keys_alt/vals_alt (an odd pass
+// count), false if in keys/vals.  ctl holds
 // radix_ctl_words(n) words, zeroed at allocation and then owned by the sort
 // (one ctl per stream: sorts sharing it must not overlap); a failed look-back
 // wait raises kFlagSortWait in *flags.
-void radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+bool radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                         uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s);
 uint32_t radix_blocks_for(uint32_t n);
 uint32_t radix_ctl_words(uint32_t n);
@@ -207,7 +208,9 @@ uint32_t radix_ctl_words(uint32_t n);
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
                  hipStream_t s);
 // keys, sort, and t.body4 = {x, y, z, m} of every body (input order)
-void launch_keys(const TreeBufs &t, const double *x, const double *y, const double *z,
+// (the sorted keys and body indices end in t.skey/t.sidx: the buffers swap
+// with their scratch copies when the sort ends there)
+void launch_keys(TreeBufs &t, const double *x, const double *y, const double *z,
                  const double *m, hipStream_t s);
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,
                      hipStream_t s);

@@ -1552,12 +1552,15 @@ void launch_aabb(const TreeBufs &t, const double *x, const double *y, const doub
                        t.hdr);
 }
 
-void launch_keys(const TreeBufs &t, const double *x, const double *y, const double *z,
+void launch_keys(TreeBufs &t, const double *x, const double *y, const double *z,
                  const double *m, hipStream_t s) {
     hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, m,
                        t.n, (const TreeHdr *)t.hdr, t.skey, t.sidx, t.body4);
-    radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels, t.radix_ctl,
-                       &t.hdr->status, s);
+    if (radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels,
+                           t.radix_ctl, &t.hdr->status, s)) {
+        std::swap(t.skey, t.skey_alt);
+        std::swap(t.sidx, t.sidx_alt);
+    }
 }
 
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,

@@ -2,7 +2,7 @@
 //
 // Both are written for 64-lane wavefronts: prefix sums use __shfl_up inside a
 // wave and LDS across the 4 waves of a 256-thread block; the radix sort ranks
-// keys stably with an 8-ballot "match any" on the 8-bit digit, so no
+// keys stably with a 9-ballot "match any" on the 9-bit digit, so no
 // per-thread histogram is needed, and takes one launch per digit (plus one
 // counting launch per sort).  The octree build uses the scan for node
 // numbering and the sort for the octant keys (rocPRIM/hipCUB are not used).
@@ -122,12 +122,16 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t *in, uint
 // on the order) come from k_radix_count for pass 0 and from pass p - 1 for
 // pass p, which counts the next digit of the keys it has loaded anyway.
 //
+// Digits are 9 bits wide: 7 passes cover the 63-bit octree keys (an odd
+// count, so the result lands in the scratch buffers and the caller swaps).
+//
 // Control words (u32):
-//   [0, 2048)              digit totals of pass p at p * 256
+//   [0, 7 * 512)           digit totals of pass p at p * kBins
 //   kCtlTiles + p          tile counter of pass p
-//   kCtlStatus + plane     two status planes of nb x 256 words, plane p & 1
+//   kCtlStatus + plane     two status planes of nb x kBins words, plane p & 1
 //                          for pass p; pass p clears its tile's row of the
-//                          other plane for pass p + 1
+//                          other plane for pass p + 1, and k_radix_count
+//                          clears plane 0 (an odd pass count leaves it used)
 // Pass p > 0 zeroes the totals and the tile counter of pass p - 1 (that
 // launch has ended), and pass 0 those of the last pass.  Between sorts every
 // word is therefore zero except the last pass's totals and tile counter,
@@ -136,9 +140,13 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const uint32_t *in, uint
 // kStatAgg | tile count, or kStatPrefix | count in tiles [0, t].  Counts stay
 // below 2^30: octree contexts hold fewer than 2^29 bodies.
 
-constexpr int kMaxPasses = 8;
-constexpr uint32_t kCtlTiles = kMaxPasses * 256;
-constexpr uint32_t kCtlStatus = kCtlTiles + 256;  // planes start 256-byte aligned
+constexpr int kDigitBits = 9;
+constexpr uint32_t kBins = 1u << kDigitBits;
+constexpr int kMaxPasses = 7;
+static_assert(kMaxPasses * kDigitBits >= 63, "the passes cover the keys");
+static_assert(kBins == (uint32_t)kPassThreads, "thread tid owns digit tid");
+constexpr uint32_t kCtlTiles = kMaxPasses * kBins;
+constexpr uint32_t kCtlStatus = kCtlTiles + 64;  // planes start 256-byte aligned
 constexpr uint32_t kStatAgg = 1u << 30;
 constexpr uint32_t kStatPrefix = 2u << 30;
 constexpr uint32_t kStatCount = kStatAgg - 1;
@@ -155,31 +163,36 @@ __device__ inline void status_store(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Totals of the lowest 8-bit digit of the keys: total[digit].
+// Totals of the lowest digit of the keys: total[digit].  Also clears the
+// status plane of the first pass, plane0[0, plane_words): the previous sort's
+// last pass used it when the pass count is odd.
 __global__ void __launch_bounds__(kCountThreads) k_radix_count(const uint64_t *__restrict__ keys,
                                                                uint32_t n,
-                                                               uint32_t *__restrict__ total) {
-    __shared__ uint32_t h[256];
-    if (threadIdx.x < 256) h[threadIdx.x] = 0;
+                                                               uint32_t *__restrict__ total,
+                                                               uint32_t *__restrict__ plane0,
+                                                               uint64_t plane_words) {
+    __shared__ uint32_t h[kBins];
+    if (threadIdx.x < kBins) h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t stride = (uint64_t)gridDim.x * kCountThreads;
+    for (uint64_t i = (uint64_t)blockIdx.x * kCountThreads + threadIdx.x; i < plane_words; i += stride)
+        plane0[i] = 0;
 #pragma unroll 4
     for (uint64_t i = (uint64_t)blockIdx.x * kCountThreads + threadIdx.x; i < n; i += stride)
-        atomicAdd(&h[(uint32_t)keys[i] & 255u], 1u);
+        atomicAdd(&h[(uint32_t)keys[i] & (kBins - 1)], 1u);
     __syncthreads();
-    if (threadIdx.x < 256 && h[threadIdx.x]) atomicAdd(&total[threadIdx.x], h[threadIdx.x]);
+    if (threadIdx.x < kBins && h[threadIdx.x]) atomicAdd(&total[threadIdx.x], h[threadIdx.x]);
 }
 
-// Exclusive prefix over threads [0, 256) of one value per thread (thread tid
-// owns digit tid).  Every thread of the block calls it (threads from 256 on
-// pass 0 and ignore the result); ws holds 4 words and is not reused.
+// Exclusive prefix over the block's threads of one value per thread (thread
+// tid owns digit tid); ws holds kPassWaves words and is not reused.
 __device__ inline uint32_t digit_exclusive_scan(uint32_t c, uint32_t *ws) {
     const uint32_t inc = wave_inclusive_scan(c);
     const uint32_t w = threadIdx.x >> 6;
-    if (w < 4 && lane_id() == 63) ws[w] = inc;
+    if (lane_id() == 63) ws[w] = inc;
     __syncthreads();
     uint32_t ex = inc - c;
-    for (uint32_t k = 0; k < w && k < 4; ++k) ex += ws[k];
+    for (uint32_t k = 0; k < w; ++k) ex += ws[k];
     return ex;
 }
 
@@ -194,7 +207,7 @@ __device__ inline bool look_back(const uint32_t *stat, uint32_t t, uint32_t d, u
         uint32_t v[kLookWindow];
 #pragma unroll
         for (int k = 0; k < kLookWindow; ++k)
-            v[k] = j - k >= 0 ? status_load(stat + (uint64_t)(j - k) * 256u + d) : kStatPrefix;
+            v[k] = j - k >= 0 ? status_load(stat + (uint64_t)(j - k) * kBins + d) : kStatPrefix;
         bool pending = false;
 #pragma unroll
         for (int k = 0; k < kLookWindow; ++k) pending |= v[k] == 0;
@@ -204,7 +217,7 @@ __device__ inline bool look_back(const uint32_t *stat, uint32_t t, uint32_t d, u
             pending = false;
 #pragma unroll
             for (int k = 0; k < kLookWindow; ++k) {
-                if (v[k] == 0) v[k] = status_load(stat + (uint64_t)(j - k) * 256u + d);
+                if (v[k] == 0) v[k] = status_load(stat + (uint64_t)(j - k) * kBins + d);
                 pending |= v[k] == 0;
             }
         }
@@ -224,7 +237,7 @@ __device__ inline bool look_back(const uint32_t *stat, uint32_t t, uint32_t d, u
     }
 }
 
-// One stable pass over digit (key >> 8 pass) & 255 of a tile of kRadixTile
+// One stable pass over digit (key >> 9 pass) & 511 of a tile of kRadixTile
 // keys.  Wave w holds the tile's keys [w * 512, (w + 1) * 512) and ranks them
 // 64 at a time against its own digit counters, with no barrier between
 // rounds: a key's place among its digit's keys in the tile is (same digit in
@@ -236,33 +249,33 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals, uint32_t n, int pass, int passes,
     uint32_t *__restrict__ ctl, uint32_t nb, uint32_t *__restrict__ flags) {
-    __shared__ uint32_t wcnt[kPassWaves][256];  // digit-d keys of wave w, then their first place
-    __shared__ uint32_t next_hist[256];          // tile histogram of the next pass's digit
-    __shared__ uint32_t dbase[256];              // first tile position of digit d
-    __shared__ uint32_t gbase[256];              // output position of the tile's first digit-d key
-    __shared__ uint32_t wsum[2][4];
+    __shared__ uint32_t wcnt[kPassWaves][kBins];  // digit-d keys of wave w, then their first place
+    __shared__ uint32_t next_hist[kBins];          // tile histogram of the next pass's digit
+    __shared__ uint32_t dbase[kBins];              // first tile position of digit d
+    __shared__ uint32_t gbase[kBins];              // output position of the tile's first digit-d key
+    __shared__ uint32_t wsum[2][kPassWaves];
     __shared__ uint64_t lk[kRadixTile];
     __shared__ uint32_t lv[kRadixTile];
     __shared__ uint32_t tile_id, stuck;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
-    const bool owner = tid < 256;  // thread tid owns digit tid
-    const int shift = 8 * pass;
-    uint32_t *total = ctl + pass * 256;
-    uint32_t *stat = ctl + kCtlStatus + (uint64_t)(pass & 1) * 256u * nb;
-    uint32_t *stat_next = ctl + kCtlStatus + (uint64_t)((pass + 1) & 1) * 256u * nb;
+    const bool owner = tid < kBins;  // thread tid owns digit tid
+    const int shift = kDigitBits * pass;
+    uint32_t *total = ctl + pass * kBins;
+    uint32_t *stat = ctl + kCtlStatus + (uint64_t)(pass & 1) * kBins * nb;
+    uint32_t *stat_next = ctl + kCtlStatus + (uint64_t)((pass + 1) & 1) * kBins * nb;
     const bool has_next = pass + 1 < passes;
     if (tid == 0) {
         tile_id = atomicAdd(ctl + kCtlTiles + pass, 1u);
         stuck = 0;
     }
-    for (uint32_t i = tid; i < kPassWaves * 256u; i += kPassThreads) (&wcnt[0][0])[i] = 0;
+    for (uint32_t i = tid; i < kPassWaves * kBins; i += kPassThreads) (&wcnt[0][0])[i] = 0;
     if (owner) next_hist[tid] = 0;
     __syncthreads();
     const uint32_t t = tile_id;
     if (t == 0 && owner) {
         // the previous pass (or the last pass of the previous sort) has ended
         const int prev = pass > 0 ? pass - 1 : passes - 1;
-        ctl[prev * 256 + tid] = 0;
+        ctl[prev * kBins + tid] = 0;
         if (tid == 0) ctl[kCtlTiles + prev] = 0;
     }
     const uint64_t tile = (uint64_t)t * kRadixTile;
@@ -281,11 +294,11 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
 #pragma unroll
     for (int r = 0; r < kPassItems; ++r) {
         const bool valid = wbase + (uint32_t)r * 64u + lane < tn;
-        const uint32_t d = (uint32_t)(kk[r] >> shift) & 255u;
+        const uint32_t d = (uint32_t)(kk[r] >> shift) & (kBins - 1);
         dg[r] = d;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < kDigitBits; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bal = __ballot(bit);
             peers &= bit ? bal : ~bal;
@@ -295,7 +308,8 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
         const uint32_t before = wcnt[w][d];
         rk[r] = before + rank;
         if (valid && rank == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
-        if (valid && has_next) atomicAdd(&next_hist[(uint32_t)(kk[r] >> (shift + 8)) & 255u], 1u);
+        if (valid && has_next)
+            atomicAdd(&next_hist[(uint32_t)(kk[r] >> (shift + kDigitBits)) & (kBins - 1)], 1u);
     }
     __syncthreads();
     uint32_t c = 0, gtot = 0;
@@ -307,9 +321,9 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
             wcnt[k][tid] = c;
             c += cnt;
         }
-        status_store(stat + (uint64_t)t * 256u + tid, (t == 0 ? kStatPrefix : kStatAgg) | c);
-        stat_next[(uint64_t)t * 256u + tid] = 0;
-        if (has_next && next_hist[tid]) atomicAdd(total + 256 + tid, next_hist[tid]);
+        status_store(stat + (uint64_t)t * kBins + tid, (t == 0 ? kStatPrefix : kStatAgg) | c);
+        stat_next[(uint64_t)t * kBins + tid] = 0;
+        if (has_next && next_hist[tid]) atomicAdd(total + kBins + tid, next_hist[tid]);
         gtot = total[tid];
     }
     const uint32_t ex = digit_exclusive_scan(c, wsum[0]);
@@ -333,7 +347,7 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
         if (t > 0) {
             if (!look_back(stat, t, tid, &excl)) stuck = 1;
             // published even when stuck, so that later tiles do not wait as well
-            status_store(stat + (uint64_t)t * 256u + tid, kStatPrefix | (excl + c));
+            status_store(stat + (uint64_t)t * kBins + tid, kStatPrefix | (excl + c));
         }
         gbase[tid] = goff + excl;
     }
@@ -344,7 +358,7 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     }
     for (uint32_t j = tid; j < tn; j += kPassThreads) {
         const uint64_t key = lk[j];
-        const uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const uint32_t d = (uint32_t)(key >> shift) & (kBins - 1);
         const uint32_t pos = gbase[d] + (j - dbase[d]);
         if (pos < n) {
             okeys[pos] = key;
@@ -370,16 +384,15 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
 
 uint32_t radix_blocks_for(uint32_t n) { return ceil_div(n ? n : 1, kRadixTile); }
 
-uint32_t radix_ctl_words(uint32_t n) { return kCtlStatus + 2u * 256u * radix_blocks_for(n); }
+uint32_t radix_ctl_words(uint32_t n) { return kCtlStatus + 2u * kBins * radix_blocks_for(n); }
 
-void radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
+bool radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
                         uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s) {
-    if (n <= 1 || bits <= 0 || bits > 8 * kMaxPasses || n >= kStatAgg) return;
+    if (n <= 1 || bits <= 0 || bits > kDigitBits * kMaxPasses || n >= kStatAgg) return false;
     const uint32_t nb = radix_blocks_for(n);
-    int passes = std::max((bits + 7) / 8, kMinPasses);
-    if (passes & 1) ++passes;  // even pass count: the result lands back in keys/vals
+    const int passes = std::max((bits + kDigitBits - 1) / kDigitBits, kMinPasses);
     hipLaunchKernelGGL(k_radix_count, dim3(std::min(nb, kCountBlocks)), dim3(kCountThreads), 0, s,
-                       (const uint64_t *)keys, n, ctl);
+                       (const uint64_t *)keys, n, ctl, ctl + kCtlStatus, (uint64_t)kBins * nb);
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int p = 0; p < passes; ++p) {
@@ -388,6 +401,7 @@ void radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint
         uint64_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }
+    return (passes & 1) != 0;  // an odd count ends in the scratch buffers
 }
 
 }  // namespace nb
