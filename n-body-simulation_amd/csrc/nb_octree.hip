@@ -322,18 +322,26 @@ __global__ void __launch_bounds__(kThreads) k_max_depth(const int32_t *__restric
 }
 
 // ---- 6. topology --------------------------------------------------------------
+// The sorted keys as the topology pass reads them.  `win` (optional) holds
+// skey[w0, w1) in LDS: the searches of one block stay mostly inside it.
 struct KeyView {
     const uint64_t *skey;
     const uint64_t *klo;
+    const uint64_t *win = nullptr;
+    uint32_t w0 = 0, w1 = 0;
+    __device__ inline uint64_t key(uint32_t p) const {
+        return p - w0 < w1 - w0 ? win[p - w0] : skey[p];
+    }
     __device__ inline uint32_t digit(uint32_t p, int l) const {
-        return l < kKeyLevels ? (uint32_t)(skey[p] >> (3 * (kKeyLevels - 1 - l))) & 7u
+        return l < kKeyLevels ? (uint32_t)(key(p) >> (3 * (kKeyLevels - 1 - l))) & 7u
                               : (uint32_t)(klo[p] >> (3 * (2 * kKeyLevels - 1 - l))) & 7u;
     }
     // do sorted bodies p and q share their first v digits?
     __device__ inline bool same(uint32_t p, uint32_t q, int v) const {
         if (v <= 0) return true;
-        if (v <= kKeyLevels) return ((skey[p] ^ skey[q]) >> (63 - 3 * v)) == 0;
-        return skey[p] == skey[q] && ((klo[p] ^ klo[q]) >> (63 - 3 * (v - kKeyLevels))) == 0;
+        const uint64_t kp = key(p), kq = key(q);
+        if (v <= kKeyLevels) return ((kp ^ kq) >> (63 - 3 * v)) == 0;
+        return kp == kq && ((klo[p] ^ klo[q]) >> (63 - 3 * (v - kKeyLevels))) == 0;
     }
     // first sorted position of the depth-v cell that holds sorted body i
     __device__ inline uint32_t cell_start(uint32_t i, int v) const {
@@ -449,6 +457,7 @@ __device__ inline void link_at(uint32_t i, uint32_t n, uint32_t icap, TreeHdr *h
     // the deepest leaf, 1 + max lcp, is taken from the lcp array (max_depth_of)
 }
 
+constexpr uint32_t kLinkBack = 768;  // keys before a link block's own kept in LDS
 __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, TreeHdr *h,
                                                    KeyView kv, const uint32_t *__restrict__ sidx,
                                                    const int32_t *__restrict__ lcp,
@@ -458,7 +467,17 @@ __global__ void __launch_bounds__(kThreads) k_link(uint32_t n, uint32_t icap, Tr
                                                    uint32_t *body_parent, uint8_t *int_local,
                                                    uint32_t *spanning, uint32_t spanning_cap,
                                                    uint32_t *int_first) {
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    // the block's keys and kLinkBack before them in LDS
+    __shared__ uint64_t win[kLinkBack + kThreads];
+    const uint32_t b0 = blockIdx.x * kThreads;
+    const uint32_t w0 = b0 > kLinkBack ? b0 - kLinkBack : 0u;
+    const uint32_t w1 = min(n, b0 + (uint32_t)kThreads);
+    for (uint32_t p = w0 + threadIdx.x; p < w1; p += kThreads) win[p - w0] = kv.skey[p];
+    __syncthreads();
+    kv.win = win;
+    kv.w0 = w0;
+    kv.w1 = w1;
+    const uint32_t i = b0 + threadIdx.x;
     if (i >= n) return;
     link_at(i, n, icap, h, kv, sidx, lcp, off, ref, int_depth, int_parent, int_node, body_parent,
             int_local, spanning, spanning_cap, int_first, kComBlock, kComSuper);
