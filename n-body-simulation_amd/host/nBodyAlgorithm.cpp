@@ -7,6 +7,8 @@
 #include <iostream>
 #include <stdexcept>
 
+#include "Parallel.hpp"
+
 void nbCheck(nb_ctx *ctx, int rc, const char *what) {
     if (rc == NB_OK) return;
     std::string msg = std::string(what) + " failed: " + nb_strerror(rc);
@@ -203,6 +205,32 @@ void nBodyAlgorithm::adjustVelocities(const SimulationData &simulationData) {
     }
 }
 
+std::vector<paraview::StoredStep> nBodyAlgorithm::storedSteps() const {
+    std::vector<paraview::StoredStep> steps;
+    auto energy = [](const std::map<d_type::int_t, double> &values, d_type::int_t i) {
+        if (!configuration::compute_energy) return 0.0;
+        auto it = values.find(i);
+        return it != values.end() ? it->second : 0.0;
+    };
+    for (d_type::int_t i = 0; i < (d_type::int_t)positions_x.size(); ++i) {
+        paraview::StoredStep s;
+        s.n = positions_x.at(i).size();
+        s.px = positions_x.at(i).data();
+        s.py = positions_y.at(i).data();
+        s.pz = positions_z.at(i).data();
+        s.vx = velocities_x.at(i).data();
+        s.vy = velocities_y.at(i).data();
+        s.vz = velocities_z.at(i).data();
+        s.acc = acceleration.at(i).data();
+        s.kinetic = energy(kineticEnergy, i);
+        s.potential = energy(potentialEnergy, i);
+        s.total = energy(totalEnergy, i);
+        s.virial = energy(virialEquilibrium, i);
+        steps.push_back(s);
+    }
+    return steps;
+}
+
 void nBodyAlgorithm::generateParaViewOutput(const SimulationData &simulationData) {
     std::time_t currentTime = std::time(nullptr);
     std::string time = static_cast<std::string>(std::ctime(&currentTime));
@@ -212,165 +240,16 @@ void nBodyAlgorithm::generateParaViewOutput(const SimulationData &simulationData
     std::filesystem::create_directories(filePathBase);
 
     timer.exportJSON(filePathBase + "times.json");
-    outputLastState(filePathBase + "lastState.csv");
-
-    std::ofstream pvdFile(filePathBase + "/simulation" + ".pvd");
-    pvdFile << "<?xml version=\"1.0\"?>" << '\n'
-            << R"(<VTKFile type="Collection" version="0.1" byte_order="LittleEndian" compressor="vtkZLibDataCompressor">)"
-            << '\n'
-            << "<Collection>" << '\n';
-    for (d_type::int_t i = 0; i < positions_x.size(); ++i) {
-        const std::string vtpFileName = "simulation_step" + std::to_string(i) + ".vtp";
-        std::ofstream vtpFile(filePathBase + vtpFileName);
-        pvdFile << "<DataSet timestep=\"" << i << "\" " << R"(group="" part="0" file=")"
-                << vtpFileName << "\"/>" << '\n';
-        writeFileHeader(i, vtpFile);
-        writePositions(i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << "</Points>" << '\n'
-                << "<PointData>" << '\n'
-                << R"(<DataArray type="Int32" Name="body_id" NumberOfComponents="1" format="ascii">)"
-                << '\n';
-        writeIDs(i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="velocity" NumberOfComponents="3" format="ascii">)"
-                << '\n';
-        writeVelocities(i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="acceleration" NumberOfComponents="1" format="ascii">)"
-                << '\n';
-        writeAccelerations(i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="mass" NumberOfComponents="1" format="ascii">)"
-                << '\n';
-        writeMasses(simulationData, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="String" Name="name" NumberOfComponents="1" format="ascii">)"
-                << '\n';
-        writeNamesASCII(simulationData, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Int32" Name="orbit_class" NumberOfComponents="1" format="ascii">)"
-                << '\n';
-        writeBodyClasses(simulationData, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << "</PointData>" << '\n'
-                << "<Verts>" << '\n'
-                << R"(<DataArray type="Int64" Name="offsets">)" << '\n';
-        writeOffsets(i, vtpFile);
-        vtpFile << '\n'
-                << "</DataArray>" << '\n'
-                << R"(<DataArray type="Int64" Name="connectivity">)" << '\n';
-        writeConnectivity(i, vtpFile);
-        vtpFile << '\n'
-                << "</DataArray>" << '\n'
-                << "</Verts>" << '\n'
-                << "</Piece>" << '\n'
-                << "<FieldData>" << '\n'
-                << R"(<DataArray type="Float64" Name="kinetic energy" NumberOfTuples="1" format="ascii">)"
-                << '\n';
-        writeFieldValue(kineticEnergy, i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="potential energy" NumberOfTuples="1" format="ascii">)"
-                << '\n';
-        writeFieldValue(potentialEnergy, i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="total energy" NumberOfTuples="1" format="ascii">)"
-                << '\n';
-        writeFieldValue(totalEnergy, i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << R"(<DataArray type="Float64" Name="virial equilibrium" NumberOfTuples="1" format="ascii">)"
-                << '\n';
-        writeFieldValue(virialEquilibrium, i, vtpFile);
-        vtpFile << "</DataArray>" << '\n'
-                << "</FieldData>" << '\n'
-                << "</PolyData>" << '\n'
-                << "</VTKFile>" << '\n';
-    }
-    pvdFile << "</Collection>" << '\n' << "</VTKFile>" << '\n';
-}
-
-void nBodyAlgorithm::writeFieldValue(const std::map<d_type::int_t, double> &values, size_t i,
-                                     std::ofstream &vtpFile) {
-    if (configuration::compute_energy) {
-        auto it = values.find((d_type::int_t)i);
-        vtpFile << (it != values.end() ? it->second : 0.0) << '\n';
-    } else {
-        vtpFile << 0 << '\n';
-    }
-}
-
-void nBodyAlgorithm::writeConnectivity(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << std::to_string(j) << ' ';
-}
-
-void nBodyAlgorithm::writeOffsets(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 1; j <= positions_x[i].size(); ++j) vtpFile << std::to_string(j) << ' ';
-}
-
-void nBodyAlgorithm::writeBodyClasses(const SimulationData &simulationData,
-                                      std::ofstream &vtpFile) {
-    // object classes of https://pdssbn.astro.umd.edu/data_other/objclass.shtml
-    // plus stars, dwarf planets, planets and moons (nBodyAlgorithm.cpp:298-342)
-    static const std::map<std::string, int> code = {
-        {"AMO", 1}, {"APO", 2},  {"ATE", 3},  {"IEO", 4},  {"MCA", 5},  {"IMB", 6},
-        {"MBA", 7}, {"OMB", 8},  {"CEN", 9},  {"TJN", 10}, {"TNO", 11}, {"AST", 12},
-        {"PAA", 13}, {"HYA", 14}, {"STA", 15}, {"DWA", 16}, {"PLA", 17}, {"SAT", 18}};
-    for (const std::string &bodyClass : simulationData.body_classes) {
-        auto it = code.find(bodyClass);
-        vtpFile << (it != code.end() ? it->second : 0) << '\n';
-    }
-}
-
-void nBodyAlgorithm::writeNamesASCII(const SimulationData &simulationData,
-                                     std::ofstream &vtpFile) const {
-    for (const std::string &name : simulationData.names) {
-        for (char c : name) vtpFile << (int)c << ' ';
-        vtpFile << " 0" << '\n';
-    }
-}
-
-void nBodyAlgorithm::writeMasses(const SimulationData &simulationData,
-                                 std::ofstream &vtpFile) const {
-    for (double m : simulationData.mass) vtpFile << m << '\n';
-}
-
-void nBodyAlgorithm::writeAccelerations(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << acceleration[i][j] << '\n';
-}
-
-void nBodyAlgorithm::writeVelocities(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 0; j < velocities_x[i].size(); ++j)
-        vtpFile << velocities_x[i].at(j) << " " << velocities_y[i].at(j) << " "
-                << velocities_z[i].at(j) << '\n';
-}
-
-void nBodyAlgorithm::writeIDs(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j) vtpFile << j << '\n';
-}
-
-void nBodyAlgorithm::writePositions(size_t i, std::ofstream &vtpFile) {
-    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j)
-        vtpFile << positions_x[i].at(j) << " " << positions_y[i].at(j) << " "
-                << positions_z[i].at(j) << '\n';
-}
-
-void nBodyAlgorithm::writeFileHeader(size_t i, std::ofstream &vtpFile) {
-    vtpFile << "<?xml version=\"1.0\"?>" << '\n'
-            << R"(<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian" header_type="UInt64">)"
-            << '\n'
-            << "<PolyData>" << '\n'
-            << "<Piece NumberOfPoints=\"" << positions_x[i].size() << "\" NumberOfVerts=\""
-            << positions_x[i].size() << "\">" << '\n'
-            << "<Points>" << '\n'
-            << R"(<DataArray type="Float64" Name="position" NumberOfComponents="3" format="ascii">)"
-            << '\n';
+    // lastState.csv, simulation.pvd and one .vtp per stored step, formatted
+    // on the host's worker threads
+    paraview::writeSeries(filePathBase, storedSteps(), simulationData,
+                          host_parallel::defaultThreads());
 }
 
 void nBodyAlgorithm::outputLastState(const std::string &path) {
-    std::ofstream csvFile(path);
-    const d_type::int_t i = (d_type::int_t)positions_x.size() - 1;
-    csvFile << "position_x, position_y, position_z \n";
-    for (d_type::int_t j = 0; j < positions_x[i].size(); ++j)
-        csvFile << positions_x[i].at(j) << "," << positions_y[i].at(j) << ","
-                << positions_z[i].at(j) << '\n';
+    const std::vector<paraview::StoredStep> steps = storedSteps();
+    if (steps.empty()) throw std::runtime_error("outputLastState: no stored step");
+    std::ofstream csvFile(path, std::ios::binary);
+    csvFile << paraview::formatLastState(steps.back(), host_parallel::defaultThreads());
+    if (!csvFile) throw std::runtime_error("write to " + path + " failed");
 }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "Configuration.hpp"
+#include "ParaViewWriter.hpp"
 #include "SimulationData.hpp"
 #include "TimeMeasurement.hpp"
 #include "nbody_capi.h"
@@ -51,7 +52,7 @@ class nBodyAlgorithm {
     virtual void startSimulation(const SimulationData &simulationData) = 0;
 
     // times.json, lastState.csv, simulation.pvd and one .vtp per stored step
-    // (nBodyAlgorithm.cpp:129-251)
+    // (nBodyAlgorithm.cpp:129-251, 286-411)
     void generateParaViewOutput(const SimulationData &simulationData);
     void outputLastState(const std::string &path);
 
@@ -63,6 +64,8 @@ class nBodyAlgorithm {
     // subtracts the centre-of-mass velocity from the stored step-0 velocities
     // only (nBodyAlgorithm.cpp:104-127)
     void adjustVelocities(const SimulationData &simulationData);
+    // views of the stored steps in step order, for the output writers
+    std::vector<paraview::StoredStep> storedSteps() const;
 
    protected:
     // The reference's time loop (NaiveAlgorithm.cpp:15-260,
@@ -71,18 +74,6 @@ class nBodyAlgorithm {
     void runSimulation(const SimulationData &simulationData, int algorithm);
 
    private:
-    void writeFileHeader(size_t i, std::ofstream &vtpFile);
-    void writePositions(size_t i, std::ofstream &vtpFile);
-    void writeIDs(size_t i, std::ofstream &vtpFile);
-    void writeAccelerations(size_t i, std::ofstream &vtpFile);
-    void writeVelocities(size_t i, std::ofstream &vtpFile);
-    void writeMasses(const SimulationData &simulationData, std::ofstream &vtpFile) const;
-    void writeNamesASCII(const SimulationData &simulationData, std::ofstream &vtpFile) const;
-    void writeBodyClasses(const SimulationData &simulationData, std::ofstream &vtpFile);
-    void writeOffsets(size_t i, std::ofstream &vtpFile);
-    void writeConnectivity(size_t i, std::ofstream &vtpFile);
-    void writeFieldValue(const std::map<d_type::int_t, double> &values, size_t i,
-                         std::ofstream &vtpFile);
     void recordStepTimes(nb_ctx *ctx, int algorithm, bool loop);
 };
 
