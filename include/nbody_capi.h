@@ -96,6 +96,23 @@ int nb_set_state(nb_ctx *ctx, const double *x, const double *y, const double *z,
 int nb_get_state(nb_ctx *ctx, double *x, double *y, double *z, double *vx, double *vy,
                  double *vz);
 int nb_get_accel(nb_ctx *ctx, double *ax, double *ay, double *az);
+/* Asynchronous copy-out of a visualised step, replacing the synchronous
+ * snapshot reads of NaiveAlgorithm.cpp:170-181, :227-243 (and the
+ * acceleration read of storeAccelerations, nBodyAlgorithm.cpp:88-102).
+ * nb_snapshot_begin enqueues, behind the work already issued on the context,
+ * a device-side copy of x, y, z, vx, vy, vz, ax, ay, az into a staging buffer
+ * and a DMA of it into pinned host memory on a second stream, and returns
+ * without waiting: the next nb_steps runs while the DMA is in flight.
+ * nb_snapshot_wait blocks until the DMA of the last begin is done and stores
+ * nine pointers (x, y, z, vx, vy, vz, ax, ay, az; n doubles each) into
+ * `arrays`.  They point into one of two pinned host slots that successive
+ * begins use in turn, hold the state as it was at that begin, and stay valid
+ * until the second nb_snapshot_begin after it (or nb_destroy): a caller may
+ * read one snapshot on its own threads while the next one is in flight.
+ * Waiting with no snapshot begun is NB_E_INVALID_ARG.  Like every call on a
+ * context, begin and wait are made from one thread at a time. */
+int nb_snapshot_begin(nb_ctx *ctx);
+int nb_snapshot_wait(nb_ctx *ctx, const double **arrays);
 /* Overwrite positions only (used by tests to evaluate forces on given points). */
 int nb_set_positions(nb_ctx *ctx, const double *x, const double *y, const double *z);
 

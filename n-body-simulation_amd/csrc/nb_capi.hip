@@ -44,6 +44,14 @@ struct nb_ctx {
     double *quad = nullptr;          // 6 per internal cell
     nb::MlpCell *mcell = nullptr;    // per internal cell
     bool quad_valid = false;         // quadrupoles match the current octree
+    // asynchronous snapshots: device staging (9 n doubles), two pinned host
+    // slots used in turn, the DMA stream and the event that ends each DMA
+    double *snap_dev = nullptr;
+    double *snap_host[2] = {nullptr, nullptr};
+    int snap_slot = 1;  // slot of the last begin
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_staged = nullptr, snap_done = nullptr;
+    bool snap_begun = false;
     std::vector<hipEvent_t> events;
     std::vector<std::pair<int, size_t>> marks;  // (phase, index of start event)
     double phase[NB_PHASE_COUNT] = {0};
@@ -387,7 +395,13 @@ int nb_create(nb_ctx **out, uint32_t n, const nb_params *p) {
 int nb_destroy(nb_ctx *c) {
     if (!c) return NB_E_INVALID_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
     for (void *p : c->allocations) (void)hipFree(p);
+    for (double *h : c->snap_host)
+        if (h) (void)hipHostFree(h);
+    if (c->snap_staged) (void)hipEventDestroy(c->snap_staged);
+    if (c->snap_done) (void)hipEventDestroy(c->snap_done);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
     if (c->hdr_host) (void)hipHostFree(c->hdr_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -484,6 +498,48 @@ int nb_get_accel(nb_ctx *c, double *ax, double *ay, double *az) {
     for (int k = 0; k < 3; ++k)
         if (dst[k]) NB_HIP(c, hipMemcpyAsync(dst[k], src[k], bytes, hipMemcpyDeviceToHost, c->stream));
     NB_HIP(c, hipStreamSynchronize(c->stream));
+    return NB_OK;
+}
+
+int nb_snapshot_begin(nb_ctx *c) {
+    if (!c) return NB_E_INVALID_ARG;
+    const size_t n = c->n, bytes = n * sizeof(double);
+    if (!c->snap_done) {
+        // first use; a failure part-way leaves what was made for the next try
+        if (!c->snap_dev) {
+            int rc = dev_alloc(c, &c->snap_dev, 9ull * n);
+            if (rc) return rc;
+        }
+        for (double *&h : c->snap_host)
+            if (!h) NB_HIP(c, hipHostMalloc((void **)&h, 9 * bytes, hipHostMallocDefault));
+        if (!c->copy_stream)
+            NB_HIP(c, hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+        if (!c->snap_staged)
+            NB_HIP(c, hipEventCreateWithFlags(&c->snap_staged, hipEventDisableTiming));
+        NB_HIP(c, hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+    } else {
+        // the staging buffer is reused: the previous DMA must have read it
+        NB_HIP(c, hipStreamWaitEvent(c->stream, c->snap_done, 0));
+    }
+    const double *src[9] = {c->x, c->y, c->z, c->vx, c->vy, c->vz, c->ax, c->ay, c->az};
+    for (int k = 0; k < 9; ++k)
+        NB_HIP(c, hipMemcpyAsync(c->snap_dev + k * n, src[k], bytes, hipMemcpyDeviceToDevice,
+                                 c->stream));
+    NB_HIP(c, hipEventRecord(c->snap_staged, c->stream));
+    NB_HIP(c, hipStreamWaitEvent(c->copy_stream, c->snap_staged, 0));
+    c->snap_slot ^= 1;
+    NB_HIP(c, hipMemcpyAsync(c->snap_host[c->snap_slot], c->snap_dev, 9 * bytes,
+                             hipMemcpyDeviceToHost, c->copy_stream));
+    NB_HIP(c, hipEventRecord(c->snap_done, c->copy_stream));
+    c->snap_begun = true;
+    return NB_OK;
+}
+
+int nb_snapshot_wait(nb_ctx *c, const double **arrays) {
+    if (!c || !arrays) return NB_E_INVALID_ARG;
+    if (!c->snap_begun) return fail(c, NB_E_INVALID_ARG, "nb_snapshot_wait: no snapshot begun");
+    NB_HIP(c, hipStreamSynchronize(c->copy_stream));
+    for (int k = 0; k < 9; ++k) arrays[k] = c->snap_host[c->snap_slot] + (size_t)k * c->n;
     return NB_OK;
 }
 

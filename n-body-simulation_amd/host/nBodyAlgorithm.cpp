@@ -1,6 +1,9 @@
 #include "nBodyAlgorithm.hpp"
 
 #include <algorithm>
+#include <array>
+#include <exception>
+#include <thread>
 #include <cmath>
 #include <ctime>
 #include <filesystem>
@@ -35,6 +38,37 @@ double phase(nb_ctx *ctx, int p) {
     nbCheck(ctx, nb_phase_ms(ctx, ms), "nb_phase_ms");
     return ms[p];
 }
+
+// One host thread that stores a snapshot; joined before the next one starts
+// and, at the latest, before the context (and its pinned slots) is destroyed.
+class SnapshotCopier {
+   public:
+    template <class F>
+    void start(F f) {
+        thread_ = std::thread([this, f] {
+            try {
+                f();
+            } catch (...) {
+                error_ = std::current_exception();
+            }
+        });
+    }
+    void join() {
+        if (thread_.joinable()) thread_.join();
+        if (error_) {
+            std::exception_ptr e = error_;
+            error_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+    ~SnapshotCopier() {
+        if (thread_.joinable()) thread_.join();
+    }
+
+   private:
+    std::thread thread_;
+    std::exception_ptr error_;
+};
 
 struct CtxGuard {
     nb_ctx *ctx = nullptr;
@@ -143,25 +177,34 @@ void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int alg
     timeSinceLastVisualization += dt;
     currentStep += 1;
 
-    std::vector<double> px(N), py(N), pz(N), vx(N), vy(N), vz(N);
+    // A visualised step's state is copied out asynchronously: the DMA runs
+    // while the GPU computes the next step, after which a host thread stores
+    // it into the time series during the step after that.  The library keeps
+    // two pinned slots, so the thread must be done before the second begin
+    // after the snapshot it reads.
+    SnapshotCopier copier;
+    d_type::int_t pendingSnapshot = 0;
+    bool snapshotPending = false;
+    auto takeSnapshot = [&] {
+        std::array<const double *, 9> a;
+        nbCheck(ctx, nb_snapshot_wait(ctx, a.data()), "nb_snapshot_wait");
+        copier.join();
+        const d_type::int_t step = pendingSnapshot;
+        copier.start([this, step, a] { storeSnapshot(step, a); });
+        snapshotPending = false;
+    };
     while (time <= t_end + 0.000001) {
         const bool visualizeCurrentStep =
             (std::abs(timeSinceLastVisualization - visualizationStepWidth) < 0.000001);
         // kick-drift, [octree], forces, kick -- one synchronisation per step
         nbCheck(ctx, nb_steps(ctx, algorithm, dt, 1), "nb_steps");
         recordStepTimes(ctx, algorithm, true);
+        if (snapshotPending) takeSnapshot();
         if (visualizeCurrentStep) {
             std::cout << "Finished step " << currentStep << std::endl << std::endl;
-            nbCheck(ctx, nb_get_state(ctx, px.data(), py.data(), pz.data(), vx.data(), vy.data(), vz.data()),
-                    "nb_get_state");
-            positions_x[currentStep] = px;
-            positions_y[currentStep] = py;
-            positions_z[currentStep] = pz;
-            nbCheck(ctx, nb_get_accel(ctx, ax.data(), ay.data(), az.data()), "nb_get_accel");
-            storeAccelerations(currentStep, ax, ay, az);
-            velocities_x[currentStep] = vx;
-            velocities_y[currentStep] = vy;
-            velocities_z[currentStep] = vz;
+            nbCheck(ctx, nb_snapshot_begin(ctx), "nb_snapshot_begin");
+            pendingSnapshot = currentStep;
+            snapshotPending = true;
             if (configuration::compute_energy) computeEnergy(ctx, currentStep);
             currentStep += 1;
             timeSinceLastVisualization = 0.0;
@@ -169,6 +212,8 @@ void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int alg
         time += dt;
         timeSinceLastVisualization += dt;
     }
+    if (snapshotPending) takeSnapshot();
+    copier.join();
 }
 
 void nBodyAlgorithm::computeEnergy(nb_ctx *ctx, d_type::int_t currentStep) {
@@ -183,10 +228,30 @@ void nBodyAlgorithm::computeEnergy(nb_ctx *ctx, d_type::int_t currentStep) {
 void nBodyAlgorithm::storeAccelerations(d_type::int_t currentStep, const std::vector<double> &ax,
                                         const std::vector<double> &ay,
                                         const std::vector<double> &az) {
+    storeAccelerations(currentStep, ax.size(), ax.data(), ay.data(), az.data());
+}
+
+void nBodyAlgorithm::storeAccelerations(d_type::int_t currentStep, size_t n, const double *ax,
+                                        const double *ay, const double *az) {
     std::vector<double> &out = acceleration[currentStep];
-    out.resize(ax.size());
-    for (size_t i = 0; i < ax.size(); ++i)
-        out[i] = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+    out.resize(n);
+    for (size_t i = 0; i < n; ++i) out[i] = std::sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+}
+
+void nBodyAlgorithm::storeSnapshot(d_type::int_t step, const std::array<const double *, 9> &a) {
+    const size_t n = configuration::numberOfBodies;
+    std::vector<double> *dst[6] = {&positions_x[step],  &positions_y[step],  &positions_z[step],
+                                   &velocities_x[step], &velocities_y[step], &velocities_z[step]};
+    std::vector<double> &acc = acceleration[step];
+    host_parallel::parallelFor(7, host_parallel::defaultThreads(), [&](size_t k) {
+        if (k < 6) {
+            dst[k]->assign(a[k], a[k] + n);
+        } else {
+            acc.resize(n);
+            for (size_t i = 0; i < n; ++i)
+                acc[i] = std::sqrt(a[6][i] * a[6][i] + a[7][i] * a[7][i] + a[8][i] * a[8][i]);
+        }
+    });
 }
 
 void nBodyAlgorithm::adjustVelocities(const SimulationData &simulationData) {

@@ -8,6 +8,7 @@
 #ifndef NBODY_HOST_NBODYALGORITHM_HPP
 #define NBODY_HOST_NBODYALGORITHM_HPP
 
+#include <array>
 #include <fstream>
 #include <map>
 #include <string>
@@ -61,6 +62,11 @@ class nBodyAlgorithm {
     // |a| per body, stored for `currentStep` (nBodyAlgorithm.cpp:88-102)
     void storeAccelerations(d_type::int_t currentStep, const std::vector<double> &ax,
                             const std::vector<double> &ay, const std::vector<double> &az);
+    void storeAccelerations(d_type::int_t currentStep, size_t n, const double *ax,
+                            const double *ay, const double *az);
+    // stores positions, velocities and |a| of visualised step `step` from the
+    // nine arrays of a snapshot (x, y, z, vx, vy, vz, ax, ay, az)
+    void storeSnapshot(d_type::int_t step, const std::array<const double *, 9> &a);
     // subtracts the centre-of-mass velocity from the stored step-0 velocities
     // only (nBodyAlgorithm.cpp:104-127)
     void adjustVelocities(const SimulationData &simulationData);

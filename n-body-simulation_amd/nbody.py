@@ -35,7 +35,7 @@ PHASES = ["AABB creation", "keys and sort", "topology", "Compute center of mass"
 EXPORTS = [
     "nb_default_params", "nb_create", "nb_destroy", "nb_set_params", "nb_get_params",
     "nb_body_count", "nb_last_error", "nb_strerror", "nb_device_name", "nb_set_state",
-    "nb_get_state", "nb_get_accel", "nb_set_positions", "nb_forces_direct",
+    "nb_get_state", "nb_get_accel", "nb_snapshot_begin", "nb_snapshot_wait", "nb_set_positions", "nb_forces_direct",
     "nb_forces_direct_targets", "nb_bh_build", "nb_bh_forces", "nb_kick_drift", "nb_kick",
     "nb_steps", "nb_phase_ms", "nb_tree_info", "nb_tree_sorted", "nb_tree_canonical",
     "nb_bh_interactions", "nb_gen_star_disk", "nb_gen_plummer", "nb_energy",
@@ -97,6 +97,8 @@ def load(path=LIB_PATH):
         "nb_set_state": (ctypes.c_int, [vp] + [_dp] * 7),
         "nb_get_state": (ctypes.c_int, [vp] + [_dp] * 6),
         "nb_get_accel": (ctypes.c_int, [vp] + [_dp] * 3),
+        "nb_snapshot_begin": (ctypes.c_int, [vp]),
+        "nb_snapshot_wait": (ctypes.c_int, [vp, ctypes.POINTER(_dp)]),
         "nb_set_positions": (ctypes.c_int, [vp] + [_dp] * 3),
         "nb_forces_direct": (ctypes.c_int, [vp]),
         "nb_forces_direct_targets": (ctypes.c_int, [vp, ctypes.c_uint32, _u32p] + [_dp] * 3),
@@ -247,6 +249,18 @@ class Context:
         out = [np.empty(self.n) for _ in range(3)]
         self._check(self.lib.nb_get_accel(self.h, *map(_d, out)), "nb_get_accel")
         return tuple(out)
+
+    def snapshot_begin(self):
+        """Start the asynchronous copy-out of x, v and a (returns at once)."""
+        self._check(self.lib.nb_snapshot_begin(self.h), "nb_snapshot_begin")
+
+    def snapshot_wait(self):
+        """Wait for the last snapshot; returns copies of its nine arrays."""
+        ptrs = (_dp * 9)()
+        self._check(self.lib.nb_snapshot_wait(self.h, ptrs), "nb_snapshot_wait")
+        keys = ["x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az"]
+        return {k: np.ctypeslib.as_array(ptrs[i], shape=(self.n,)).copy()
+                for i, k in enumerate(keys)}
 
     def forces_direct(self):
         self._check(self.lib.nb_forces_direct(self.h), "nb_forces_direct")

@@ -296,6 +296,50 @@ def test_simulation_loop_bitexact(gpu, oracle, algo):
         np.testing.assert_array_equal(bits(got), bits(r[k]), err_msg=k)
 
 
+@pytest.mark.parametrize("algo", ["naive", "BarnesHut"])
+def test_snapshot_async_matches_sync_reads(gpu, oracle, algo):
+    # nb_snapshot_begin/wait return the state as it was at begin, although
+    # further steps overwrite it while the DMA is in flight
+    n = 5000
+    d = make(gpu, "disk", n)
+    dt, theta = 1.0 / 24, 0.5
+    keys = ["x", "y", "z", "vx", "vy", "vz"]
+    with ctx_for(gpu, d, theta=theta) as c:
+        with pytest.raises(gpu.NBodyError):
+            c.snapshot_wait()  # nothing begun yet
+        if algo == "naive":
+            c.forces_direct()
+        else:
+            c.bh_build()
+            c.bh_forces()
+        for rep in range(3):
+            c.steps(algo, dt, 2)
+            want, acc = c.state(), c.accel()
+            c.snapshot_begin()
+            c.steps(algo, dt, 3)
+            got = c.snapshot_wait()
+            for k in keys:
+                np.testing.assert_array_equal(bits(got[k]), bits(want[k]), err_msg=f"{rep} {k}")
+            for k, a in zip(("ax", "ay", "az"), acc):
+                np.testing.assert_array_equal(bits(got[k]), bits(a), err_msg=f"{rep} {k}")
+            if rep == 0:
+                r = oracle.simulate(algo, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"],
+                                    d["m"], dt, 2 * dt, theta=theta)
+                for k in keys + ["ax", "ay", "az"]:
+                    np.testing.assert_array_equal(bits(got[k]), bits(r[k]), err_msg=k)
+        # a second begin before any wait: the wait returns the second state
+        c.snapshot_begin()
+        c.steps(algo, dt, 1)
+        want = c.state()
+        c.snapshot_begin()
+        c.steps(algo, dt, 1)
+        got = c.snapshot_wait()
+        again = c.snapshot_wait()  # waiting twice returns the same snapshot
+        for k in keys:
+            np.testing.assert_array_equal(bits(got[k]), bits(want[k]), err_msg=k)
+            np.testing.assert_array_equal(bits(again[k]), bits(want[k]), err_msg=k)
+
+
 # ---- error handling ----------------------------------------------------------------------------
 def test_coincident_bodies_reported(gpu):
     d = uniform_cube(100, 2)
