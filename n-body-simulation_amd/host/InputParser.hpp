@@ -11,9 +11,18 @@
 
 class InputParser {
    public:
-    // Throws std::invalid_argument on an unreadable file or a malformed
-    // number (the reference's std::stod throws the same exception type).
+    // Throws std::invalid_argument on an unreadable file, a line with fewer
+    // than 10 fields or a field with no leading number, and std::out_of_range
+    // on a number std::stod cannot represent: the exceptions of the
+    // reference's std::stod, thrown for the first bad line in file order.
+    // The file is parsed in line-aligned pieces on the host's worker threads.
     static void parse_input(std::string &path, SimulationData &simulationData);
+
+    // parse_input on the text of a file (header line included), cut into
+    // `pieces` pieces parsed on up to `threads` threads; the result does not
+    // depend on either.  `source` names the input in error messages.
+    static void parse_text(const std::string &text, const std::string &source,
+                           SimulationData &simulationData, unsigned threads, size_t pieces);
 
     // Splits at every ',' and keeps empty fields (InputParser.cpp:34-50).
     static std::vector<std::string> splitString(std::string string);

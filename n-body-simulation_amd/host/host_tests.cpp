@@ -7,13 +7,16 @@
 //
 //   host_tests                    run the checks
 //   host_tests --bench-writer N   time both writers on one step of N bodies
+//   host_tests --bench-parser F   time both parsers on the CSV file F
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <fstream>
 #include <functional>
+#include <iterator>
 #include <limits>
 #include <map>
 #include <random>
@@ -171,6 +174,89 @@ void makeCase(WriterCase &w, size_t n, uint64_t seed) {
     w.step.virial = 1e300;
 }
 
+// The serial parser: the reference's getline / splitString / std::stod loop
+// (InputParser.cpp:7-32) with a trailing '\r' dropped and empty lines skipped.
+void serialParse(const std::string &text, const std::string &source, SimulationData &d) {
+    std::istringstream in(text);
+    std::string line;
+    std::getline(in, line);
+    size_t lineNo = 1;
+    while (std::getline(in, line)) {
+        ++lineNo;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line.empty()) continue;
+        const std::vector<std::string> t = InputParser::splitString(line);
+        if (t.size() < 10)
+            throw std::invalid_argument("line " + std::to_string(lineNo) + " of " + source +
+                                        ": expected 10 comma-separated fields");
+        d.names.push_back(t[1]);
+        d.body_classes.push_back(t[2]);
+        d.mass.push_back(std::stod(t[3]));
+        d.positions_x.push_back(std::stod(t[4]));
+        d.positions_y.push_back(std::stod(t[5]));
+        d.positions_z.push_back(std::stod(t[6]));
+        d.velocities_x.push_back(std::stod(t[7]));
+        d.velocities_y.push_back(std::stod(t[8]));
+        d.velocities_z.push_back(std::stod(t[9]));
+    }
+}
+
+bool sameBits(const std::vector<double> &a, const std::vector<double> &b) {
+    return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * 8) == 0);
+}
+
+bool sameData(const SimulationData &a, const SimulationData &b) {
+    return a.names == b.names && a.body_classes == b.body_classes && sameBits(a.mass, b.mass) &&
+           sameBits(a.positions_x, b.positions_x) && sameBits(a.positions_y, b.positions_y) &&
+           sameBits(a.positions_z, b.positions_z) && sameBits(a.velocities_x, b.velocities_x) &&
+           sameBits(a.velocities_y, b.velocities_y) && sameBits(a.velocities_z, b.velocities_z);
+}
+
+// "ok", or the exception's type and message
+template <class F>
+std::string outcome(F f) {
+    try {
+        f();
+        return "ok";
+    } catch (const std::invalid_argument &e) {
+        return std::string("invalid_argument: ") + e.what();
+    } catch (const std::out_of_range &e) {
+        return std::string("out_of_range: ") + e.what();
+    }
+}
+
+// A CSV whose numbers exercise std::stod: signs, exponents, hex floats,
+// inf/nan, leading blanks, trailing junk after a number, 17-digit values;
+// with CRLF and LF line ends, empty lines, extra fields and no final newline.
+std::string makeCsv(size_t rows, uint64_t seed) {
+    const char *odd[] = {" 1.5",    "+.5",  "-0",        "0x1p-3",  "inf",     "-INF",
+                         "nan",     "1e5x", "7.25e+300", "  -2e-7", "1E3",     "0.000",
+                         "12345678901234567890", "4.9406564584124654e-310",  // subnormal
+                         "1e400",   "abc"};
+    std::mt19937_64 rng(seed);
+    std::uniform_real_distribution<double> u(-1.0, 1.0);
+    std::string s = "id,name,class,mass,x,y,z,vx,vy,vz\n";
+    char buf[40];
+    for (size_t i = 0; i < rows; ++i) {
+        s += std::to_string(i) + ",body " + std::to_string(i) + (i % 5 ? ",MBA" : ",");
+        for (int k = 0; k < 7; ++k) {
+            const uint64_t r = rng();
+            if (r % 23 == 0) {
+                const size_t which = (r >> 8) % 13;  // the 13 that std::stod accepts
+                s += std::string(",") + odd[which];
+            } else {
+                std::snprintf(buf, sizeof buf, ",%.17g", u(rng) * std::pow(10.0, (int)(r % 40) - 20));
+                s += buf;
+            }
+        }
+        if (i % 11 == 3) s += ",extra,fields";
+        s += i % 7 == 2 ? "\r\n" : "\n";
+        if (i % 13 == 5) s += i % 2 ? "\n" : "\r\n";
+    }
+    s.pop_back();  // no final newline
+    return s;
+}
+
 int benchWriter(size_t n) {
     WriterCase w;
     makeCase(w, n, 1);
@@ -195,7 +281,25 @@ int benchWriter(size_t n) {
 
 }  // namespace
 
+int benchParser(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    using clk = std::chrono::steady_clock;
+    SimulationData a, b;
+    auto t0 = clk::now();
+    serialParse(text, path, a);
+    auto t1 = clk::now();
+    InputParser::parse_text(text, path, b, 8, 32);
+    auto t2 = clk::now();
+    std::printf("{\"rows\": %zu, \"bytes\": %zu, \"serial_s\": %.3f, \"parallel_8_threads_s\": %.3f, "
+                "\"identical\": %s}\n",
+                a.mass.size(), text.size(), std::chrono::duration<double>(t1 - t0).count(),
+                std::chrono::duration<double>(t2 - t1).count(), sameData(a, b) ? "true" : "false");
+    return sameData(a, b) ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
+    if (argc == 3 && std::string(argv[1]) == "--bench-parser") return benchParser(argv[2]);
     if (argc == 3 && std::string(argv[1]) == "--bench-writer")
         return benchWriter((size_t)std::strtoull(argv[2], nullptr, 10));
     {
@@ -246,6 +350,60 @@ int main(int argc, char **argv) {
         pvd << "</Collection>" << '\n' << "</VTKFile>" << '\n';
         check(paraview::formatPvd(3) == pvd.str(),
               "ParaViewWriter.pvd");
+    }
+    {
+        // parallel parser == serial parser: values bit for bit, names, classes,
+        // and the first error's type and message, for any piece count
+        const std::string good = makeCsv(700, 3);
+        SimulationData want;
+        serialParse(good, "in.csv", want);
+        bool ok = want.mass.size() == 700;
+        for (unsigned t : {1u, 4u})
+            for (size_t pieces : {(size_t)1, (size_t)2, (size_t)7, (size_t)64, (size_t)5000}) {
+                SimulationData got;
+                InputParser::parse_text(good, "in.csv", got, t, pieces);
+                ok = ok && sameData(got, want);
+            }
+        check(ok, "InputParser.parallelMatchesSerial");
+        // each error alone, placed on different lines; then two errors, where
+        // the earlier line must win whatever the pieces
+        const char *bad[] = {"1,a,b,1,2,3,4,5,6", "1,a,b,1,2,3,abc,5,6,7", "1,a,b,1e400,2,3,4,5,6,7",
+                             "1,a,b,1,2,3,4,5,6,1e-320", "1,a,b,1,2,3,4,5,,7", ""};
+        bool same = true;
+        std::map<std::string, int> seen;
+        for (const char *line : bad)
+            for (size_t at : {(size_t)0, (size_t)350, (size_t)699}) {
+                std::string text = good;
+                size_t pos = 0;
+                for (size_t l = 0; l <= at; ++l) pos = text.find('\n', pos) + 1;
+                text.insert(pos, std::string(line) + "\n");
+                std::string text2 = text;
+                pos = 0;
+                for (size_t l = 0; l <= at + 200 && pos != std::string::npos + 1; ++l)
+                    pos = text2.find('\n', pos) + 1;
+                if (pos != 0) text2.insert(pos, "9,x,y,oops,0,0,0,0,0,0\n");
+                for (const std::string *tx : {&text, &text2}) {
+                    SimulationData a;
+                    const std::string w = outcome([&] { serialParse(*tx, "in.csv", a); });
+                    ++seen[w.substr(0, w.find(' ', 22))];
+                    for (size_t pieces : {(size_t)1, (size_t)3, (size_t)40, (size_t)999}) {
+                        SimulationData b;
+                        const std::string g = outcome(
+                            [&] { InputParser::parse_text(*tx, "in.csv", b, 4, pieces); });
+                        if (g != w || (w == "ok" && !sameData(a, b))) {
+                            same = false;
+                            std::printf("  mismatch at %zu pieces %zu: '%s' vs '%s'\n", at,
+                                        pieces, g.c_str(), w.c_str());
+                        }
+                    }
+                }
+            }
+        check(same && seen.size() >= 4, "InputParser.firstErrorMatchesSerial");
+        for (auto &kv : seen) std::printf("  outcome %s x%d\n", kv.first.c_str(), kv.second);
+        SimulationData e1, e2;
+        InputParser::parse_text("", "in.csv", e1, 4, 8);
+        InputParser::parse_text("id,name\n", "in.csv", e2, 4, 8);
+        check(e1.mass.empty() && e2.mass.empty(), "InputParser.headerOnly");
     }
     std::printf("%d failure(s)\n", failures);
     return failures == 0 ? 0 : 1;
