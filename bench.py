@@ -47,9 +47,9 @@ CPU_SAMPLE_S = 10.0
 # HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE runs of this script scaled by a device copy of known size
 # (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0):
-# 566,777 KiB x 2 read + 134,450 KiB written.
+# 565,982 KiB x 2 read + 134,447 KiB written.
 TRAFFIC_BYTES = {
-    "bh_disk_1m": 1.298e9,
+    "bh_disk_1m": 1.297e9,
 }
 
 WORKLOADS = {
