@@ -1219,10 +1219,12 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__
 // the workgroup, and every cell is local (one centre-of-mass pass).
 constexpr int kForestThreads = 1024;
 constexpr int kForestWaves = kForestThreads / 64;
-constexpr int kSortBits = 8;  // radix digit width: 256 bins x 16 waves of LDS counters
+constexpr int kSortBits = 9;  // radix digit width: 512 bins x 16 waves of LDS counters (32 KB)
 constexpr int kSortBins = 1 << kSortBits;
-constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;  // 8: the result ends in place
-static_assert(kSortPasses % 2 == 0, "the sorted pairs must end in the input buffers");
+// 7 passes: an odd count, so keys written to the alternate buffers end sorted
+// in the main ones
+constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;
+static_assert(kSortPasses % 2 == 1, "the forest's keys start in the alternate buffers");
 constexpr int kSortUnroll = 4;  // 64-key groups per iteration of a wave
 
 // lanes below this one, as a ballot mask
@@ -1293,8 +1295,9 @@ __device__ inline uint64_t digit_peers(uint32_t dig, bool valid) {
     return peers;
 }
 
-// Stable LSD radix sort of (k, v) pairs by the 63-bit keys, 8-bit digits,
-// (k2, v2) the ping-pong buffers.  Wave w owns a contiguous range and reads
+// Stable LSD radix sort of (k, v) pairs by the 63-bit keys, kSortBits-bit
+// digits, (k2, v2) the ping-pong buffers (the result is in (k2, v2) after an
+// odd number of passes).  Wave w owns a contiguous range and reads
 // it 64 consecutive pairs at a time; a pair's rank among the equal digits of
 // its 64 comes from ballots, and the offsets run digit-major over the waves
 // (whist[w * 256 + d]), so equal digits keep their order: stable.
@@ -1316,9 +1319,11 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
             bool val[kSortUnroll];
 #pragma unroll
             for (int u = 0; u < kSortUnroll; ++u) {
+                // unconditional loads (index clamped into the range, e > g
+                // here): a load under a condition would be waited for at once
                 const uint32_t i = g + 64u * u + lane;
                 val[u] = i < e;
-                dig[u] = val[u] ? (uint32_t)(sk[i] >> sh) & (kSortBins - 1) : 0u;
+                dig[u] = (uint32_t)(sk[min(i, e - 1)] >> sh) & (kSortBins - 1);
             }
 #pragma unroll
             for (int u = 0; u < kSortUnroll; ++u) {
@@ -1328,7 +1333,7 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
         }
         __syncthreads();
         // offsets: digit-major exclusive scan over (digit, wave); threads
-        // 0..255 (waves 0..3) take one digit each
+        // 0..kSortBins-1 take one digit each
         uint32_t dig_excl = 0;
         if (t < (uint32_t)kSortBins) {
             uint32_t run = 0;
@@ -1360,8 +1365,8 @@ __device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uin
             for (int u = 0; u < kSortUnroll; ++u) {
                 const uint32_t i = g + 64u * u + lane;
                 ok[u] = i < e;
-                key[u] = ok[u] ? sk[i] : 0ull;
-                val[u] = ok[u] ? sv[i] : 0u;
+                key[u] = sk[min(i, e - 1)];
+                val[u] = sv[min(i, e - 1)];
             }
             // the groups in order: group u's ranks follow group u-1's
 #pragma unroll
@@ -1482,12 +1487,12 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
         for (uint32_t i = t; i < n; i += kForestThreads) {
             Cell c = root;
             const double px = x[i], py = y[i], pz = z[i];
-            skey[i] = key_from(c, px, py, pz);
-            sidx[i] = i;
+            skey_alt[i] = key_from(c, px, py, pz);
+            sidx_alt[i] = i;
             body4[i] = make_double4(px, py, pz, m[i]);
         }
         __syncthreads();
-        wg_radix_sort(skey, sidx, skey_alt, sidx_alt, n, whist, wsum);
+        wg_radix_sort(skey_alt, sidx_alt, skey, sidx, n, whist, wsum);  // sorted into skey, sidx
     }
     // 4./5. lcp, deep runs, cell numbering
     for (uint32_t i = t; i + 1 < n; i += kForestThreads)
