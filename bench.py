@@ -128,6 +128,79 @@ def copy_calibration(mib=256, reps=3):
     print(json.dumps({"copy_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def snapshot_cost(reps=40, rounds=3):
+    """Cost of a visualised step of the default workload (every step stored),
+    three ways, alternated in one process: nb_steps alone; nb_steps then
+    synchronous nb_get_state + nb_get_accel into reused buffers and a copy
+    into a time series (the reference's loop); nb_steps with the asynchronous
+    snapshot begun after each step and stored on a second thread during the
+    next one (the application's loop)."""
+    import ctypes
+    import threading
+    kind, n, seed, algorithm, theta, dt, prec = WORKLOADS["bh_disk_1m"]
+    d = gen(kind, n, seed)
+    ctx = nb.Context(n, theta=theta, precision=prec)
+    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    ctx.bh_build()
+    ctx.bh_forces()
+    lib, h = ctx.lib, ctx.h
+    P = ctypes.POINTER(ctypes.c_double)
+    buf = [np.empty(n) for _ in range(9)]
+    series = [np.empty(n) for _ in range(9)]
+
+    def step():
+        rc = lib.nb_steps(h, nb.ALGO_BARNES_HUT, dt, 1)
+        if rc:
+            raise nb.NBodyError(rc, "nb_steps")
+
+    def plain():
+        for _ in range(reps):
+            step()
+
+    def sync_reads():
+        for _ in range(reps):
+            step()
+            assert lib.nb_get_state(h, *[b.ctypes.data_as(P) for b in buf[:6]]) == 0
+            assert lib.nb_get_accel(h, *[b.ctypes.data_as(P) for b in buf[6:]]) == 0
+            for dst, src in zip(series, buf):
+                np.copyto(dst, src)
+
+    def async_snapshots():
+        worker = None
+        ptrs = (P * 9)()
+
+        def store(views):
+            for dst, src in zip(series, views):
+                np.copyto(dst, src)
+        for r in range(reps):
+            step()
+            if r:
+                assert lib.nb_snapshot_wait(h, ptrs) == 0
+                views = [np.ctypeslib.as_array(ptrs[i], shape=(n,)) for i in range(9)]
+                if worker:
+                    worker.join()
+                worker = threading.Thread(target=store, args=(views,))
+                worker.start()
+            assert lib.nb_snapshot_begin(h) == 0
+        assert lib.nb_snapshot_wait(h, ptrs) == 0
+        if worker:
+            worker.join()
+
+    ways = {"steps_only": plain, "sync_reads": sync_reads, "async_snapshot": async_snapshots}
+    for f in ways.values():  # warm-up, also allocates the snapshot slots
+        f()
+    ms = {k: [] for k in ways}
+    for _ in range(rounds):
+        for k, f in ways.items():
+            t0 = time.perf_counter()
+            f()
+            ms[k].append((time.perf_counter() - t0) / reps * 1e3)
+    ctx.close()
+    print(json.dumps({"snapshot_cost": "ms per step, every step stored", "bodies": n,
+                      "workload": "bh_disk_1m", "reps": reps, "rounds": rounds,
+                      **{k: round(float(np.median(v)), 3) for k, v in ms.items()}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +213,13 @@ def main():
                          "each (same input), reported under config.theta_sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration)")
+    ap.add_argument("--snapshot-cost", action="store_true",
+                    help="only time a visualised step of the default workload: synchronous "
+                         "reads against the asynchronous snapshot")
     args = ap.parse_args()
+    if args.snapshot_cost:
+        snapshot_cost()
+        return
     if args.copy_calibration:
         copy_calibration()
         return
