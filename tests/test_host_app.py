@@ -46,6 +46,19 @@ def test_reference_unit_tests():
     assert r.stdout.count("PASS") >= 12
 
 
+def test_parallel_writer_and_parser_match_serial():
+    # the parallel ParaView/lastState.csv writer against a serial std::ostream
+    # rendition, byte for byte, and the parallel CSV parser against the serial
+    # getline/std::stod loop (values, names, first error), for several thread
+    # counts and piece sizes (host/host_tests.cpp)
+    r = subprocess.run([HOST_TESTS], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout
+    for name in ["ParaViewWriter.matchesStreamOutput n=0", "ParaViewWriter.matchesStreamOutput n=1000",
+                 "ParaViewWriter.pvd", "InputParser.parallelMatchesSerial",
+                 "InputParser.firstErrorMatchesSerial", "InputParser.headerOnly"]:
+        assert f"PASS {name}\n" in r.stdout, name
+
+
 def test_generator_csv_roundtrip(nb, tmp_path):
     out = tmp_path / "disk.csv"
     subprocess.run([GEN, "--kind=disk", "--n=1000", "--seed=7", f"--out={out}"], check=True)
