@@ -133,6 +133,7 @@ int finish(nb_ensemble *e) {
         return efail(e, NB_E_NODE_OVERFLOW, "octree storage exhausted" + where);
     if (st & nb::kFlagZeroMass) return efail(e, NB_E_ZERO_MASS, "zero-mass octree cell" + where);
     if (st & nb::kFlagStack) return efail(e, NB_E_STACK_OVERFLOW, "traversal stack exhausted" + where);
+    if (st & nb::kFlagSortWait) return efail(e, NB_E_HIP, "radix sort: a look-back wait did not end");
     return efail(e, NB_E_HIP, "unknown device status " + std::to_string(st));
 }
 
@@ -172,8 +173,8 @@ int apply_tuning(nb_ensemble *e) {
 
 void enqueue_forces(nb_ensemble *e) {
     mark(e, NB_PHASE_OCTREE, true);
-    nb::launch_forest_build(e->sys, e->bh_order, e->nbh, e->x, e->y, e->z, e->m, e->p.G, e->fb,
-                            e->items, e->status, e->stream);
+    nb::launch_forest_build(e->sys, e->bh_order, e->nbh, e->nsys, e->x, e->y, e->z, e->m, e->p.G,
+                            e->fb, e->items, e->status, e->stream);
     mark(e, NB_PHASE_OCTREE, false);
     mark(e, NB_PHASE_ACCEL, true);
     nb::launch_ens_items(e->nsys, e->qorder, e->items, e->item_off, e->cursor, e->status,
@@ -277,8 +278,11 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &e->items, nsys)) || (rc = ealloc(e, &e->item_off, nsys + 1ull)) ||
         (rc = ealloc(e, &e->qorder, nsys)) || (rc = ealloc(e, &e->bh_order, nsys)) ||
         (rc = ealloc(e, &e->cursor, 1)) || (rc = ealloc(e, &e->status, 1)) ||
-        (rc = ealloc(e, &e->inter, total)))
+        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) ||
+        (rc = ealloc(e, &f.radix_ctl, nb::radix_ctl_words((uint32_t)fo))) ||
+        (rc = ealloc(e, &f.radix_ctl_sys, nb::radix_ctl_words((uint32_t)fo))))
         return bail(rc);
+    f.slots = (uint32_t)fo;
     if (hipHostMalloc((void **)&e->status_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return bail(NB_E_HIP);
     *e->status_host = 0;
@@ -288,6 +292,16 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         std::fill(so.begin() + e->sys_host[s].body_off,
                   so.begin() + e->sys_host[s].body_off + sizes[s], (uint16_t)s);
     if (hipMemcpy(e->sys_of, so.data(), total * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(NB_E_HIP);
+    // slot -> system over the forest's slots; padding belongs to the system before it
+    std::vector<uint16_t> fs(fo);
+    for (uint32_t s = 0; s < nsys; ++s) {
+        const uint64_t end = s + 1 < nsys ? e->sys_host[s + 1].fb_off : fo;
+        std::fill(fs.begin() + e->sys_host[s].fb_off, fs.begin() + end, (uint16_t)s);
+    }
+    if (hipMemcpy(f.fb_sys, fs.data(), fo * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(f.radix_ctl, 0, (size_t)nb::radix_ctl_words((uint32_t)fo) * 4) != hipSuccess ||
+        hipMemset(f.radix_ctl_sys, 0, (size_t)nb::radix_ctl_words((uint32_t)fo) * 4) != hipSuccess)
         return bail(NB_E_HIP);
     for (double **a : st)
         if (hipMemsetAsync(*a, 0, total * sizeof(double), e->stream) != hipSuccess) return bail(NB_E_HIP);

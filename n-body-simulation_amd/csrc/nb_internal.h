@@ -185,6 +185,12 @@ struct ForestBufs {
     // per node (offset node_off)
     int32_t *ref;
     NodeF32 *rec;
+    // the body slots of all systems (padding included): slot -> system, and
+    // the control words of the forest-wide radix sorts, one buffer per sort
+    // (by key, then by system): a buffer's rotation assumes one pass count
+    uint32_t slots;
+    uint16_t *fb_sys;
+    uint32_t *radix_ctl, *radix_ctl_sys;
 };
 
 // ---- launchers (each .hip file defines the kernels it launches) -------------
@@ -225,9 +231,9 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s);
 // (order: the BH systems' indices, largest first); items[s] receives the
 // system's traversal-group count.
 void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
-                         const double *x, const double *y, const double *z, const double *m,
-                         double G, const ForestBufs &fb, uint32_t *items, uint32_t *status,
-                         hipStream_t s);
+                         uint32_t nsys, const double *x, const double *y, const double *z,
+                         const double *m, double G, const ForestBufs &fb, uint32_t *items,
+                         uint32_t *status, hipStream_t s);
 
 // nb_forces.hip
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,

@@ -1219,20 +1219,6 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__
 // the workgroup, and every cell is local (one centre-of-mass pass).
 constexpr int kForestThreads = 1024;
 constexpr int kForestWaves = kForestThreads / 64;
-constexpr int kSortBits = 9;  // radix digit width: 512 bins x 16 waves of LDS counters (32 KB)
-constexpr int kSortBins = 1 << kSortBits;
-// 7 passes: an odd count, so keys written to the alternate buffers end sorted
-// in the main ones
-constexpr int kSortPasses = (63 + kSortBits - 1) / kSortBits;
-static_assert(kSortPasses % 2 == 1, "the forest's keys start in the alternate buffers");
-constexpr int kSortUnroll = 4;  // 64-key groups per iteration of a wave
-
-// lanes below this one, as a ballot mask
-__device__ inline uint64_t forest_lanes_below() {
-    const uint32_t lane = threadIdx.x & 63u;
-    return lane == 0 ? 0ull : (~0ull >> (64u - lane));
-}
-
 // Per-wave contiguous range [*b, *e) of m items: 16 ranges of whole 64-item
 // groups, so a wave reads 64 consecutive items at a time (coalesced) and the
 // ranges keep the item order wave by wave.
@@ -1283,127 +1269,113 @@ __device__ inline uint32_t wg_scan_array(uint32_t *a, uint32_t m, uint32_t *wsum
     return total;
 }
 
-// Lanes of the wave holding the same digit as this lane (valid lanes only).
-__device__ inline uint64_t digit_peers(uint32_t dig, bool valid) {
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < kSortBits; ++b) {
-        const bool bit = (dig >> b) & 1u;
-        const uint64_t mask = __ballot(bit);
-        peers &= bit ? mask : ~mask;
+// 1. Per Barnes-Hut system (one workgroup each): header reset, AABB including
+// the origin (every partial starts at 0.0), then cubify into the header.
+__global__ void __launch_bounds__(kForestThreads) k_forest_aabb(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order,
+    const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
+    ForestBufs fb) {
+    __shared__ double red[kForestWaves][6];
+    const uint32_t s = order[blockIdx.x];
+    const EnsSystem S = sys[s];
+    const uint32_t n = S.n, t = threadIdx.x;
+    const double *x = gx + S.body_off, *y = gy + S.body_off, *z = gz + S.body_off;
+    TreeHdr *h = fb.hdr + s;
+    if (t == 0) {
+        h->internal_count = 0;
+        h->max_depth = 0;
+        h->status = 0;
+        h->deep_runs = 0;
+        h->stack_peak = 0;
+        h->group_count = 0;
+        h->group_next = 0;
+        h->cross_count = 0;
+        h->corr_pairs = 0;
     }
-    return peers;
+    __syncthreads();  // the reset precedes every flag raised below
+    double mn[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
+    bool bad = false;
+    for (uint32_t i = t; i < n; i += kForestThreads) {
+        const double p[3] = {x[i], y[i], z[i]};
+        for (int c = 0; c < 3; ++c) {
+            bad |= !isfinite(p[c]);
+            mn[c] = sycl_min(mn[c], p[c]);
+            mx[c] = sycl_max(mx[c], p[c]);
+        }
+    }
+    if (bad) raise_flag(h, kFlagAabb);
+    for (int off = 32; off >= 1; off >>= 1)
+        for (int c = 0; c < 3; ++c) {
+            mn[c] = sycl_min(mn[c], __shfl_xor(mn[c], off, 64));
+            mx[c] = sycl_max(mx[c], __shfl_xor(mx[c], off, 64));
+        }
+    if ((t & 63) == 0)
+        for (int c = 0; c < 3; ++c) {
+            red[t >> 6][c] = mn[c];
+            red[t >> 6][3 + c] = mx[c];
+        }
+    __syncthreads();
+    if (t == 0) {
+        double v[6];
+        for (int c = 0; c < 6; ++c) v[c] = red[0][c];
+        for (int w = 1; w < kForestWaves; ++w)
+            for (int c = 0; c < 3; ++c) {
+                v[c] = sycl_min(v[c], red[w][c]);
+                v[3 + c] = sycl_max(v[3 + c], red[w][3 + c]);
+            }
+        cubify(v, h);
+    }
 }
 
-// Stable LSD radix sort of (k, v) pairs by the 63-bit keys, kSortBits-bit
-// digits, (k2, v2) the ping-pong buffers (the result is in (k2, v2) after an
-// odd number of passes).  Wave w owns a contiguous range and reads
-// it 64 consecutive pairs at a time; a pair's rank among the equal digits of
-// its 64 comes from ballots, and the offsets run digit-major over the waves
-// (whist[w * 256 + d]), so equal digits keep their order: stable.
-__device__ inline void wg_radix_sort(uint64_t *k, uint32_t *v, uint64_t *k2, uint32_t *v2,
-                                     uint32_t n, uint32_t *whist, uint32_t *wsum) {
-    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
-    const uint64_t below = forest_lanes_below();
-    uint32_t b, e;
-    wave_range(n, &b, &e);
-    uint32_t *row = whist + w * kSortBins;
-    uint64_t *sk = k, *dk = k2;
-    uint32_t *sv = v, *dv = v2;
-    for (int p = 0; p < kSortPasses; ++p) {
-        const int sh = kSortBits * p;
-        for (uint32_t d = lane; d < (uint32_t)kSortBins; d += 64) row[d] = 0;
-        // kSortUnroll groups of 64 per iteration: their loads are in flight together
-        for (uint32_t g = b; g < e; g += 64 * kSortUnroll) {
-            uint32_t dig[kSortUnroll];
-            bool val[kSortUnroll];
-#pragma unroll
-            for (int u = 0; u < kSortUnroll; ++u) {
-                // unconditional loads (index clamped into the range, e > g
-                // here): a load under a condition would be waited for at once
-                const uint32_t i = g + 64u * u + lane;
-                val[u] = i < e;
-                dig[u] = (uint32_t)(sk[min(i, e - 1)] >> sh) & (kSortBins - 1);
-            }
-#pragma unroll
-            for (int u = 0; u < kSortUnroll; ++u) {
-                const uint64_t peers = digit_peers(dig[u], val[u]);
-                if (val[u] && (peers & below) == 0) row[dig[u]] += (uint32_t)__popcll(peers);
-            }
-        }
-        __syncthreads();
-        // offsets: digit-major exclusive scan over (digit, wave); threads
-        // 0..kSortBins-1 take one digit each
-        uint32_t dig_excl = 0;
-        if (t < (uint32_t)kSortBins) {
-            uint32_t run = 0;
-            for (int ww = 0; ww < kForestWaves; ++ww) {
-                const uint32_t c = whist[ww * kSortBins + t];
-                whist[ww * kSortBins + t] = run;
-                run += c;
-            }
-            uint32_t inc = run;
-#pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-                const uint32_t o = __shfl_up(inc, off, 64);
-                if (lane >= (uint32_t)off) inc += o;
-            }
-            dig_excl = inc - run;
-            if (lane == 63) wsum[w] = inc;
-        }
-        __syncthreads();
-        if (t < (uint32_t)kSortBins) {
-            for (uint32_t ww = 0; ww < w; ++ww) dig_excl += wsum[ww];
-            for (int ww = 0; ww < kForestWaves; ++ww) whist[ww * kSortBins + t] += dig_excl;
-        }
-        __syncthreads();
-        for (uint32_t g = b; g < e; g += 64 * kSortUnroll) {
-            uint64_t key[kSortUnroll];
-            uint32_t val[kSortUnroll];
-            bool ok[kSortUnroll];
-#pragma unroll
-            for (int u = 0; u < kSortUnroll; ++u) {
-                const uint32_t i = g + 64u * u + lane;
-                ok[u] = i < e;
-                key[u] = sk[min(i, e - 1)];
-                val[u] = sv[min(i, e - 1)];
-            }
-            // the groups in order: group u's ranks follow group u-1's
-#pragma unroll
-            for (int u = 0; u < kSortUnroll; ++u) {
-                const uint32_t dig = (uint32_t)(key[u] >> sh) & (kSortBins - 1);
-                const uint64_t peers = digit_peers(dig, ok[u]);
-                // every lane reads its digit's offset before the leader advances it
-                const uint32_t base = row[dig];
-                if (ok[u]) {
-                    const uint32_t pos = base + (uint32_t)__popcll(peers & below);
-                    dk[pos] = key[u];
-                    dv[pos] = val[u];
-                    if ((peers & below) == 0) row[dig] = base + (uint32_t)__popcll(peers);
-                }
-            }
-        }
-        __syncthreads();
-        uint64_t *tk = sk;
-        sk = dk;
-        dk = tk;
-        uint32_t *tv = sv;
-        sv = dv;
-        dv = tv;
+// The largest 63-bit key: the padding between two systems' slots sorts after
+// their bodies (ties keep slot order, and padding slots follow the bodies).
+constexpr uint64_t kPadKey = ~0ull >> 1;
+
+// 2. Keys of every slot j of the forest's body space (all systems at once):
+// a Barnes-Hut system's body i = j - fb_off gets its key in the system's root
+// cell and its packed record; padding and direct-sum slots get kPadKey.
+__global__ void __launch_bounds__(kThreads) k_forest_keys(
+    uint32_t slots, const uint16_t *__restrict__ fb_sys, const EnsSystem *__restrict__ sys,
+    const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
+    const double *__restrict__ gm, ForestBufs fb, uint64_t *key, uint32_t *val) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= slots) return;
+    const uint32_t s = fb_sys[j];
+    const EnsSystem S = sys[s];
+    const uint32_t i = j - S.fb_off;
+    uint64_t k = kPadKey;
+    if (S.bh && i < S.n) {
+        const TreeHdr *h = fb.hdr + s;
+        Cell c{h->minx, h->miny, h->minz, h->edge};
+        const uint32_t b = S.body_off + i;
+        const double px = gx[b], py = gy[b], pz = gz[b];
+        k = key_from(c, px, py, pz);
+        fb.body4[j] = make_double4(px, py, pz, gm[b]);
     }
+    key[j] = k;
+    val[j] = j;
+}
+
+// 3b. After the Morton passes: each slot's key becomes its system, for the
+// stable pass that gathers every system's slots back into its own range.
+__global__ void __launch_bounds__(kThreads) k_forest_sysrank(uint32_t slots,
+                                                             const uint16_t *__restrict__ fb_sys,
+                                                             uint64_t *key,
+                                                             const uint32_t *__restrict__ val) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j < slots) key[j] = fb_sys[val[j]];
 }
 
 __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order,
     const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
-    const double *__restrict__ gm, double G, ForestBufs fb, uint32_t *items, uint32_t *status) {
-    __shared__ uint32_t whist[kSortBins * kForestWaves];
+    const double *__restrict__ gm, double G, ForestBufs fb,
+    const uint32_t *sorted,  // may be fb.sidx itself: each slot is read, then rewritten
+    uint32_t *items, uint32_t *status) {
     __shared__ uint32_t wsum[kForestWaves + 1];
     __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
     __shared__ uint32_t lvl_c[kMaxDepth];      // fill cursors
     __shared__ int lcp_max;
-    __shared__ double red[kForestWaves][6];
-    __shared__ double cube[4];  // root cell {min x, min y, min z, edge}
     const uint32_t s = order[blockIdx.x];
     const EnsSystem S = sys[s];
     const uint32_t n = S.n, t = threadIdx.x;
@@ -1411,8 +1383,8 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
                  *m = gm + S.body_off;
     TreeHdr *h = fb.hdr + s;
     const uint32_t fo = S.fb_off;
-    uint64_t *skey = fb.skey + fo, *skey_alt = fb.skey_alt + fo, *klo = fb.klo + fo;
-    uint32_t *sidx = fb.sidx + fo, *sidx_alt = fb.sidx_alt + fo, *chain = fb.chain + fo,
+    uint64_t *skey = fb.skey + fo, *klo = fb.klo + fo;
+    uint32_t *sidx = fb.sidx + fo, *chain = fb.chain + fo,
              *body_parent = fb.body_parent + fo, *group_size = fb.group_size + fo,
              *group_slot = fb.group_slot + fo;
     int32_t *lcp = fb.lcp + fo;
@@ -1427,73 +1399,22 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     NodeF32 *rec = fb.rec + S.node_off;
     const uint32_t icap = S.icap;
 
-    if (t == 0) {
-        h->internal_count = 0;
-        h->max_depth = 0;
-        h->status = 0;
-        h->deep_runs = 0;
-        h->stack_peak = 0;
-        h->group_count = 0;
-        h->group_next = 0;
-        h->cross_count = 0;
-        h->corr_pairs = 0;
-        lcp_max = -1;
-    }
-    __syncthreads();  // the reset precedes every flag raised below
-    // 1. AABB including the origin (every partial starts at 0.0), then cubify
-    {
-        double mn[3] = {0.0, 0.0, 0.0}, mx[3] = {0.0, 0.0, 0.0};
-        bool bad = false;
-        for (uint32_t i = t; i < n; i += kForestThreads) {
-            const double p[3] = {x[i], y[i], z[i]};
-            for (int c = 0; c < 3; ++c) {
-                bad |= !isfinite(p[c]);
-                mn[c] = sycl_min(mn[c], p[c]);
-                mx[c] = sycl_max(mx[c], p[c]);
-            }
+    if (t == 0) lcp_max = -1;
+    // the root cell, from the header k_forest_aabb wrote
+    const Cell root{h->minx, h->miny, h->minz, h->edge};
+    // 2./3. the system's slots come sorted from the forest-wide radix sort:
+    // body indices in key order (ties in index order), and their keys
+    for (uint32_t i = t; i < n; i += kForestThreads) {
+        uint32_t li = sorted[fo + i] - fo;
+        if (li >= n) {  // only after a failed sort, which the flag reports
+            raise_flag(h, kFlagSortWait);
+            li = 0;
         }
-        if (bad) raise_flag(h, kFlagAabb);
-        for (int off = 32; off >= 1; off >>= 1)
-            for (int c = 0; c < 3; ++c) {
-                mn[c] = sycl_min(mn[c], __shfl_xor(mn[c], off, 64));
-                mx[c] = sycl_max(mx[c], __shfl_xor(mx[c], off, 64));
-            }
-        if ((t & 63) == 0)
-            for (int c = 0; c < 3; ++c) {
-                red[t >> 6][c] = mn[c];
-                red[t >> 6][3 + c] = mx[c];
-            }
-        __syncthreads();
-        if (t == 0) {
-            double v[6];
-            for (int c = 0; c < 6; ++c) v[c] = red[0][c];
-            for (int w = 1; w < kForestWaves; ++w)
-                for (int c = 0; c < 3; ++c) {
-                    v[c] = sycl_min(v[c], red[w][c]);
-                    v[3 + c] = sycl_max(v[3 + c], red[w][3 + c]);
-                }
-            cubify(v, h);
-            cube[0] = h->minx;
-            cube[1] = h->miny;
-            cube[2] = h->minz;
-            cube[3] = h->edge;
-        }
-        __syncthreads();
+        Cell c = root;
+        skey[i] = key_from(c, x[li], y[li], z[li]);
+        sidx[i] = li;
     }
-    // the workgroup reads the root cell from LDS, never back from the header
-    const Cell root{cube[0], cube[1], cube[2], cube[3]};
-    // 2. keys, 3. sort
-    {
-        for (uint32_t i = t; i < n; i += kForestThreads) {
-            Cell c = root;
-            const double px = x[i], py = y[i], pz = z[i];
-            skey_alt[i] = key_from(c, px, py, pz);
-            sidx_alt[i] = i;
-            body4[i] = make_double4(px, py, pz, m[i]);
-        }
-        __syncthreads();
-        wg_radix_sort(skey_alt, sidx_alt, skey, sidx, n, whist, wsum);  // sorted into skey, sidx
-    }
+    __syncthreads();
     // 4./5. lcp, deep runs, cell numbering
     for (uint32_t i = t; i + 1 < n; i += kForestThreads)
         lcp_at(i, x, y, z, n, root, h, skey, sidx, klo, lcp);
@@ -1558,12 +1479,35 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
 }  // namespace
 
 void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
-                         const double *x, const double *y, const double *z, const double *m,
-                         double G, const ForestBufs &fb, uint32_t *items, uint32_t *status,
-                         hipStream_t s) {
+                         uint32_t nsys, const double *x, const double *y, const double *z,
+                         const double *m, double G, const ForestBufs &fb, uint32_t *items,
+                         uint32_t *status, hipStream_t s) {
     if (nbh == 0) return;
+    hipLaunchKernelGGL(k_forest_aabb, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
+                       fb);
+    // one radix sort of every system's slots by key, then a stable one by
+    // system: each system's bodies end in its own slot range, in key order
+    // with ties in index order (the order of a sort of that system alone)
+    const uint32_t slots = fb.slots, blocks = ceil_div(slots, kThreads);
+    uint64_t *k = fb.skey, *ka = fb.skey_alt;
+    uint32_t *v = fb.sidx, *va = fb.sidx_alt;
+    hipLaunchKernelGGL(k_forest_keys, dim3(blocks), dim3(kThreads), 0, s, slots,
+                       (const uint16_t *)fb.fb_sys, sys, x, y, z, m, fb, k, v);
+    if (radix_sort_u64_u32(k, v, ka, va, slots, 3 * kKeyLevels, fb.radix_ctl, status, s)) {
+        std::swap(k, ka);
+        std::swap(v, va);
+    }
+    hipLaunchKernelGGL(k_forest_sysrank, dim3(blocks), dim3(kThreads), 0, s, slots,
+                       (const uint16_t *)fb.fb_sys, k, (const uint32_t *)v);
+    int sys_bits = 1;
+    while (sys_bits < 16 && (1u << sys_bits) < nsys) ++sys_bits;
+    // its own control words: a control buffer must always see the same pass count
+    if (radix_sort_u64_u32(k, v, ka, va, slots, sys_bits, fb.radix_ctl_sys, status, s)) {
+        std::swap(k, ka);
+        std::swap(v, va);
+    }
     hipLaunchKernelGGL(k_forest_build, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
-                       m, G, fb, items, status);
+                       m, G, fb, (const uint32_t *)v, items, status);
 }
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,

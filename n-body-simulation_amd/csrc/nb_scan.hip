@@ -272,6 +272,10 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     if (owner) next_hist[tid] = 0;
     __syncthreads();
     const uint32_t t = tile_id;
+    if (t >= nb) {  // only with corrupt control words: report it and touch nothing
+        if (tid == 0) atomicOr(flags, kFlagSortWait);
+        return;  // block-uniform, before any further barrier
+    }
     if (t == 0 && owner) {
         // the previous pass (or the last pass of the previous sort) has ended
         const int prev = pass > 0 ? pass - 1 : passes - 1;
