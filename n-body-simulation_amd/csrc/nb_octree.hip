@@ -1356,14 +1356,22 @@ __global__ void __launch_bounds__(kThreads) k_forest_keys(
     val[j] = j;
 }
 
-// 3b. After the Morton passes: each slot's key becomes its system, for the
-// stable pass that gathers every system's slots back into its own range.
+// Low key bits sorted by the first forest-wide sort; the second sorts the
+// system above the remaining high bits.  Both sorts then take 4 passes of 9
+// bits (for up to 512 systems), the minimum of a sort.
+constexpr int kForestLowBits = 36;
+
+// 3b. After the sort by the low key bits: each slot's key becomes (its system,
+// its high key bits), for the stable sort that gathers every system's slots
+// back into its own range, in key order.
 __global__ void __launch_bounds__(kThreads) k_forest_sysrank(uint32_t slots,
                                                              const uint16_t *__restrict__ fb_sys,
                                                              uint64_t *key,
                                                              const uint32_t *__restrict__ val) {
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j < slots) key[j] = fb_sys[val[j]];
+    if (j < slots)
+        key[j] = ((uint64_t)fb_sys[val[j]] << (3 * kKeyLevels - kForestLowBits)) |
+                 (key[j] >> kForestLowBits);
 }
 
 __global__ void __launch_bounds__(kForestThreads) k_forest_build(
@@ -1485,15 +1493,16 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     if (nbh == 0) return;
     hipLaunchKernelGGL(k_forest_aabb, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
                        fb);
-    // one radix sort of every system's slots by key, then a stable one by
-    // system: each system's bodies end in its own slot range, in key order
-    // with ties in index order (the order of a sort of that system alone)
+    // LSD over (system, key): one radix sort of every system's slots by the
+    // low key bits, then a stable one by (system, high key bits): each
+    // system's bodies end in its own slot range, in key order with ties in
+    // index order (the order of a sort of that system alone)
     const uint32_t slots = fb.slots, blocks = ceil_div(slots, kThreads);
     uint64_t *k = fb.skey, *ka = fb.skey_alt;
     uint32_t *v = fb.sidx, *va = fb.sidx_alt;
     hipLaunchKernelGGL(k_forest_keys, dim3(blocks), dim3(kThreads), 0, s, slots,
                        (const uint16_t *)fb.fb_sys, sys, x, y, z, m, fb, k, v);
-    if (radix_sort_u64_u32(k, v, ka, va, slots, 3 * kKeyLevels, fb.radix_ctl, status, s)) {
+    if (radix_sort_u64_u32(k, v, ka, va, slots, kForestLowBits, fb.radix_ctl, status, s)) {
         std::swap(k, ka);
         std::swap(v, va);
     }
@@ -1502,7 +1511,8 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     int sys_bits = 1;
     while (sys_bits < 16 && (1u << sys_bits) < nsys) ++sys_bits;
     // its own control words: a control buffer must always see the same pass count
-    if (radix_sort_u64_u32(k, v, ka, va, slots, sys_bits, fb.radix_ctl_sys, status, s)) {
+    if (radix_sort_u64_u32(k, v, ka, va, slots, 3 * kKeyLevels - kForestLowBits + sys_bits,
+                           fb.radix_ctl_sys, status, s)) {
         std::swap(k, ka);
         std::swap(v, va);
     }
