@@ -46,6 +46,36 @@ def test_ensemble_bh_forces_match_one_system(gpu):
 
 
 @pytest.mark.gpu
+def test_ensemble_many_systems_match_one_system(gpu):
+    # 600 systems: the sort that gathers each system's slots takes 10 system
+    # bits (5 radix passes, its own control words); two builds in a row reuse
+    # both sorts' control words
+    rng = np.random.default_rng(5)
+    sizes = [int(v) for v in rng.integers(1, 200, 600)]
+    sizes[0], sizes[1] = 1, 2
+    parts, st, dt = _ensemble(gpu, sizes, 300)
+    with gpu.Ensemble(sizes, theta=0.6) as e:
+        e.set_tuning(bh_threshold=0, waves=4096)  # every system Barnes-Hut
+        e.set_state(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], st["m"], dt)
+        e.forces()
+        first = e.accel()
+        e.forces()
+        ax, ay, az = e.accel()
+    for a, b in zip(first, (ax, ay, az)):
+        assert np.array_equal(a, b)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for s in list(range(0, 600, 15)) + [599]:
+        ctx = _single(gpu, parts[s], 0.6)
+        ctx.bh_build()
+        ctx.bh_forces()
+        rx, ry, rz = ctx.accel()
+        ctx.close()
+        sl = slice(off[s], off[s + 1])
+        assert np.array_equal(ax[sl], rx) and np.array_equal(ay[sl], ry) and \
+            np.array_equal(az[sl], rz), f"system {s} (n={sizes[s]})"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("waves", [1, 16, 4096])
 def test_ensemble_few_waves_match_one_system(gpu, waves):
     """Each persistent wave walks many groups in turn when there are few
