@@ -65,6 +65,8 @@ struct alignas(128) TreeHdr {
 //  - 64 / 96 and 64 / 192: slower on both; 96 / 192 and 128 / 256: +26 % and
 //    +22 % on the disk;
 //  - 32 / 128 (this): disk -0.4 %, Plummer -2.2 %, with 8 % fewer interactions;
+//  - against 32 / 128: 24 / 96 disk +23 %, Plummer +9 %; 40 / 160 disk +13 %,
+//    Plummer +16 %; 32 / 96 disk +4 %, Plummer -0.2 %;
 //  - packing without a limit: the Plummer traversal ten times slower, because
 //    siblings around a dense non-group sibling make a box that contains it and
 //    the walk must resolve that cell to its leaves.
