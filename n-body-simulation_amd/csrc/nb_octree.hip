@@ -1381,6 +1381,7 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const uint32_t *sorted,  // may be fb.sidx itself: each slot is read, then rewritten
     uint32_t *items, uint32_t *status) {
     __shared__ uint32_t wsum[kForestWaves + 1];
+    __shared__ uint64_t kwin[kLinkBack + kForestThreads];  // topology: keys of a chunk
     __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
     __shared__ uint32_t lvl_c[kMaxDepth];      // fill cursors
     __shared__ int lcp_max;
@@ -1445,10 +1446,24 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     // 6. topology
     for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
     __syncthreads();
-    const KeyView kv{skey, klo};
-    for (uint32_t i = t; i < n; i += kForestThreads)
-        link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node,
-                body_parent, int_local, nullptr, 0u, int_first, n, n);
+    // a chunk of kForestThreads bodies at a time, with the chunk's keys and
+    // kLinkBack before them in LDS: the cell-start searches walk back from
+    // each body, mostly inside that window (outside it they read global memory)
+    for (uint32_t c0 = 0; c0 < n; c0 += kForestThreads) {
+        const uint32_t w0 = c0 > kLinkBack ? c0 - kLinkBack : 0u;
+        const uint32_t w1 = min(n, c0 + (uint32_t)kForestThreads);
+        __syncthreads();  // the previous chunk is done with the window
+        for (uint32_t p = w0 + t; p < w1; p += kForestThreads) kwin[p - w0] = skey[p];
+        __syncthreads();
+        KeyView kv{skey, klo};
+        kv.win = kwin;
+        kv.w0 = w0;
+        kv.w1 = w1;
+        const uint32_t i = c0 + t;
+        if (i < n)
+            link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node,
+                    body_parent, int_local, nullptr, 0u, int_first, n, n);
+    }
     __syncthreads();
     // 7. centre of mass, level by level: every cell is local to this workgroup
     for (uint32_t i = t; i < n; i += kForestThreads)
