@@ -1182,16 +1182,22 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__
                                  const uint8_t *__restrict__ int_depth, double G, int write_f64,
                                  int write_f32, double4 *cm, double *edge, NodeF32 *rec) {
     const int32_t r = ref[j];
+    // both candidates are loaded unconditionally (an index that does not
+    // apply reads entry 0, which every system has), then one is used: a load
+    // under a branch would be waited for as soon as it is issued
+    const bool cell = r >= 0, body = r < kRefEmpty;
+    const uint32_t ci = cell ? (uint32_t)r : 0u;
+    const InternalSums s = sums[ci];
+    const int depth = (int)int_depth[ci];
+    const double4 p = body4[body ? body_of_ref(r) : 0u];
     double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
     double e = 0.0;
-    if (r >= 0) {
+    if (cell) {
         // the reference divides numerator by mass at every visit
         // (BarnesHutAlgorithm.cpp:351-353); the quotient is the same
-        const InternalSums s = sums[r];
         c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
-        e = ldexp(root_edge, -(int)int_depth[r]);
-    } else if (r != kRefEmpty) {
-        const double4 p = body4[body_of_ref(r)];
+        e = ldexp(root_edge, -depth);
+    } else if (body) {
         const double mb = p.w;
         c = make_double4((p.x * mb) / mb, (p.y * mb) / mb, (p.z * mb) / mb, mb);
     }
