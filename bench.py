@@ -1,9 +1,18 @@
 """Throughput of the n-body hot path on one MI355X.
 
-Default workload (the reference's Barnes-Hut configuration used for the
-headline metric): star + disk, N = 1,048,576, theta = 0.5, dt = 1 h, FP32 pair
-forces with FP64 accumulation, FP64 octree sums and integrator.  A step is one
-iteration of the reference's loop: kick-drift, octree build, forces, kick.
+Default workload: Barnes-Hut on a Plummer sphere of N = 4,194,304 equal
+masses (scale radius 1, G = 1 in model units), dt = 1e-3, FP32 pair forces
+with FP64 accumulation, FP64 octree sums and integrator, at the largest
+opening angle theta whose median relative force error against the FP64 direct
+sum is at most 1e-4 (chosen on the initial state from a fixed candidate list,
+then checked again after the timed steps).  A step is one iteration of the
+reference's loop (BarnesHutAlgorithm.cpp:149-277): kick-drift, octree build,
+forces, kick.  The line also carries the theta sweep {0.3, 0.5, 0.7, 1.0} of
+the same input.
+
+Every line states whether its error meets the 1e-4 bound (config.gate_met);
+a line that misses it reports value null and the measured rate as
+config.ungated_value.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -28,6 +37,12 @@ HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention, reference README.md:158
 MLP_FLOP_PER_PAIR = 18432  # 2 * (13*64 + 64*64 + 64*64 + 64*3) per network evaluation
 ERR_SAMPLE = 16384
+ERR_GATE = 1e-4
+# opening angles tried for the default workload, largest first; the first whose
+# median error on the initial state is at most ERR_GATE * THETA_MARGIN is used
+THETA_CANDIDATES = (0.6, 0.55, 0.5, 0.45, 0.42, 0.4, 0.38, 0.36, 0.34, 0.32, 0.3, 0.25, 0.2)
+THETA_MARGIN = 0.95
+THETA_SWEEP = (0.3, 0.5, 0.7, 1.0)
 
 # Learned correction: training set (Plummer snapshots, accepted (cell, body)
 # pairs of a per-body walk at theta 1.0, targets from the exact cell sums) and
@@ -49,7 +64,7 @@ CPU_SAMPLE_S = 10.0
 # (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0):
 # 565,982 KiB x 2 read + 134,447 KiB written.
 TRAFFIC_BYTES = {
-    "bh_disk_1m": 1.297e9,
+    ("bh_disk_1m", 0.5): 1.297e9,
 }
 
 WORKLOADS = {
@@ -57,7 +72,8 @@ WORKLOADS = {
     "bh_disk_1m": ("disk", 1 << 20, 0x4E424F44590001, "BarnesHut", 0.5, 1.0 / 24, nb.PREC_FP32C),
     "naive_disk_16k": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP64),
     "naive_disk_16k_fp32": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP32C),
-    "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", 0.5, 1e-3, nb.PREC_FP32C),
+    # theta None: the largest candidate meeting the error bound (choose_theta)
+    "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", None, 1e-3, nb.PREC_FP32C),
     # theta 1.0 with the learned far-field correction on every accepted cell
     "bh_plummer_4m_mlp": ("plummer", 1 << 22, 0x4E424F44590003, "BarnesHut", 1.0, 1e-3,
                           nb.PREC_FP32C),
@@ -104,7 +120,7 @@ def cpu_baseline(d, algorithm, theta, dt):
                      iters * dt, theta=theta)
     secs = r["loop_ms"] / 1000.0
     return {"value": n * r["steps"] / secs, "unit": "body-steps/s", "cores": threads,
-            "kind": "port",
+            "cpu": cpu_model(), "kind": "port",
             "sample": f"{r['steps']} loop iteration(s) (kick-drift, forces"
                       f"{', octree build' if algorithm == 'BarnesHut' else ''}, kick) of the same "
                       f"n={n} input, FP64, t=0 force untimed; {secs:.1f} s"}
@@ -201,20 +217,75 @@ def snapshot_cost(reps=40, rounds=3):
                       **{k: round(float(np.median(v)), 3) for k, v in ms.items()}}), flush=True)
 
 
+def cpu_model():
+    """The host CPU's model name (first /proc/cpuinfo entry)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def gated(value, err):
+    """(value, gate_met): a rate counts only at median error <= ERR_GATE."""
+    met = err is not None and err <= ERR_GATE
+    return (round(value, 1) if met else None), met
+
+
+def err_targets(n):
+    """The sampled error targets: every body up to 65,536, else a seeded
+    sample of ERR_SAMPLE."""
+    if n <= 65536:
+        return np.arange(n, dtype=np.uint32)
+    return np.sort(np.random.default_rng(1).choice(n, ERR_SAMPLE, replace=False)).astype(np.uint32)
+
+
+def choose_theta(ctx, d, dt):
+    """The largest candidate opening angle whose median error on the initial
+    state is at most ERR_GATE * THETA_MARGIN (one build + traversal each,
+    against one FP64 direct sum over the sampled targets)."""
+    n = d["m"].size
+    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    t = err_targets(n)
+    rx, ry, rz = ctx.forces_direct_targets(t)
+    den = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    trials = []
+    for th in THETA_CANDIDATES:
+        ctx.set_params(theta=th)
+        ctx.bh_build()
+        ax, ay, az = ctx.bh_forces()
+        e = np.sqrt((ax[t] - rx) ** 2 + (ay[t] - ry) ** 2 + (az[t] - rz) ** 2) / den
+        err = float(np.median(e))
+        trials.append({"theta": th, "median_rel_err": err})
+        log(f"theta {th}: initial-state median rel err {err:.3e}")
+        if err <= ERR_GATE * THETA_MARGIN:
+            return th, trials
+    raise RuntimeError(f"no candidate opening angle meets {ERR_GATE}: {trials}")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="bh_disk_1m", choices=sorted(WORKLOADS) + ["ensemble_512"])
+    ap.add_argument("--workload", default="bh_plummer_4m",
+                    choices=sorted(WORKLOADS) + ["ensemble_512"])
+    ap.add_argument("--theta", type=float, default=None,
+                    help="opening angle (default: the workload's; for bh_plummer_4m the largest "
+                         "candidate meeting the error bound)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--theta-sweep", default="",
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the theta sweep of the Barnes-Hut workloads")
+    ap.add_argument("--theta-sweep", default=",".join(str(t) for t in THETA_SWEEP),
                     help="comma-separated opening angles: also time the Barnes-Hut workload at "
                          "each (same input), reported under config.theta_sweep")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 256 MiB device copy (HBM counter calibration)")
     ap.add_argument("--snapshot-cost", action="store_true",
-                    help="only time a visualised step of the default workload: synchronous "
+                    help="only time a visualised step of the 1M-body disk workload: synchronous "
                          "reads against the asynchronous snapshot")
     args = ap.parse_args()
     if args.snapshot_cost:
@@ -231,16 +302,24 @@ def main():
         run_ensemble(args)
         return
     kind, n, seed, algorithm, theta, dt, prec = WORKLOADS[args.workload]
+    if args.theta is not None:
+        theta = args.theta
     mlp = args.workload.endswith("_mlp")
     log(f"device: {nb.device_name()}")
     net, train_info = (train_correction() if mlp else (None, None))
     log(f"workload {args.workload}: generating {kind} n={n} ...")
     d = gen(kind, n, seed)
-    ctx = nb.Context(n, theta=theta, precision=prec)
+    ctx = nb.Context(n, theta=theta if theta is not None else 0.5, precision=prec)
+    theta_search = None
+    if algorithm == "BarnesHut" and theta is None:
+        theta, theta_search = choose_theta(ctx, d, dt)
+        ctx.set_params(theta=theta)
+        log(f"opening angle: {theta}")
     if net is not None:
         ctx.set_correction(net, 0.0)
     secs, phases, err, sample, inter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
-    value = n * args.steps / secs
+    raw_value = n * args.steps / secs
+    value, gate_met = gated(raw_value, err)
 
     # roofline of the dominant kernel: the force kernel
     accel_ms = phases["Acceleration Kernel Time"] / args.steps
@@ -250,31 +329,33 @@ def main():
         peak, bound, kernel = BF16_PEAK_TFLOPS, "mfma", "k_bh_group<corrected>"
     elif algorithm == "BarnesHut":
         flops = FLOP_PER_INTERACTION * float(inter.sum())
-        peak, bound, kernel = FP32_PEAK_TFLOPS, "mfma", "k_bh_group"
+        peak, bound, kernel = FP32_PEAK_TFLOPS, "valu", "k_bh_group"
     else:
         flops = FLOP_PER_INTERACTION * float(n) * float(n)
         peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
-        bound, kernel = "mfma", "k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"
+        bound, kernel = "valu", "k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"
     achieved = flops / (accel_ms * 1e-3) / 1e12
-    # compute-bound; "pipe" says which unit does the arithmetic (the pair
-    # maths runs on the vector ALU, the network on the matrix cores)
-    roof = {"bound": bound, "pipe": "mfma" if mlp else "valu", "achieved": round(achieved, 3),
-            "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get(args.workload),
+    # compute-bound: "valu" is the vector ALU (the pair maths), "mfma" the
+    # matrix cores (the network); the peak is that unit's datasheet figure
+    roof = {"bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get((args.workload, theta)),
             "kernel": kernel, "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops}
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1,
+        "metric": METRIC, "value": value, "unit": "body-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": ("bf16+f32+f64" if mlp else "f32+f64") if prec == nb.PREC_FP32C else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
-                   "dt_days": dt, "median_rel_err": err, "err_sample": sample,
+                   "dt": dt, "median_rel_err": err, "err_sample": sample, "gate": ERR_GATE,
+                   "gate_met": gate_met, "ungated_value": round(raw_value, 1),
                    "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
                    "mean_interactions": None if inter is None else float(inter.mean())},
         "roofline": roof,
     }
+    if theta_search is not None:
+        out["config"]["theta_search"] = theta_search
     if mlp:
         # the same input with plain Barnes-Hut at the comparison opening angle
         ctx.set_correction(None)
@@ -285,21 +366,23 @@ def main():
             train_info, tau=0.0, network_evals_per_step=pairs,
             plain_theta=MLP_PLAIN_THETA, plain_value=round(pvalue, 1),
             plain_median_rel_err=perr, plain_mean_interactions=float(pinter.mean()),
-            speedup_vs_plain=round(value / pvalue, 4))
+            speedup_vs_plain=round(raw_value / pvalue, 4))
         log(f"plain theta={MLP_PLAIN_THETA}: {pvalue:.4g} body-steps/s, err {perr:.3e}; "
-            f"corrected: {value:.4g}, err {err:.3e}")
-    if args.theta_sweep and algorithm == "BarnesHut" and not mlp:
+            f"corrected: {raw_value:.4g}, err {err:.3e}")
+    if algorithm == "BarnesHut" and not mlp and not args.no_sweep and args.theta_sweep:
         sweep = []
         for th in (float(v) for v in args.theta_sweep.split(",")):
             ctx.set_params(theta=th)
             ssecs, sph, serr, _, sinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
             sms = sph["Acceleration Kernel Time"] / args.steps
             sflop = FLOP_PER_INTERACTION * float(sinter.sum())
-            sweep.append({"theta": th, "value": round(n * args.steps / ssecs, 1),
-                          "median_rel_err": serr, "mean_interactions": float(sinter.mean()),
-                          "force_ms": round(sms, 4),
+            svalue = n * args.steps / ssecs
+            sweep.append({"theta": th, "value": gated(svalue, serr)[0],
+                          "ungated_value": round(svalue, 1), "median_rel_err": serr,
+                          "gate_met": gated(svalue, serr)[1],
+                          "mean_interactions": float(sinter.mean()), "force_ms": round(sms, 4),
                           "frac": round(sflop / (sms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4)})
-            log(f"theta {th}: {sweep[-1]['value']:.4g} body-steps/s, err {serr:.3e}")
+            log(f"theta {th}: {svalue:.4g} body-steps/s, err {serr:.3e}")
         out["config"]["theta_sweep"] = sweep
     ctx.close()
     if not args.no_cpu_baseline:
@@ -360,12 +443,14 @@ def run_ensemble(args):
     accel_ms = phases["Acceleration Kernel Time"] / args.steps
     flops = FLOP_PER_INTERACTION * float(inter.sum())
     achieved = flops / (accel_ms * 1e-3) / 1e12
+    gvalue, gate_met = gated(value, err)
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "body-steps/s", "n_gpus": 1,
+        "metric": METRIC, "value": gvalue, "unit": "body-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": "ensemble_512", "systems": c["nsys"], "bodies": n,
+                   "gate": ERR_GATE, "gate_met": gate_met, "ungated_value": round(value, 1),
                    "sizes": f"floor(2^U[{c['lo_log2']},{c['hi_log2']}])",
                    "distribution": "alternating star+disk (dt 1 h) and plummer (dt 1e-3)",
                    "theta": c["theta"], "tuning": tune, "tuned_ms_per_step": round(tune_ms, 4),
@@ -373,10 +458,11 @@ def run_ensemble(args):
                    "err_sample": int(errs.size),
                    "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
                    "mean_interactions": float(inter.mean())},
-        "roofline": {"bound": "mfma", "pipe": "valu", "achieved": round(achieved, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP32_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                     "traffic": TRAFFIC_BYTES.get("ensemble_512"), "kernel": "k_ens_forces",
+                     "traffic": TRAFFIC_BYTES.get(("ensemble_512", c["theta"])),
+                     "kernel": "k_ens_forces",
                      "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops},
     }
     e.close()
@@ -406,7 +492,7 @@ def ensemble_cpu_baseline(sizes, st, dts, uses_bh, off):
         if ms >= CPU_SAMPLE_S * 1e3:
             break
     return {"value": bodies / (ms * 1e-3), "unit": "body-steps/s", "cores": orc.get_threads(),
-            "kind": "port",
+            "cpu": cpu_model(), "kind": "port",
             "sample": f"1 loop iteration of each of {used} systems in random order "
                       f"({bodies} body-steps), FP64, naive below the tuned threshold and "
                       f"Barnes-Hut above; {ms / 1e3:.1f} s"}

@@ -158,7 +158,10 @@ enum {
     NB_PHASE_ACCEL = 5,       /* "Acceleration Kernel Time" */
     NB_PHASE_LF1 = 6,         /* "Leapfrog Part 1" */
     NB_PHASE_LF2 = 7,         /* "Leapfrog Part 2" */
-    NB_PHASE_COUNT = 8
+    /* the two parts of NB_PHASE_KEYS_SORT */
+    NB_PHASE_KEYS = 8,        /* octant keys of every body ("Build octree to level") */
+    NB_PHASE_SORT = 9,        /* radix sort of the keys ("Sort bodies for subtrees") */
+    NB_PHASE_COUNT = 10
 };
 /* GPU time (HIP events) of each phase summed over the last nb_* call;
  * ms must hold NB_PHASE_COUNT doubles. */

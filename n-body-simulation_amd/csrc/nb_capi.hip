@@ -142,6 +142,8 @@ int status_to_code(nb_ctx *c, uint32_t st) {
     if (st & nb::kFlagStack) return fail(c, NB_E_STACK_OVERFLOW, "traversal stack exhausted");
     if (st & nb::kFlagSortWait)
         return fail(c, NB_E_HIP, "radix sort: a look-back wait did not end");
+    if (st & nb::kFlagSortSize)
+        return fail(c, NB_E_INVALID_ARG, "radix sort: 2^30 bodies or more, nothing sorted");
     return NB_OK;
 }
 
@@ -165,61 +167,98 @@ uint32_t exact_stack_cap(const nb_ctx *c) {
     return std::max<uint32_t>(ref, 1 + 7 * nb::kMaxDepth + 8);
 }
 
-int alloc_tree(nb_ctx *c) {
-    TreeBufs &t = c->tree;
+// Internal-cell capacity for storage_size_param * n nodes, in 64 bits; the
+// group traversal packs an internal-cell index into 26 bits.
+bool tree_capacity(uint32_t storage_size_param, uint32_t n, uint32_t *icap) {
+    const uint64_t storage = (uint64_t)std::max<uint32_t>(storage_size_param, 1) * n;
+    const uint64_t cells = std::max<uint64_t>(1, storage / 8);
+    if (cells >= (1u << 26)) return false;
+    *icap = (uint32_t)cells;
+    return true;
+}
+
+// Every device buffer of an octree (null pointers are skipped).
+void tree_free(nb_ctx *c, TreeBufs &t) {
+    void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
+                    t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
+                    t.int_lvl, t.int_count, t.int_sums, t.int_local, t.int_first, t.groups,
+                    t.group_size, t.group_slot, t.body4, t.gq, t.spanning, t.cross, t.ref,
+                    t.cm, t.edge, t.rec, t.radix_ctl, t.scan_part};
+    for (void *b : bufs) dev_free(c, b);
+    t = TreeBufs();
+}
+
+// Allocates the octree buffers of storage_size_param * n nodes into t (which
+// the caller passes empty); on failure whatever was allocated is freed again
+// and t is left empty.
+int alloc_tree_into(nb_ctx *c, TreeBufs &t, uint32_t storage_size_param);
+
+int alloc_tree(nb_ctx *c) { return alloc_tree_into(c, c->tree, c->p.storage_size_param); }
+
+int alloc_tree_into(nb_ctx *c, TreeBufs &t, uint32_t storage_size_param) {
     const uint32_t n = c->n;
-    t.n = n;
-    const uint64_t storage = (uint64_t)std::max<uint32_t>(c->p.storage_size_param, 1) * n;
-    t.icap = (uint32_t)std::max<uint64_t>(1, storage / 8);
-    t.mcap = 1 + 8ull * t.icap;
-    // the group traversal packs an internal-cell index into 26 bits
-    if (t.icap >= (1u << 26))
+    uint32_t icap = 0;
+    if (!tree_capacity(storage_size_param, n, &icap))
         return fail(c, NB_E_INVALID_ARG, "storage_size_param * N exceeds 2^29 octree nodes");
-    int rc;
-    if ((rc = dev_alloc(c, &t.hdr, 1))) return rc;
-    if ((rc = dev_alloc(c, &t.aabb_part, 6ull * nb::kAabbBlocks))) return rc;
-    if ((rc = dev_alloc(c, &t.skey, n))) return rc;
-    if ((rc = dev_alloc(c, &t.skey_alt, n))) return rc;
-    if ((rc = dev_alloc(c, &t.klo, n))) return rc;
-    if ((rc = dev_alloc(c, &t.sidx, n))) return rc;
-    if ((rc = dev_alloc(c, &t.sidx_alt, n))) return rc;
-    if ((rc = dev_alloc(c, &t.lcp, n))) return rc;
-    if ((rc = dev_alloc(c, &t.chain, n))) return rc;
-    if ((rc = dev_alloc(c, &t.body_parent, n))) return rc;
-    if ((rc = dev_alloc(c, &t.int_depth, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_parent, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_node, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_lvl, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_count, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_sums, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_local, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.int_first, t.icap))) return rc;
-    if ((rc = dev_alloc(c, &t.groups, n))) return rc;
-    if ((rc = dev_alloc(c, &t.group_size, n))) return rc;
-    if ((rc = dev_alloc(c, &t.group_slot, n))) return rc;
-    if ((rc = dev_alloc(c, &t.body4, n))) return rc;
-    if ((rc = dev_alloc(c, &t.gq, TreeBufs::kGroupQueueWords))) return rc;
+    t.n = n;
+    t.icap = icap;
+    t.mcap = 1 + 8ull * t.icap;
+    int rc = NB_OK;
+    auto undo = [&](int code) {
+        tree_free(c, t);
+        return code;
+    };
+#define NB_TREE_ALLOC(ptr, count) \
+    if ((rc = dev_alloc(c, &(ptr), (count)))) return undo(rc)
+    NB_TREE_ALLOC(t.hdr, 1);
+    NB_TREE_ALLOC(t.aabb_part, 6ull * nb::kAabbBlocks);
+    NB_TREE_ALLOC(t.skey, n);
+    NB_TREE_ALLOC(t.skey_alt, n);
+    NB_TREE_ALLOC(t.klo, n);
+    NB_TREE_ALLOC(t.sidx, n);
+    NB_TREE_ALLOC(t.sidx_alt, n);
+    NB_TREE_ALLOC(t.lcp, n);
+    NB_TREE_ALLOC(t.chain, n);
+    NB_TREE_ALLOC(t.body_parent, n);
+    NB_TREE_ALLOC(t.int_depth, t.icap);
+    NB_TREE_ALLOC(t.int_parent, t.icap);
+    NB_TREE_ALLOC(t.int_node, t.icap);
+    NB_TREE_ALLOC(t.int_lvl, t.icap);
+    NB_TREE_ALLOC(t.int_count, t.icap);
+    NB_TREE_ALLOC(t.int_sums, t.icap);
+    NB_TREE_ALLOC(t.int_local, t.icap);
+    NB_TREE_ALLOC(t.int_first, t.icap);
+    NB_TREE_ALLOC(t.groups, n);
+    NB_TREE_ALLOC(t.group_size, n);
+    NB_TREE_ALLOC(t.group_slot, n);
+    NB_TREE_ALLOC(t.body4, n);
+    NB_TREE_ALLOC(t.gq, TreeBufs::kGroupQueueWords);
     // a cell crossing a COM block boundary holds that boundary's two bodies:
     // at most one per depth (kMaxDepth) and block
     t.spanning_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComBlock) + 1);
-    if ((rc = dev_alloc(c, &t.spanning, t.spanning_cap))) return rc;
+    NB_TREE_ALLOC(t.spanning, t.spanning_cap);
     // a super-block owns at most one crossing cell per depth (the cells holding its end)
     t.cross_cap = (uint32_t)nb::kMaxDepth * (nb::ceil_div(n, nb::kComSuper) + 1);
-    if ((rc = dev_alloc(c, &t.cross, t.cross_cap))) return rc;
-    if ((rc = dev_alloc(c, &t.ref, t.mcap))) return rc;
-    if ((rc = dev_alloc(c, &t.cm, t.mcap))) return rc;
-    if ((rc = dev_alloc(c, &t.edge, t.mcap))) return rc;
-    if ((rc = dev_alloc(c, &t.rec, t.mcap))) return rc;
+    NB_TREE_ALLOC(t.cross, t.cross_cap);
+    NB_TREE_ALLOC(t.ref, t.mcap);
+    NB_TREE_ALLOC(t.cm, t.mcap);
+    NB_TREE_ALLOC(t.edge, t.mcap);
+    NB_TREE_ALLOC(t.rec, t.mcap);
     const uint32_t ctl_words = nb::radix_ctl_words(n);
-    if ((rc = dev_alloc(c, &t.radix_ctl, ctl_words))) return rc;
+    NB_TREE_ALLOC(t.radix_ctl, ctl_words);
     t.scan_part_cap = nb::scan_scratch_words(n);
-    if ((rc = dev_alloc(c, &t.scan_part, t.scan_part_cap))) return rc;
-    NB_HIP(c, hipMemsetAsync(t.hdr, 0, sizeof(TreeHdr), c->stream));
-    NB_HIP(c, hipMemsetAsync(t.radix_ctl, 0, 4ull * ctl_words, c->stream));
-    // sort scratch starts as valid data (body index 0), so that nothing the
-    // sort could leave behind is out of range
-    NB_HIP(c, hipMemsetAsync(t.skey_alt, 0, 8ull * n, c->stream));
-    NB_HIP(c, hipMemsetAsync(t.sidx_alt, 0, 4ull * n, c->stream));
+    NB_TREE_ALLOC(t.scan_part, t.scan_part_cap);
+#undef NB_TREE_ALLOC
+    const hipError_t e[] = {
+        hipMemsetAsync(t.hdr, 0, sizeof(TreeHdr), c->stream),
+        hipMemsetAsync(t.radix_ctl, 0, 4ull * ctl_words, c->stream),
+        // sort scratch starts as valid data (body index 0), so that nothing
+        // the sort could leave behind is out of range
+        hipMemsetAsync(t.skey_alt, 0, 8ull * n, c->stream),
+        hipMemsetAsync(t.sidx_alt, 0, 4ull * n, c->stream)};
+    for (hipError_t err : e)
+        if (err != hipSuccess)
+            return undo(fail(c, NB_E_HIP, std::string("octree buffers: ") + hipGetErrorString(err)));
     return NB_OK;
 }
 
@@ -235,7 +274,12 @@ int build_tree(nb_ctx *c) {
     nb::launch_aabb(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_AABB);
     mark_begin(c, NB_PHASE_KEYS_SORT);
+    mark_begin(c, NB_PHASE_KEYS);
     nb::launch_keys(t, c->x, c->y, c->z, c->m, c->stream);
+    mark_end(c, NB_PHASE_KEYS);
+    mark_begin(c, NB_PHASE_SORT);
+    nb::launch_key_sort(t, c->stream);
+    mark_end(c, NB_PHASE_SORT);
     mark_end(c, NB_PHASE_KEYS_SORT);
     mark_begin(c, NB_PHASE_TOPOLOGY);
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
@@ -411,33 +455,38 @@ int nb_destroy(nb_ctx *c) {
 
 int nb_set_params(nb_ctx *c, const nb_params *p) {
     if (!c || !params_ok(p)) return fail(c, NB_E_INVALID_ARG, "invalid parameters");
-    const bool storage_changed = p->storage_size_param != c->p.storage_size_param;
-    c->p = *p;
-    if (storage_changed) {
-        // reallocate octree buffers for the new capacity
-        TreeBufs &t = c->tree;
-        void *bufs[] = {t.hdr, t.aabb_part, t.skey, t.skey_alt, t.klo, t.sidx, t.sidx_alt,
-                        t.lcp, t.chain, t.body_parent, t.int_depth, t.int_parent, t.int_node,
-                        t.int_lvl, t.int_count, t.int_sums, t.int_local,
-                        t.int_first, t.groups, t.group_size, t.group_slot, t.body4, t.gq,
-                        t.spanning, t.cross,
-                        t.ref,
-                        t.cm, t.edge, t.rec,
-                        t.radix_ctl, t.scan_part};
-        (void)hipStreamSynchronize(c->stream);
-        for (void *b : bufs) dev_free(c, b);
+    if (p->storage_size_param != c->p.storage_size_param) {
+        // The new octree storage is allocated first; on any failure (capacity
+        // beyond 2^29 nodes, out of memory) the old parameters and buffers
+        // stay as they were and the context remains usable.
+        TreeBufs fresh;
+        int rc = alloc_tree_into(c, fresh, p->storage_size_param);
+        if (rc) return rc;
+        double *quad = nullptr;
+        nb::MlpCell *mcell = nullptr;
+        if (c->corr) {
+            if ((rc = dev_alloc(c, &quad, 6ull * fresh.icap)) ||
+                (rc = dev_alloc(c, &mcell, fresh.icap))) {
+                dev_free(c, quad);
+                tree_free(c, fresh);
+                return rc;
+            }
+        }
+        if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            dev_free(c, quad);
+            dev_free(c, mcell);
+            tree_free(c, fresh);
+            return fail(c, NB_E_HIP, "nb_set_params: stream synchronisation failed");
+        }
+        tree_free(c, c->tree);
         dev_free(c, c->quad);
         dev_free(c, c->mcell);
-        c->quad = nullptr;
-        c->mcell = nullptr;
+        c->tree = fresh;
+        c->quad = quad;
+        c->mcell = mcell;
         c->quad_valid = false;
-        c->tree = TreeBufs();
-        c->tree_valid = false;
-        int rc = alloc_tree(c);
-        if (rc) return rc;
-        if (c->corr && (rc = ensure_quad_buffers(c))) return rc;
-        NB_HIP(c, hipStreamSynchronize(c->stream));
     }
+    c->p = *p;
     c->tree_valid = false;  // opening radii depend on theta
     return NB_OK;
 }

@@ -134,6 +134,8 @@ int finish(nb_ensemble *e) {
     if (st & nb::kFlagZeroMass) return efail(e, NB_E_ZERO_MASS, "zero-mass octree cell" + where);
     if (st & nb::kFlagStack) return efail(e, NB_E_STACK_OVERFLOW, "traversal stack exhausted" + where);
     if (st & nb::kFlagSortWait) return efail(e, NB_E_HIP, "radix sort: a look-back wait did not end");
+    if (st & nb::kFlagSortSize)
+        return efail(e, NB_E_INVALID_ARG, "radix sort: 2^30 body slots or more, nothing sorted");
     return efail(e, NB_E_HIP, "unknown device status " + std::to_string(st));
 }
 
@@ -243,8 +245,10 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         EnsSystem &es = e->sys_host[s];
         es.body_off = (uint32_t)bo;
         es.n = sizes[s];
-        es.icap = (uint32_t)std::max<uint64_t>(1, (uint64_t)p->storage_size_param * sizes[s] / 8);
-        if (es.icap >= (1u << 26)) return bail(NB_E_INVALID_ARG);
+        // checked in 64 bits before the narrowing
+        const uint64_t icap = std::max<uint64_t>(1, (uint64_t)p->storage_size_param * sizes[s] / 8);
+        if (icap >= (1u << 26)) return bail(NB_E_INVALID_ARG);
+        es.icap = (uint32_t)icap;
         es.fb_off = (uint32_t)fo;
         es.int_off = (uint32_t)io;
         es.node_off = no;
@@ -254,7 +258,8 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         io = up(io + es.icap);
         no = up(no + 1 + 8ull * es.icap);
     }
-    if (io >= (1ull << 32) || fo >= (1ull << 32)) return bail(NB_E_INVALID_ARG);
+    // the forest-wide radix sorts take fewer than 2^30 body slots
+    if (io >= (1ull << 32) || fo >= (1ull << 30)) return bail(NB_E_INVALID_ARG);
     int rc;
     double **st[] = {&e->x, &e->y, &e->z, &e->vx, &e->vy, &e->vz, &e->vhx,
                      &e->vhy, &e->vhz, &e->ax, &e->ay, &e->az, &e->m};

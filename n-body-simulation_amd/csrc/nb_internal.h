@@ -30,6 +30,7 @@ enum : uint32_t {
     kFlagStack = 8u,
     kFlagAabb = 16u,
     kFlagSortWait = 32u,  // radix sort look-back safety net (never expected)
+    kFlagSortSize = 64u,  // radix sort asked for 2^30 keys or more: nothing sorted
 };
 
 // node reference encoding
@@ -201,25 +202,33 @@ struct ForestBufs {
 void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *total_dev,
                         uint32_t *scratch, uint32_t scratch_cap, hipStream_t s);
 uint32_t scan_scratch_words(uint32_t n);
+// Where a radix sort left its result.
+enum class SortEnd {
+    kInPlace,   // keys/vals (an even pass count, or n <= 1)
+    kInAlt,     // keys_alt/vals_alt (an odd pass count)
+    kRejected,  // n >= 2^30 or bits outside [1, 63]: nothing sorted, kFlagSortSize raised
+};
 // Stable LSD radix sort of (key, val) pairs on key bits [0, bits), bits <= 63,
-// n < 2^30.  Returns true if the result is in keys_alt/vals_alt (an odd pass
-// count), false if in keys/vals.  ctl holds
+// n < 2^30.  ctl holds
 // radix_ctl_words(n) words, zeroed at allocation and then owned by the sort
 // (one ctl per stream: sorts sharing it must not overlap); a failed look-back
 // wait raises kFlagSortWait in *flags.
-bool radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                        uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s);
+SortEnd radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                           uint32_t *vals_alt, uint32_t n, int bits, uint32_t *ctl,
+                           uint32_t *flags, hipStream_t s);
 uint32_t radix_blocks_for(uint32_t n);
 uint32_t radix_ctl_words(uint32_t n);
 
 // nb_octree.hip
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
                  hipStream_t s);
-// keys, sort, and t.body4 = {x, y, z, m} of every body (input order)
-// (the sorted keys and body indices end in t.skey/t.sidx: the buffers swap
-// with their scratch copies when the sort ends there)
+// keys and t.body4 = {x, y, z, m} of every body (input order)
 void launch_keys(TreeBufs &t, const double *x, const double *y, const double *z,
                  const double *m, hipStream_t s);
+// radix sort of the keys (the sorted keys and body indices end in
+// t.skey/t.sidx: the buffers swap with their scratch copies when the sort
+// ends there)
+void launch_key_sort(TreeBufs &t, hipStream_t s);
 void launch_topology(const TreeBufs &t, const double *x, const double *y, const double *z,
                      hipStream_t s);
 // Centre of mass with the node records and traversal groups.  f64: FP64

@@ -1523,7 +1523,9 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     uint32_t *v = fb.sidx, *va = fb.sidx_alt;
     hipLaunchKernelGGL(k_forest_keys, dim3(blocks), dim3(kThreads), 0, s, slots,
                        (const uint16_t *)fb.fb_sys, sys, x, y, z, m, fb, k, v);
-    if (radix_sort_u64_u32(k, v, ka, va, slots, kForestLowBits, fb.radix_ctl, status, s)) {
+    SortEnd end = radix_sort_u64_u32(k, v, ka, va, slots, kForestLowBits, fb.radix_ctl, status, s);
+    if (end == SortEnd::kRejected) return;  // reported in *status
+    if (end == SortEnd::kInAlt) {
         std::swap(k, ka);
         std::swap(v, va);
     }
@@ -1532,8 +1534,10 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     int sys_bits = 1;
     while (sys_bits < 16 && (1u << sys_bits) < nsys) ++sys_bits;
     // its own control words: a control buffer must always see the same pass count
-    if (radix_sort_u64_u32(k, v, ka, va, slots, 3 * kKeyLevels - kForestLowBits + sys_bits,
-                           fb.radix_ctl_sys, status, s)) {
+    end = radix_sort_u64_u32(k, v, ka, va, slots, 3 * kKeyLevels - kForestLowBits + sys_bits,
+                             fb.radix_ctl_sys, status, s);
+    if (end == SortEnd::kRejected) return;
+    if (end == SortEnd::kInAlt) {
         std::swap(k, ka);
         std::swap(v, va);
     }
@@ -1555,8 +1559,13 @@ void launch_keys(TreeBufs &t, const double *x, const double *y, const double *z,
                  const double *m, hipStream_t s) {
     hipLaunchKernelGGL(k_keys, dim3(ceil_div(t.n, kThreads)), dim3(kThreads), 0, s, x, y, z, m,
                        t.n, (const TreeHdr *)t.hdr, t.skey, t.sidx, t.body4);
+}
+
+void launch_key_sort(TreeBufs &t, hipStream_t s) {
+    // a rejected sort (2^30 bodies or more, which the context's capacity
+    // check already excludes) raises kFlagSortSize; the build reports it
     if (radix_sort_u64_u32(t.skey, t.sidx, t.skey_alt, t.sidx_alt, t.n, 3 * kKeyLevels,
-                           t.radix_ctl, &t.hdr->status, s)) {
+                           t.radix_ctl, &t.hdr->status, s) == SortEnd::kInAlt) {
         std::swap(t.skey, t.skey_alt);
         std::swap(t.sidx, t.sidx_alt);
     }

@@ -163,6 +163,11 @@ __device__ inline void status_store(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A sort that cannot run reports it in the caller's status word.
+__global__ void k_raise_flag(uint32_t *flags, uint32_t f) {
+    if (threadIdx.x == 0) atomicOr(flags, f);
+}
+
 // Totals of the lowest digit of the keys: total[digit].  Also clears the
 // status plane of the first pass, plane0[0, plane_words): the previous sort's
 // last pass used it when the pass count is odd.
@@ -390,9 +395,16 @@ uint32_t radix_blocks_for(uint32_t n) { return ceil_div(n ? n : 1, kRadixTile); 
 
 uint32_t radix_ctl_words(uint32_t n) { return kCtlStatus + 2u * kBins * radix_blocks_for(n); }
 
-bool radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint32_t *vals_alt,
-                        uint32_t n, int bits, uint32_t *ctl, uint32_t *flags, hipStream_t s) {
-    if (n <= 1 || bits <= 0 || bits > kDigitBits * kMaxPasses || n >= kStatAgg) return false;
+SortEnd radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
+                           uint32_t *vals_alt, uint32_t n, int bits, uint32_t *ctl,
+                           uint32_t *flags, hipStream_t s) {
+    if (n >= kStatAgg || bits <= 0 || bits > kDigitBits * kMaxPasses) {
+        // counts of 2^30 keys and more do not fit the status words: nothing
+        // is sorted, and the caller's status word says so
+        hipLaunchKernelGGL(k_raise_flag, dim3(1), dim3(64), 0, s, flags, kFlagSortSize);
+        return SortEnd::kRejected;
+    }
+    if (n <= 1) return SortEnd::kInPlace;
     const uint32_t nb = radix_blocks_for(n);
     const int passes = std::max((bits + kDigitBits - 1) / kDigitBits, kMinPasses);
     hipLaunchKernelGGL(k_radix_count, dim3(std::min(nb, kCountBlocks)), dim3(kCountThreads), 0, s,
@@ -405,7 +417,8 @@ bool radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt, uint
         uint64_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }
-    return (passes & 1) != 0;  // an odd count ends in the scratch buffers
+    // an odd count ends in the scratch buffers
+    return (passes & 1) ? SortEnd::kInAlt : SortEnd::kInPlace;
 }
 
 }  // namespace nb

@@ -15,8 +15,6 @@ int configuration::naive_algorithm::blockSize = 64;
 int configuration::naive_algorithm::optimization_stage = 2;
 
 d_type::int_t configuration::barnes_hut_algorithm::storageSizeParameter = 0;
-int configuration::barnes_hut_algorithm::storageSizeParam = 16;
-int configuration::barnes_hut_algorithm::stackSizeParam = 16;
 int configuration::barnes_hut_algorithm::AABBWorkItemCount = 1024;
 int configuration::barnes_hut_algorithm::octreeWorkItemCount = 640;
 int configuration::barnes_hut_algorithm::octreeTopWorkItemCount = 1024;
@@ -30,8 +28,6 @@ int configuration::barnes_hut_algorithm::workGroupSize = 64;
 void configuration::initializeConfigValues(d_type::int_t bodyCount, int storageSizeParam,
                                            int stackSizeParam) {
     numberOfBodies = bodyCount;
-    barnes_hut_algorithm::storageSizeParam = storageSizeParam;
-    barnes_hut_algorithm::stackSizeParam = stackSizeParam;
     barnes_hut_algorithm::storageSizeParameter = storageSizeParam * numberOfBodies;
     const d_type::int_t lg = (d_type::int_t)std::ceil(std::log2(bodyCount));
     if (bodyCount < 15000)

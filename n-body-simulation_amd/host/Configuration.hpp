@@ -33,8 +33,6 @@ extern double theta;
 extern int workGroupSize;
 extern d_type::int_t stackSize;
 extern d_type::int_t storageSizeParameter;
-extern int storageSizeParam;  // the multiplier behind storageSizeParameter
-extern int stackSizeParam;    // the multiplier behind stackSize
 extern int AABBWorkItemCount;
 extern int octreeWorkItemCount;
 extern int octreeTopWorkItemCount;

@@ -9,6 +9,7 @@
 #include <filesystem>
 #include <iostream>
 #include <stdexcept>
+#include <utility>
 
 #include "Parallel.hpp"
 
@@ -88,12 +89,40 @@ void nBodyAlgorithm::recordStepTimes(nb_ctx *ctx, int algorithm, bool loop) {
     }
     if (algorithm == NB_ALGO_BARNES_HUT) {
         timer.addTimeToSequence("Octree creation", ms[NB_PHASE_OCTREE]);
-        timer.addTimeToSequence("AABB creation", ms[NB_PHASE_AABB]);
-        timer.addTimeToSequence("Build octree", ms[NB_PHASE_KEYS_SORT] + ms[NB_PHASE_TOPOLOGY]);
-        timer.addTimeToSequence("Compute center of mass", ms[NB_PHASE_COM]);
+        recordBuildPhases(ms);
         timer.addTimeToSequence("Total Time", ms[NB_PHASE_OCTREE] + ms[NB_PHASE_ACCEL]);
     }
     timer.addTimeToSequence("Acceleration Kernel Time", ms[NB_PHASE_ACCEL]);
+}
+
+// The default (Subtrees) builder's sequences, ParallelOctreeTopDownSubtrees.cpp:76-92,
+// with this build's phases in their places: the octant keys descend every
+// body through the top levels ("Build octree to level"); the radix sort
+// places the bodies of each subtree together ("Sort bodies for subtrees");
+// the cell numbering and linking build all levels below ("Build subtrees").
+// No step prepares per-subtree counts, and the in-order body order is the
+// radix sort's output, so "Prepare subtrees" and "Sort bodies" hold 0.
+void nBodyAlgorithm::recordBuildPhases(const double *ms) {
+    timer.addTimeToSequence("AABB creation", ms[NB_PHASE_AABB]);
+    timer.addTimeToSequence("Build octree to level", ms[NB_PHASE_KEYS]);
+    timer.addTimeToSequence("Prepare subtrees", 0.0);
+    timer.addTimeToSequence("Sort bodies for subtrees", ms[NB_PHASE_SORT]);
+    timer.addTimeToSequence("Build subtrees", ms[NB_PHASE_TOPOLOGY]);
+    timer.addTimeToSequence("Compute center of mass", ms[NB_PHASE_COM]);
+    if (configuration::barnes_hut_algorithm::sortBodies) timer.addTimeToSequence("Sort bodies", 0.0);
+}
+
+// The library's storage and stack multipliers from the reference's scaled
+// sizes (the inverse of Configuration.cpp:24-33).
+static std::pair<uint32_t, uint32_t> librarySizeParams(d_type::int_t N) {
+    namespace bh = configuration::barnes_hut_algorithm;
+    const uint64_t n = std::max<d_type::int_t>(N, 1);
+    const uint64_t lg = std::max<uint64_t>(1, (uint64_t)std::ceil(std::log2((double)n)));
+    const uint64_t extra = N < 15000 ? 500 : 0;
+    const uint64_t storage = std::max<uint64_t>(1, (uint64_t)bh::storageSizeParameter / n);
+    const uint64_t stack = bh::stackSize > extra ? std::max<uint64_t>(1, (bh::stackSize - extra) / lg) : 1;
+    return {(uint32_t)std::min<uint64_t>(storage, UINT32_MAX),
+            (uint32_t)std::min<uint64_t>(stack, UINT32_MAX)};
 }
 
 void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int algorithm) {
@@ -117,8 +146,13 @@ void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int alg
     p.theta = configuration::barnes_hut_algorithm::theta;
     p.precision = (uint32_t)configuration::precision;
     p.sort_bodies = configuration::barnes_hut_algorithm::sortBodies ? 1 : 0;
-    p.storage_size_param = (uint32_t)configuration::barnes_hut_algorithm::storageSizeParam;
-    p.stack_size_param = (uint32_t)configuration::barnes_hut_algorithm::stackSizeParam;
+    // The reference keeps only the scaled sizes (Configuration.cpp:24-33):
+    // storageSizeParameter = storage multiplier * N, and stackSize =
+    // stack multiplier * ceil(log2 N) (+ 500 below 15000 bodies); the library
+    // takes the multipliers.
+    const std::pair<uint32_t, uint32_t> sizes = librarySizeParams(N);
+    p.storage_size_param = sizes.first;
+    p.stack_size_param = sizes.second;
     CtxGuard guard;
     nbCheck(nullptr, nb_create(&guard.ctx, N, &p), "nb_create");
     nb_ctx *ctx = guard.ctx;
@@ -138,11 +172,12 @@ void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int alg
     timer.addTimingSequence("Leapfrog Part 1");
     timer.addTimingSequence("Leapfrog Part 2");
     if (algorithm == NB_ALGO_BARNES_HUT) {
-        timer.addTimingSequence("Total Time");
-        timer.addTimingSequence("Octree creation");
-        timer.addTimingSequence("AABB creation");
-        timer.addTimingSequence("Compute center of mass");
-        timer.addTimingSequence("Build octree");
+        // BarnesHutAlgorithm.cpp:81-100 with the default builder
+        for (const char *key : {"Total Time", "Octree creation", "AABB creation",
+                                "Compute center of mass", "Build octree to level",
+                                "Prepare subtrees", "Sort bodies for subtrees", "Build subtrees"})
+            timer.addTimingSequence(key);
+        if (configuration::barnes_hut_algorithm::sortBodies) timer.addTimingSequence("Sort bodies");
     }
 
     std::vector<double> ax(N), ay(N), az(N);
@@ -153,12 +188,11 @@ void nBodyAlgorithm::runSimulation(const SimulationData &simulationData, int alg
     // initial accelerations (NaiveAlgorithm.cpp:86-113, BarnesHutAlgorithm.cpp:105-133)
     if (algorithm == NB_ALGO_BARNES_HUT) {
         nbCheck(ctx, nb_bh_build(ctx), "nb_bh_build");
-        const double octree = phase(ctx, NB_PHASE_OCTREE);
+        double build_ms[NB_PHASE_COUNT];
+        nbCheck(ctx, nb_phase_ms(ctx, build_ms), "nb_phase_ms");
+        const double octree = build_ms[NB_PHASE_OCTREE];
         timer.addTimeToSequence("Octree creation", octree);
-        timer.addTimeToSequence("AABB creation", phase(ctx, NB_PHASE_AABB));
-        timer.addTimeToSequence("Build octree",
-                                phase(ctx, NB_PHASE_KEYS_SORT) + phase(ctx, NB_PHASE_TOPOLOGY));
-        timer.addTimeToSequence("Compute center of mass", phase(ctx, NB_PHASE_COM));
+        recordBuildPhases(build_ms);
         nbCheck(ctx, nb_bh_forces(ctx), "nb_bh_forces");
         const double accel = phase(ctx, NB_PHASE_ACCEL);
         timer.addTimeToSequence("Acceleration Kernel Time", accel);

@@ -81,6 +81,8 @@ class nBodyAlgorithm {
 
    private:
     void recordStepTimes(nb_ctx *ctx, int algorithm, bool loop);
+    // the octree builder's sequences from phase times ms[NB_PHASE_COUNT]
+    void recordBuildPhases(const double *ms);
 };
 
 // Throws std::runtime_error carrying the library's message when rc != NB_OK.

@@ -29,7 +29,8 @@ ALGO_NAIVE = 0
 ALGO_BARNES_HUT = 1
 
 PHASES = ["AABB creation", "keys and sort", "topology", "Compute center of mass",
-          "Octree creation", "Acceleration Kernel Time", "Leapfrog Part 1", "Leapfrog Part 2"]
+          "Octree creation", "Acceleration Kernel Time", "Leapfrog Part 1", "Leapfrog Part 2",
+          "keys", "sort"]
 
 # every symbol include/nbody_capi.h declares
 EXPORTS = [

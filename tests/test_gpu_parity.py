@@ -384,6 +384,30 @@ def test_storage_overflow_reported(gpu):
         assert e.value.code == gpu.NB_E_NODE_OVERFLOW
 
 
+@pytest.mark.parametrize("n,storage", [(4096, 1 << 20), (16, 1 << 31)])
+def test_oversized_storage_keeps_context(gpu, n, storage):
+    # storage_size_param * n beyond 2^29 nodes (the second case wraps a 32-bit
+    # cell count to 0): rejected before any buffer is touched; the context
+    # keeps its parameters and octree storage and builds as before
+    d = uniform_cube(n, 7)
+    with ctx_for(gpu, d, theta=0.5, precision=gpu.PREC_FP32C) as c:
+        c.bh_build()
+        want = c.bh_forces()
+        with pytest.raises(gpu.NBodyError) as e:
+            c.set_params(storage_size_param=storage)
+        assert e.value.code == gpu.NB_E_INVALID_ARG
+        assert c.params.storage_size_param == 16
+        c.bh_build()
+        got = c.bh_forces()
+        # a valid change reallocates and still builds the same tree
+        c.set_params(storage_size_param=24)
+        c.bh_build()
+        again = c.bh_forces()
+    for u, v, w in zip(want, got, again):
+        np.testing.assert_array_equal(bits(u), bits(v))
+        np.testing.assert_array_equal(bits(u), bits(w))
+
+
 # ---- full-size properties ------------------------------------------------------------------------
 def test_million_body_tree_properties(gpu):
     n = 1 << 20

@@ -132,6 +132,18 @@ def test_end_to_end(gpu, oracle, tmp_path, algorithm, extra):
     if algorithm == "BarnesHut":
         assert len(times["Octree creation"]) == steps + 1
         assert times["theta"] == 0.6
+        # the default (Subtrees) builder's sequences, ParallelOctreeTopDownSubtrees.cpp:76-92,
+        # one entry per build; sorting is on by default (Configuration.cpp:15)
+        for key in ("AABB creation", "Build octree to level", "Prepare subtrees",
+                    "Sort bodies for subtrees", "Build subtrees", "Compute center of mass",
+                    "Sort bodies", "Total Time"):
+            assert len(times[key]) == steps + 1, key
+        assert "Build octree" not in times  # the Synchronized builder's key
+        for k in range(steps + 1):
+            parts = sum(times[key][k] for key in ("AABB creation", "Build octree to level",
+                                                  "Sort bodies for subtrees", "Build subtrees",
+                                                  "Compute center of mass"))
+            assert 0 < parts <= times["Octree creation"][k] * 1.001 + 1e-3
     # lastState.csv vs the oracle's loop, printed with the same 6 digits
     theta = 0.6 if algorithm == "BarnesHut" else 1.05
     ref = oracle.simulate(algorithm, d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"],
@@ -151,3 +163,47 @@ def test_end_to_end(gpu, oracle, tmp_path, algorithm, extra):
     field = text.split('Name="kinetic energy" NumberOfTuples="1" format="ascii">\n')[1].split("\n")[0]
     assert float(field) == float("%.6g" % e0["kinetic"])
     assert "83 117 110  0" in text  # "Sun" as ASCII codes + terminating 0
+
+
+@pytest.mark.gpu
+def test_app_million_bodies_matches_library(gpu, tmp_path):
+    """The drop-in application at N = 2^20 (Barnes-Hut, FP32 pair forces): its
+    library parameters come from the reference's scaled sizes
+    (storageSizeParameter = 16 N, stackSize = 16 ceil(log2 N); Configuration.cpp:24-33),
+    its loop and visualisation rule are the reference's (BarnesHutAlgorithm.cpp:149-277),
+    and its last state equals the same loop driven through the C ABI directly."""
+    n = 1 << 20
+    csv = tmp_path / "disk.csv"
+    subprocess.run([GEN, "--kind=disk", f"--n={n}", "--seed=5", f"--out={csv}"], check=True,
+                   timeout=300)
+    out = tmp_path / "out"
+    out.mkdir()
+    print(f"app: BarnesHut fp32, n={n}", flush=True)
+    r = run_app([f"--file={csv}", "--dt=1h", "--t_end=2h", "--vs=2h", f"--vs_dir={out}",
+                 "--algorithm=BarnesHut", "--theta=0.5", "--precision=fp32"])
+    assert r.returncode == 0, r.stderr
+    res = _output_dir(str(out))
+    times = json.load(open(os.path.join(res, "times.json")))
+    assert times["body count"] == n
+    assert len(times["Build subtrees"]) == 3 and len(times["Sort bodies"]) == 3
+    last = open(os.path.join(res, "lastState.csv")).read().splitlines()
+    assert len(last) == n + 1
+
+    d = gpu.gen_star_disk(n, 5)
+    with gpu.Context(n, theta=0.5, precision=gpu.PREC_FP32C) as c:
+        c.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+        c.bh_build()
+        c.bh_forces()
+        c.steps(gpu.ALGO_BARNES_HUT, 1 / 24, 2)
+        x, y, z, _, _, _ = c.state()
+    for k in (0, 1, 12345, n // 2, n - 1):
+        assert last[1 + k] == ",".join("%.6g" % v for v in (x[k], y[k], z[k])), k
+
+    # a storage multiplier of 1 leaves N/8 internal cells, too few for this
+    # disk: reported as exhausted storage (not read as 1 * N per body)
+    out2 = tmp_path / "out2"
+    out2.mkdir()
+    r = run_app([f"--file={csv}", "--dt=1h", "--t_end=1h", "--vs=1h", f"--vs_dir={out2}",
+                 "--algorithm=BarnesHut", "--precision=fp32", "--storage_size_param=1"])
+    assert r.returncode == 1
+    assert "octree storage exhausted" in r.stderr
