@@ -70,7 +70,12 @@ TRAFFIC_BYTES = {
 WORKLOADS = {
     # name: (generator, n, seed, algorithm, theta, dt, precision)
     "bh_disk_1m": ("disk", 1 << 20, 0x4E424F44590001, "BarnesHut", 0.5, 1.0 / 24, nb.PREC_FP32C),
-    "naive_disk_16k": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP64),
+    # FP64 pairs, split sums in a fixed order (within 1e-12 of the reference sum)
+    "naive_disk_16k": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24,
+                       nb.PREC_FP64_FAST),
+    # the reference's arithmetic and summation order, bit-identical to the oracle
+    "naive_disk_16k_exact": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24,
+                             nb.PREC_FP64),
     "naive_disk_16k_fp32": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP32C),
     # theta None: the largest candidate meeting the error bound (choose_theta)
     "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", None, 1e-3, nb.PREC_FP32C),
@@ -333,7 +338,9 @@ def main():
     else:
         flops = FLOP_PER_INTERACTION * float(n) * float(n)
         peak = FP32_PEAK_TFLOPS if prec == nb.PREC_FP32C else FP64_PEAK_TFLOPS
-        bound, kernel = "valu", "k_direct_f32c" if prec == nb.PREC_FP32C else "k_direct_f64"
+        kernel = {nb.PREC_FP32C: "k_direct_f32c", nb.PREC_FP64: "k_direct_f64",
+                  nb.PREC_FP64_FAST: "k_direct_f64f"}[prec]
+        bound = "valu"
     achieved = flops / (accel_ms * 1e-3) / 1e12
     # compute-bound: "valu" is the vector ALU (the pair maths), "mfma" the
     # matrix cores (the network); the peak is that unit's datasheet figure

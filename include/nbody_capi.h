@@ -53,7 +53,13 @@ enum {
     NB_PREC_FP64 = 0,
     /* FP32 pair arithmetic with FP64 accumulation of per-tile FP32 partials;
      * Barnes-Hut opens a cell when any body of a 64-body group needs it. */
-    NB_PREC_FP32C = 1
+    NB_PREC_FP32C = 1,
+    /* Direct sum in FP64 pair arithmetic with the inverse square root from
+     * the hardware estimate and a Newton step, fused multiply-adds, and the
+     * sources split over many waves whose partial sums are added in a fixed
+     * order: deterministic, within a stated relative tolerance of the
+     * reference's sum rather than bit-identical.  Barnes-Hut runs as NB_PREC_FP64. */
+    NB_PREC_FP64_FAST = 2
 };
 
 enum { NB_ALGO_NAIVE = 0, NB_ALGO_BARNES_HUT = 1 };

@@ -32,8 +32,8 @@ struct nb_ctx {
     TreeHdr *hdr_host = nullptr;  // pinned copy of tree.hdr
     uint32_t *exact_stack = nullptr;
     uint32_t exact_cap = 0;
-    double *f32_scratch = nullptr;
-    uint32_t f32_splits = 0;
+    void *split_scratch = nullptr;  // partial sums and tickets of the split direct sums
+    uint64_t split_scratch_bytes = 0;
     uint32_t *targets_dev = nullptr;
     double *targets_out = nullptr;
     uint32_t targets_cap = 0;
@@ -263,7 +263,9 @@ int alloc_tree_into(nb_ctx *c, TreeBufs &t, uint32_t storage_size_param) {
 }
 
 bool params_ok(const nb_params *p) {
-    return p && (p->precision == NB_PREC_FP64 || p->precision == NB_PREC_FP32C) &&
+    return p &&
+           (p->precision == NB_PREC_FP64 || p->precision == NB_PREC_FP32C ||
+            p->precision == NB_PREC_FP64_FAST) &&
            p->storage_size_param >= 1 && p->stack_size_param >= 1 && std::isfinite(p->G) &&
            std::isfinite(p->eps2) && p->eps2 >= 0 && std::isfinite(p->theta) && p->theta >= 0;
 }
@@ -285,7 +287,7 @@ int build_tree(nb_ctx *c) {
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
-    const bool fp64 = c->p.precision == NB_PREC_FP64;
+    const bool fp64 = c->p.precision != NB_PREC_FP32C;  // FP64 and FP64_FAST
     nb::launch_com(t, c->p.G, fp64, !fp64, c->stream);
     c->quad_valid = false;
     if (c->corr && !fp64) {
@@ -306,7 +308,7 @@ int ensure_quad_buffers(nb_ctx *c) {
 
 int bh_forces_enqueue(nb_ctx *c) {
     mark_begin(c, NB_PHASE_ACCEL);
-    if (c->p.precision == NB_PREC_FP64) {
+    if (c->p.precision != NB_PREC_FP32C) {  // FP64 and FP64_FAST: the reference traversal
         const uint32_t cap = exact_stack_cap(c);
         if (!c->exact_stack || c->exact_cap != cap) {
             dev_free(c, c->exact_stack);
@@ -337,17 +339,21 @@ int direct_enqueue(nb_ctx *c) {
         nb::launch_direct_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay, c->az,
                               c->stream);
     } else {
-        const uint32_t splits = nb::direct_f32c_splits(c->n);
-        if (!c->f32_scratch || c->f32_splits != splits) {
-            dev_free(c, c->f32_scratch);
-            c->f32_scratch = nullptr;
-            // 3*n*splits partials + alignment pad + n float4 (= 2 doubles each)
-            int rc = dev_alloc(c, &c->f32_scratch, 3ull * c->n * splits + 2ull * c->n + 2);
+        const uint64_t bytes = nb::direct_split_scratch_bytes(c->n);
+        if (!c->split_scratch || c->split_scratch_bytes < bytes) {
+            dev_free(c, c->split_scratch);
+            c->split_scratch = nullptr;
+            uint8_t *buf = nullptr;
+            int rc = dev_alloc(c, &buf, bytes);
             if (rc) return rc;
-            c->f32_splits = splits;
+            // the tickets start at zero; every launch leaves them zero
+            NB_HIP(c, hipMemsetAsync(buf, 0, bytes, c->stream));
+            c->split_scratch = buf;
+            c->split_scratch_bytes = bytes;
         }
-        nb::launch_direct_f32c(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay,
-                               c->az, c->f32_scratch, c->f32_splits, c->stream);
+        nb::launch_direct_split(c->p.precision == NB_PREC_FP64_FAST, c->n, c->x, c->y, c->z,
+                                c->m, c->p.G, c->p.eps2, c->ax, c->ay, c->az, c->split_scratch,
+                                c->stream);
     }
     mark_end(c, NB_PHASE_ACCEL);
     return NB_OK;

@@ -130,20 +130,22 @@ __global__ void __launch_bounds__(kF64Threads) k_direct_targets_f64(
 }
 
 // ---------------------------------------------------------------------------
-// FP32 direct sum with FP64 tile accumulation.
-constexpr int kF32Threads = 256;
-constexpr int kF32Tile = 256;  // sources per LDS tile
-constexpr int kF32PerLane = 2; // targets per lane (each LDS read feeds 2 pairs)
-
-__global__ void __launch_bounds__(kF32Threads) k_pack_f32(uint32_t n,
-                                                          const double *__restrict__ x,
-                                                          const double *__restrict__ y,
-                                                          const double *__restrict__ z,
-                                                          const double *__restrict__ m, double G,
-                                                          float4 *pos) {
-    const uint32_t i = blockIdx.x * kF32Threads + threadIdx.x;
-    if (i < n) pos[i] = make_float4((float)x[i], (float)y[i], (float)z[i], (float)(G * m[i]));
-}
+// Split direct sums: FP32 pairs with FP64 tile accumulation (k_direct_f32c),
+// and FP64 pairs with v_rsq_f64 and a Newton step (k_direct_f64f).
+//
+// grid.x: tiles of kSplitTargets targets (kSplitPerLane per lane); grid.y:
+// source splits, each a contiguous source range.  Sources are staged in LDS
+// kSplitTile at a time, converted from the FP64 state arrays on the way (no
+// separate packing pass).  With one split a block writes its targets' sums;
+// with more, each block stores its partial sums, and the block that completes
+// a tile (the last of its splits to take the tile's ticket) adds the tile's
+// partials in split order 0, 1, ..., so results do not depend on block
+// timing.  The ticket is reset by that block for the next launch.
+constexpr int kSplitThreads = 256;
+constexpr int kSplitPerLane = 2;  // targets per lane (each LDS read feeds 2 pairs)
+constexpr uint32_t kSplitTargets = kSplitThreads * kSplitPerLane;
+constexpr int kSplitTile = 256;   // sources per LDS stage
+constexpr uint32_t kSplitBlocks = 2048;  // blocks a launch aims for (2 per SIMD)
 
 __device__ inline void pair_f32(const float4 s, float px, float py, float pz, float eps2,
                                 float &fx, float &fy, float &fz) {
@@ -156,74 +158,175 @@ __device__ inline void pair_f32(const float4 s, float px, float py, float pz, fl
     fz = fmaf(f, dz, fz);
 }
 
-// grid.x: blocks of kF32Threads*kF32PerLane targets; grid.y: source splits.
-__global__ void __launch_bounds__(kF32Threads) k_direct_f32c(uint32_t n,
-                                                             const float4 *__restrict__ pos,
-                                                             float eps2, uint32_t chunk,
-                                                             double *__restrict__ partial) {
-    __shared__ float4 tile[kF32Tile];
-    float px[kF32PerLane], py[kF32PerLane], pz[kF32PerLane];
-    double ax[kF32PerLane], ay[kF32PerLane], az[kF32PerLane];
-    uint32_t ti[kF32PerLane];
+// (r^2 + eps^2)^(-1/2) in FP64: the hardware estimate v_rsq_f64 and one
+// Newton-Raphson step y' = y + y (1 - v y^2) / 2, which squares the estimate's
+// relative error (bounded in the GPU tests against 1/sqrt).
+__device__ inline double rsqrt_f64(double v) {
+    const double y = __builtin_amdgcn_rsq(v);
+    const double e = fma(-(v * y), y, 1.0);
+    return fma(y * e, 0.5, y);
+}
+
+__device__ inline void pair_f64f(const double4 s, double px, double py, double pz, double eps2,
+                                 double &fx, double &fy, double &fz) {
+    const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+    const double r2 = fma(dx, dx, fma(dy, dy, fma(dz, dz, eps2)));
+    const double inv = rsqrt_f64(r2);
+    const double f = (s.w * inv) * (inv * inv);
+    fx = fma(f, dx, fx);
+    fy = fma(f, dy, fy);
+    fz = fma(f, dz, fz);
+}
+
+__device__ inline uint32_t split_target(uint32_t q) {
+    return blockIdx.x * kSplitTargets + q * kSplitThreads + threadIdx.x;
+}
+
+// End of a split direct sum for this lane's targets ti[] with sums (sx, sy,
+// sz)[]: out = scale * sum over splits in split order.  partial holds
+// 3 n gridDim.y doubles, ticket one zero word per tile.
+__device__ inline void split_finish(uint32_t n, double scale, const double (&sx)[kSplitPerLane],
+                                    const double (&sy)[kSplitPerLane],
+                                    const double (&sz)[kSplitPerLane], double *partial,
+                                    uint32_t *ticket, double *out_x, double *out_y,
+                                    double *out_z) {
+    __shared__ uint32_t s_last;
+    const uint32_t splits = gridDim.y;
+    if (splits == 1) {
 #pragma unroll
-    for (int q = 0; q < kF32PerLane; ++q) {
-        ti[q] = blockIdx.x * (kF32Threads * kF32PerLane) + q * kF32Threads + threadIdx.x;
-        const float4 p = pos[ti[q] < n ? ti[q] : n - 1];
-        px[q] = p.x;
-        py[q] = p.y;
-        pz[q] = p.z;
+        for (int q = 0; q < kSplitPerLane; ++q) {
+            const uint32_t i = split_target(q);
+            if (i < n) {
+                out_x[i] = sx[q] * scale;
+                out_y[i] = sy[q] * scale;
+                out_z[i] = sz[q] * scale;
+            }
+        }
+        return;
+    }
+    const uint64_t base = (uint64_t)blockIdx.y * 3ull * n;
+#pragma unroll
+    for (int q = 0; q < kSplitPerLane; ++q) {
+        const uint32_t i = split_target(q);
+        if (i < n) {
+            partial[base + i] = sx[q];
+            partial[base + n + i] = sy[q];
+            partial[base + 2ull * n + i] = sz[q];
+        }
+    }
+    // release: this block's partials reach device scope before its ticket
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(&ticket[blockIdx.x], 1u) + 1u == splits;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // acquire: the other splits' partials
+#pragma unroll
+    for (int q = 0; q < kSplitPerLane; ++q) {
+        const uint32_t i = split_target(q);
+        if (i >= n) continue;
+        double ox = 0, oy = 0, oz = 0;
+        for (uint32_t sp = 0; sp < splits; ++sp) {
+            const uint64_t b = (uint64_t)sp * 3ull * n;
+            ox += __hip_atomic_load(partial + b + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            oy += __hip_atomic_load(partial + b + n + i, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+            oz += __hip_atomic_load(partial + b + 2ull * n + i, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
+        out_x[i] = ox * scale;
+        out_y[i] = oy * scale;
+        out_z[i] = oz * scale;
+    }
+    if (threadIdx.x == 0) ticket[blockIdx.x] = 0;
+}
+
+// FP32 pairs on absolute FP32 positions with G*m as the source weight; each
+// stage's FP32 sums are folded into FP64.
+__global__ void __launch_bounds__(kSplitThreads) k_direct_f32c(
+    uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, double G, float eps2,
+    uint32_t chunk, double *partial, uint32_t *ticket, double *ax_out, double *ay_out,
+    double *az_out) {
+    __shared__ float4 tile[kSplitTile];
+    float px[kSplitPerLane], py[kSplitPerLane], pz[kSplitPerLane];
+    double ax[kSplitPerLane], ay[kSplitPerLane], az[kSplitPerLane];
+#pragma unroll
+    for (int q = 0; q < kSplitPerLane; ++q) {
+        const uint32_t i = split_target(q);
+        const uint32_t ii = i < n ? i : n - 1;
+        px[q] = (float)x[ii];
+        py[q] = (float)y[ii];
+        pz[q] = (float)z[ii];
         ax[q] = ay[q] = az[q] = 0.0;
     }
     const uint32_t j0 = blockIdx.y * chunk;
     const uint32_t j1 = min(n, j0 + chunk);
-    for (uint32_t k = j0; k < j1; k += kF32Tile) {
+    for (uint32_t k = j0; k < j1; k += kSplitTile) {
         const uint32_t j = k + threadIdx.x;
-        tile[threadIdx.x] = j < j1 ? pos[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+        tile[threadIdx.x] = j < j1 ? make_float4((float)x[j], (float)y[j], (float)z[j],
+                                                 (float)(G * m[j]))
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
         __syncthreads();
-        float fx[kF32PerLane], fy[kF32PerLane], fz[kF32PerLane];
+        float fx[kSplitPerLane], fy[kSplitPerLane], fz[kSplitPerLane];
 #pragma unroll
-        for (int q = 0; q < kF32PerLane; ++q) fx[q] = fy[q] = fz[q] = 0.f;
+        for (int q = 0; q < kSplitPerLane; ++q) fx[q] = fy[q] = fz[q] = 0.f;
 #pragma unroll 8
-        for (int jj = 0; jj < kF32Tile; ++jj) {
-            const float4 s = tile[jj];
+        for (int jj = 0; jj < kSplitTile; ++jj) {
+            const float4 sj = tile[jj];
 #pragma unroll
-            for (int q = 0; q < kF32PerLane; ++q) pair_f32(s, px[q], py[q], pz[q], eps2, fx[q], fy[q], fz[q]);
+            for (int q = 0; q < kSplitPerLane; ++q)
+                pair_f32(sj, px[q], py[q], pz[q], eps2, fx[q], fy[q], fz[q]);
         }
 #pragma unroll
-        for (int q = 0; q < kF32PerLane; ++q) {
+        for (int q = 0; q < kSplitPerLane; ++q) {
             ax[q] += (double)fx[q];
             ay[q] += (double)fy[q];
             az[q] += (double)fz[q];
         }
         __syncthreads();
     }
-    const uint64_t base = (uint64_t)blockIdx.y * 3ull * n;
-#pragma unroll
-    for (int q = 0; q < kF32PerLane; ++q) {
-        if (ti[q] < n) {
-            partial[base + ti[q]] = ax[q];
-            partial[base + n + ti[q]] = ay[q];
-            partial[base + 2ull * n + ti[q]] = az[q];
-        }
-    }
+    split_finish(n, 1.0, ax, ay, az, partial, ticket, ax_out, ay_out, az_out);
 }
 
-// Deterministic reduction of the split partials in split order.
-__global__ void __launch_bounds__(kF32Threads) k_direct_f32c_reduce(
-    uint32_t n, uint32_t splits, const double *__restrict__ partial, double *ax, double *ay,
-    double *az) {
-    const uint32_t i = blockIdx.x * kF32Threads + threadIdx.x;
-    if (i >= n) return;
-    double sx = 0, sy = 0, sz = 0;
-    for (uint32_t s = 0; s < splits; ++s) {
-        const uint64_t base = (uint64_t)s * 3ull * n;
-        sx += partial[base + i];
-        sy += partial[base + n + i];
-        sz += partial[base + 2ull * n + i];
+// FP64 pairs (the reference's formula, NaiveAlgorithm.cpp:318-351, with the
+// inverse square root from v_rsq_f64 and a Newton step, and fused
+// multiply-adds); G is applied to the sum, as in the reference (:348-351).
+__global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
+    uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
+    uint32_t chunk, double *partial, uint32_t *ticket, double *ax_out, double *ay_out,
+    double *az_out) {
+    __shared__ double4 tile[kSplitTile];
+    double px[kSplitPerLane], py[kSplitPerLane], pz[kSplitPerLane];
+    double ax[kSplitPerLane], ay[kSplitPerLane], az[kSplitPerLane];
+#pragma unroll
+    for (int q = 0; q < kSplitPerLane; ++q) {
+        const uint32_t i = split_target(q);
+        const uint32_t ii = i < n ? i : n - 1;
+        px[q] = x[ii];
+        py[q] = y[ii];
+        pz[q] = z[ii];
+        ax[q] = ay[q] = az[q] = 0.0;
     }
-    ax[i] = sx;
-    ay[i] = sy;
-    az[i] = sz;
+    const uint32_t j0 = blockIdx.y * chunk;
+    const uint32_t j1 = min(n, j0 + chunk);
+    for (uint32_t k = j0; k < j1; k += kSplitTile) {
+        const uint32_t j = k + threadIdx.x;
+        // a padding source has zero mass and a far position: it adds 0
+        tile[threadIdx.x] = j < j1 ? make_double4(x[j], y[j], z[j], m[j])
+                                   : make_double4(1e300, 0.0, 0.0, 0.0);
+        __syncthreads();
+#pragma unroll 4
+        for (int jj = 0; jj < kSplitTile; ++jj) {
+            const double4 sj = tile[jj];
+#pragma unroll
+            for (int q = 0; q < kSplitPerLane; ++q)
+                pair_f64f(sj, px[q], py[q], pz[q], eps2, ax[q], ay[q], az[q]);
+        }
+        __syncthreads();
+    }
+    split_finish(n, G, ax, ay, az, partial, ticket, ax_out, ay_out, az_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -1031,7 +1134,19 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
     if (overflow && lane == 0) atomicOr(a.status, kFlagStack);
 }
 
+__global__ void k_fill_u32(uint32_t *p, uint64_t words, uint32_t value) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
 }  // namespace
+
+void launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s) {
+    if (words == 0) return;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (words + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, words, value);
+}
 
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
                        const double *m, double G, double eps2, double *ax, double *ay,
@@ -1049,26 +1164,46 @@ void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, con
                        dim3(kF64Threads), 0, s, n, x, y, z, m, G, eps2, nt, targets, ax, ay, az);
 }
 
-uint32_t direct_f32c_splits(uint32_t n) {
-    const uint32_t blocks = ceil_div(n, kF32Threads * kF32PerLane);
-    uint32_t splits = ceil_div(2048, blocks);
-    const uint32_t max_splits = std::max<uint32_t>(1, n / kF32Tile);
+// About kSplitBlocks blocks per launch; rounding the split count down keeps
+// the partial sums at most 3 * 8 * kSplitBlocks * kSplitTargets bytes (24 MiB)
+// for every n.
+uint32_t direct_split_count(uint32_t n) {
+    const uint32_t tiles = ceil_div(n, kSplitTargets);
+    const uint32_t splits = kSplitBlocks / tiles;
+    const uint32_t max_splits = std::max<uint32_t>(1, n / kSplitTile);
     return std::max<uint32_t>(1, std::min(splits, max_splits));
 }
 
-void launch_direct_f32c(uint32_t n, const double *x, const double *y, const double *z,
-                        const double *m, double G, double eps2, double *ax, double *ay,
-                        double *az, double *partial, uint32_t splits, hipStream_t s) {
-    // partial holds 3*n*splits doubles, then (16-byte aligned) n float4 positions
-    float4 *pos = reinterpret_cast<float4 *>(partial + ((3ull * n * splits + 1) & ~1ull));
-    hipLaunchKernelGGL(k_pack_f32, dim3(ceil_div(n, kF32Threads)), dim3(kF32Threads), 0, s, n, x,
-                       y, z, m, G, pos);
-    const uint32_t chunk = ceil_div(ceil_div(n, splits), kF32Tile) * kF32Tile;
-    const dim3 grid(ceil_div(n, kF32Threads * kF32PerLane), splits);
-    hipLaunchKernelGGL(k_direct_f32c, grid, dim3(kF32Threads), 0, s, n, (const float4 *)pos,
-                       (float)eps2, chunk, partial);
-    hipLaunchKernelGGL(k_direct_f32c_reduce, dim3(ceil_div(n, kF32Threads)), dim3(kF32Threads), 0,
-                       s, n, splits, (const double *)partial, ax, ay, az);
+// scratch layout: partial sums [split][3][n] doubles (none with one split),
+// then one ticket word per tile, each part starting on a 256-byte line
+uint64_t direct_split_ticket_offset(uint32_t n) {
+    const uint32_t splits = direct_split_count(n);
+    const uint64_t partial = splits > 1 ? 3ull * n * splits * sizeof(double) : 0;
+    return (partial + 255) & ~255ull;
+}
+
+uint64_t direct_split_scratch_bytes(uint32_t n) {
+    const uint64_t tickets = (uint64_t)ceil_div(n, kSplitTargets) * sizeof(uint32_t);
+    return direct_split_ticket_offset(n) + ((tickets + 255) & ~255ull);
+}
+
+void launch_direct_split(bool fp64, uint32_t n, const double *x, const double *y,
+                         const double *z, const double *m, double G, double eps2, double *ax,
+                         double *ay, double *az, void *scratch, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t splits = direct_split_count(n);
+    const uint32_t tiles = ceil_div(n, kSplitTargets);
+    double *partial = static_cast<double *>(scratch);
+    uint32_t *ticket = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch) +
+                                                    direct_split_ticket_offset(n));
+    const uint32_t chunk = ceil_div(ceil_div(n, splits), kSplitTile) * kSplitTile;
+    const dim3 grid(tiles, splits);
+    if (fp64)
+        hipLaunchKernelGGL(k_direct_f64f, grid, dim3(kSplitThreads), 0, s, n, x, y, z, m, G, eps2,
+                           chunk, partial, ticket, ax, ay, az);
+    else
+        hipLaunchKernelGGL(k_direct_f32c, grid, dim3(kSplitThreads), 0, s, n, x, y, z, m, G,
+                           (float)eps2, chunk, partial, ticket, ax, ay, az);
 }
 
 void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const double *z,

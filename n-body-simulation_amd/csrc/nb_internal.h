@@ -254,10 +254,18 @@ void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, con
                                const double *m, double G, double eps2, uint32_t nt,
                                const uint32_t *targets, double *ax, double *ay, double *az,
                                hipStream_t s);
-void launch_direct_f32c(uint32_t n, const double *x, const double *y, const double *z,
-                        const double *m, double G, double eps2, double *ax, double *ay,
-                        double *az, double *partial, uint32_t splits, hipStream_t s);
-uint32_t direct_f32c_splits(uint32_t n);
+// Split direct sum (FP32 pairs with FP64 accumulation, or fp64: FP64 pairs
+// with v_rsq_f64 and a Newton step), one launch, deterministic.  scratch
+// holds direct_split_scratch_bytes(n) bytes (256-byte aligned) whose ticket
+// words are zero before the first launch; every launch leaves them zero.
+void launch_direct_split(bool fp64, uint32_t n, const double *x, const double *y,
+                         const double *z, const double *m, double G, double eps2, double *ax,
+                         double *ay, double *az, void *scratch, hipStream_t s);
+uint32_t direct_split_count(uint32_t n);
+uint64_t direct_split_scratch_bytes(uint32_t n);
+uint64_t direct_split_ticket_offset(uint32_t n);  // bytes before the ticket words
+// zero `words` 32-bit words with a kernel (no memset node under graph capture)
+void launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
 void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const double *z,
                      double G, double eps2, double theta, int sort_bodies, uint32_t *stack,
                      uint32_t stack_cap, double *ax, double *ay, double *az,

@@ -25,6 +25,7 @@ NB_E_NO_DEVICE = 10
 
 PREC_FP64 = 0
 PREC_FP32C = 1
+PREC_FP64_FAST = 2
 ALGO_NAIVE = 0
 ALGO_BARNES_HUT = 1
 

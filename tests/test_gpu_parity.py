@@ -77,6 +77,59 @@ def test_direct_f32c_accuracy(gpu, oracle, kind, n, med, worst):
     assert np.quantile(e, 0.999) < worst, np.quantile(e, 0.999)
 
 
+def term_sum(d, G, eps2):
+    """S_i = G sum_j m_j / (r_ij^2 + eps^2): the sum of the pair terms'
+    magnitudes, the scale of a summation's rounding error."""
+    x, y, z, m = (np.asarray(d[k], dtype=np.float64) for k in ("x", "y", "z", "m"))
+    n = x.size
+    S = np.empty(n)
+    for i in range(0, n, 512):
+        sl = slice(i, min(n, i + 512))
+        r2 = (x[sl, None] - x) ** 2 + (y[sl, None] - y) ** 2 + (z[sl, None] - z) ** 2 + eps2
+        t = m[None, :] / r2
+        t[np.arange(sl.stop - sl.start), np.arange(sl.start, sl.stop)] = 0.0  # no self term
+        S[sl] = G * t.sum(axis=1)
+    return S
+
+
+@pytest.mark.parametrize("kind,n", [("disk", 16384), ("plummer", 5000), ("cube", 1000),
+                                    ("cluster", 3001), ("cube", 2), ("cube", 1)])
+def test_direct_f64_fast_tolerance(gpu, oracle, kind, n):
+    # FP64 pairs with v_rsq_f64 + one Newton step, split sums added in a fixed
+    # order.  Bound: per body ||a - a_ref|| <= 1e-12 S_i (S_i above), and
+    # median relative error <= 1e-14, against the reference arithmetic.
+    d = make(gpu, kind, n)
+    with ctx_for(gpu, d, precision=gpu.PREC_FP64_FAST) as c:
+        ax, ay, az = c.forces_direct()
+        again = c.forces_direct()
+    for u, v in zip((ax, ay, az), again):
+        np.testing.assert_array_equal(bits(u), bits(v))  # deterministic
+    rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"])
+    diff = np.sqrt((ax - rx) ** 2 + (ay - ry) ** 2 + (az - rz) ** 2)
+    S = term_sum(d, oracle.G_ref(), oracle.eps2_ref())
+    assert np.all(diff <= 1e-12 * S), (diff / np.where(S > 0, S, 1)).max()
+    if n > 1:
+        e = rel_err(ax, ay, az, rx, ry, rz)
+        assert np.median(e) <= 1e-14, np.median(e)
+
+
+def test_rsq_f64_newton_accuracy(gpu):
+    # The inverse square root of the fast FP64 sum, checked on two bodies:
+    # a(0) = G m1 r / (r^2 + eps^2)^(3/2) over a sweep of separations spanning
+    # 1e-10 .. 1e10 AU, against the closed form in long double.
+    G, eps2 = 1.0, 1e-22
+    worst = 0.0
+    for r in np.logspace(-10, 10, 41):
+        d = dict(x=np.array([0.0, r]), y=np.zeros(2), z=np.zeros(2), vx=np.zeros(2),
+                 vy=np.zeros(2), vz=np.zeros(2), m=np.array([1.0, 3.0]))
+        with ctx_for(gpu, d, precision=gpu.PREC_FP64_FAST, G=G, eps2=eps2) as c:
+            ax, _, _ = c.forces_direct()
+        rl = np.longdouble(r)
+        want = np.longdouble(3.0) * rl / (rl * rl + np.longdouble(eps2)) ** np.longdouble(1.5)
+        worst = max(worst, float(abs((np.longdouble(ax[0]) - want) / want)))
+    assert worst < 4e-15, worst
+
+
 # ---- integrator -----------------------------------------------------------------------
 def test_leapfrog_bitexact(gpu, oracle):
     d = make(gpu, "cube", 3001, 5)

@@ -195,9 +195,9 @@ def test_app_million_bodies_matches_library(gpu, tmp_path):
         c.bh_build()
         c.bh_forces()
         c.steps(gpu.ALGO_BARNES_HUT, 1 / 24, 2)
-        x, y, z, _, _, _ = c.state()
+        st = c.state()
     for k in (0, 1, 12345, n // 2, n - 1):
-        assert last[1 + k] == ",".join("%.6g" % v for v in (x[k], y[k], z[k])), k
+        assert last[1 + k] == ",".join("%.6g" % st[a][k] for a in ("x", "y", "z")), k
 
     # a storage multiplier of 1 leaves N/8 internal cells, too few for this
     # disk: reported as exhausted storage (not read as 1 * N per body)
