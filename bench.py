@@ -35,12 +35,14 @@ FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0  # dense matrix-core peak
 HBM_PEAK_GBS = 8000.0
 FLOP_PER_INTERACTION = 20  # GPU Gems 3 convention, reference README.md:158
+QUAD_FLOP_PER_PAIR = 49  # monopole + traceless quadrupole of a cell, same counting
 MLP_FLOP_PER_PAIR = 18432  # 2 * (13*64 + 64*64 + 64*64 + 64*3) per network evaluation
 ERR_SAMPLE = 16384
 ERR_GATE = 1e-4
 # opening angles tried for the default workload, largest first; the first whose
 # median error on the initial state is at most ERR_GATE * THETA_MARGIN is used
-THETA_CANDIDATES = (0.6, 0.55, 0.5, 0.45, 0.42, 0.4, 0.38, 0.36, 0.34, 0.32, 0.3, 0.25, 0.2)
+THETA_CANDIDATES = (1.0, 0.9, 0.8, 0.75, 0.7, 0.65, 0.6, 0.55, 0.5, 0.45, 0.42, 0.4, 0.38, 0.36,
+                    0.34, 0.32, 0.3, 0.25, 0.2)
 THETA_MARGIN = 0.95
 THETA_SWEEP = (0.3, 0.5, 0.7, 1.0)
 
@@ -281,6 +283,8 @@ def main():
     ap.add_argument("--theta", type=float, default=None,
                     help="opening angle (default: the workload's; for bh_plummer_4m the largest "
                          "candidate meeting the error bound)")
+    ap.add_argument("--quadrupole", action="store_true",
+                    help="Barnes-Hut cells with their quadrupole moments (FP32C traversal)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the theta sweep of the Barnes-Hut workloads")
@@ -314,7 +318,8 @@ def main():
     net, train_info = (train_correction() if mlp else (None, None))
     log(f"workload {args.workload}: generating {kind} n={n} ...")
     d = gen(kind, n, seed)
-    ctx = nb.Context(n, theta=theta if theta is not None else 0.5, precision=prec)
+    ctx = nb.Context(n, theta=theta if theta is not None else 0.5, precision=prec,
+                     quadrupole=int(args.quadrupole))
     theta_search = None
     if algorithm == "BarnesHut" and theta is None:
         theta, theta_search = choose_theta(ctx, d, dt)
@@ -333,7 +338,10 @@ def main():
         flops = MLP_FLOP_PER_PAIR * float(pairs)
         peak, bound, kernel = BF16_PEAK_TFLOPS, "mfma", "k_bh_group<corrected>"
     elif algorithm == "BarnesHut":
-        flops = FLOP_PER_INTERACTION * float(inter.sum())
+        # quadrupole cell pairs count 49 flop (the direct sum's counting of
+        # their arithmetic), monopole cells and bodies 20
+        qpairs = float(ctx.quadrupole_pairs()) if args.quadrupole else 0.0
+        flops = FLOP_PER_INTERACTION * (float(inter.sum()) - qpairs) + QUAD_FLOP_PER_PAIR * qpairs
         peak, bound, kernel = FP32_PEAK_TFLOPS, "valu", "k_bh_group"
     else:
         flops = FLOP_PER_INTERACTION * float(n) * float(n)
@@ -355,6 +363,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
+                   "cells": "quadrupole" if args.quadrupole else "monopole",
                    "dt": dt, "median_rel_err": err, "err_sample": sample, "gate": ERR_GATE,
                    "gate_met": gate_met, "ungated_value": round(raw_value, 1),
                    "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
@@ -382,7 +391,8 @@ def main():
             ctx.set_params(theta=th)
             ssecs, sph, serr, _, sinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
             sms = sph["Acceleration Kernel Time"] / args.steps
-            sflop = FLOP_PER_INTERACTION * float(sinter.sum())
+            sq = float(ctx.quadrupole_pairs()) if args.quadrupole else 0.0
+            sflop = FLOP_PER_INTERACTION * (float(sinter.sum()) - sq) + QUAD_FLOP_PER_PAIR * sq
             svalue = n * args.steps / ssecs
             sweep.append({"theta": th, "value": gated(svalue, serr)[0],
                           "ungated_value": round(svalue, 1), "median_rel_err": serr,

@@ -55,7 +55,8 @@ enum {
      * Barnes-Hut opens a cell when any body of a 64-body group needs it. */
     NB_PREC_FP32C = 1,
     /* Direct sum in FP64 pair arithmetic with the inverse square root from
-     * the hardware estimate and a Newton step, fused multiply-adds, and the
+     * the hardware estimate and a second-order correction, fused
+     * multiply-adds, and the
      * sources split over many waves whose partial sums are added in a fixed
      * order: deterministic, within a stated relative tolerance of the
      * reference's sum rather than bit-identical.  Barnes-Hut runs as NB_PREC_FP64. */
@@ -72,10 +73,17 @@ typedef struct nb_params {
     uint32_t sort_bodies; /* configuration::barnes_hut_algorithm::sortBodies, Configuration.cpp:15 */
     uint32_t storage_size_param; /* main.cpp:122-127; node capacity = param * N (default 16) */
     uint32_t stack_size_param;   /* main.cpp:129-134; traversal stack rule (default 16) */
+    /* No counterpart in the reference (which uses monopole cells,
+     * BarnesHutAlgorithm.cpp:361-367): nonzero adds each accepted cell's
+     * traceless quadrupole moment to its monopole force in the NB_PREC_FP32C
+     * group traversal, so a larger theta reaches the same accuracy.  The
+     * FP64 modes ignore it.  Default 0. */
+    uint32_t quadrupole;
 } nb_params;
 
 /* Fill *p with the reference defaults (G, eps2 = 1e-22, theta = 1.05,
- * NB_PREC_FP64, sorting on, storage 16, stack 16; Configuration.cpp:5-33). */
+ * NB_PREC_FP64, sorting on, storage 16, stack 16, monopole cells;
+ * Configuration.cpp:5-33). */
 void nb_default_params(nb_params *p);
 
 /* ---- context ------------------------------------------------------------- */
@@ -263,6 +271,10 @@ int nb_set_correction(nb_ctx *ctx, nb_mlp *mlp, double tau);
 /* Network evaluations ((cell, body) pairs) of the last corrected nb_bh_forces
  * (18,432 flop each, forward). */
 int nb_correction_pairs(nb_ctx *ctx, uint64_t *pairs);
+/* (cell, body) interactions evaluated with the quadrupole term in the last
+ * nb_bh_forces (0 unless nb_params.quadrupole); included in the counts of
+ * nb_bh_interactions.  No counterpart in the reference. */
+int nb_quadrupole_pairs(nb_ctx *ctx, uint64_t *pairs);
 /* Quadrupole moments Q = sum m (3 s s^T - |s|^2 I) of every node, about its
  * centre of mass, as {xx, yy, zz, xy, xz, yz}, in the canonical order of
  * nb_tree_canonical (zero for leaves).  q has 6 * cap doubles.  Returns the

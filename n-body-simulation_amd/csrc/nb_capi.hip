@@ -32,7 +32,7 @@ struct nb_ctx {
     TreeHdr *hdr_host = nullptr;  // pinned copy of tree.hdr
     uint32_t *exact_stack = nullptr;
     uint32_t exact_cap = 0;
-    void *split_scratch = nullptr;  // partial sums and tickets of the split direct sums
+    double *split_scratch = nullptr;  // partial sums of the split direct sums
     uint64_t split_scratch_bytes = 0;
     uint32_t *targets_dev = nullptr;
     double *targets_out = nullptr;
@@ -43,6 +43,7 @@ struct nb_ctx {
     float corr_tau2 = 0.f;
     double *quad = nullptr;          // 6 per internal cell
     nb::MlpCell *mcell = nullptr;    // per internal cell
+    float4 *gquad = nullptr;         // 2 per internal cell: G*Q for the quadrupole walk
     bool quad_valid = false;         // quadrupoles match the current octree
     // asynchronous snapshots: device staging (9 n doubles), two pinned host
     // slots used in turn, the DMA stream and the event that ends each DMA
@@ -290,19 +291,21 @@ int build_tree(nb_ctx *c) {
     const bool fp64 = c->p.precision != NB_PREC_FP32C;  // FP64 and FP64_FAST
     nb::launch_com(t, c->p.G, fp64, !fp64, c->stream);
     c->quad_valid = false;
-    if (c->corr && !fp64) {
-        nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);
+    if (!fp64 && (c->corr || c->p.quadrupole)) {
+        nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->corr ? c->mcell : nullptr,
+                               c->p.quadrupole ? c->gquad : nullptr, c->p.G, c->stream);
         c->quad_valid = true;
     }
     mark_end(c, NB_PHASE_COM);
     return NB_OK;
 }
 
+// the moment buffers of the learned correction and the quadrupole walk
 int ensure_quad_buffers(nb_ctx *c) {
-    if (c->quad && c->mcell) return NB_OK;
     int rc;
-    if ((rc = dev_alloc(c, &c->quad, 6ull * c->tree.icap))) return rc;
-    if ((rc = dev_alloc(c, &c->mcell, c->tree.icap))) return rc;
+    if (!c->quad && (rc = dev_alloc(c, &c->quad, 6ull * c->tree.icap))) return rc;
+    if (!c->mcell && (rc = dev_alloc(c, &c->mcell, c->tree.icap))) return rc;
+    if (!c->gquad && (rc = dev_alloc(c, &c->gquad, 2ull * c->tree.icap))) return rc;
     return NB_OK;
 }
 
@@ -324,10 +327,12 @@ int bh_forces_enqueue(nb_ctx *c) {
         const nb::MlpWeights w = nb::mlp_weights(c->corr);
         const nb::CorrArgs ca{w.pack, w.master, c->mcell, c->corr_tau2};
         nb::launch_bh_group_corr(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta, ca,
-                                 c->ax, c->ay, c->az, c->inter, c->stream);
+                                 c->p.quadrupole ? c->gquad : nullptr, c->ax, c->ay, c->az,
+                                 c->inter, c->stream);
     } else {
-        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta, c->ax,
-                            c->ay, c->az, c->inter, c->stream);
+        const bool quad = c->p.quadrupole && c->quad_valid;
+        nb::launch_bh_group(c->tree, c->x, c->y, c->z, c->m, c->p.eps2, c->p.theta,
+                            quad ? c->gquad : nullptr, c->ax, c->ay, c->az, c->inter, c->stream);
     }
     mark_end(c, NB_PHASE_ACCEL);
     return NB_OK;
@@ -340,15 +345,11 @@ int direct_enqueue(nb_ctx *c) {
                               c->stream);
     } else {
         const uint64_t bytes = nb::direct_split_scratch_bytes(c->n);
-        if (!c->split_scratch || c->split_scratch_bytes < bytes) {
+        if (bytes && (!c->split_scratch || c->split_scratch_bytes < bytes)) {
             dev_free(c, c->split_scratch);
             c->split_scratch = nullptr;
-            uint8_t *buf = nullptr;
-            int rc = dev_alloc(c, &buf, bytes);
+            int rc = dev_alloc(c, &c->split_scratch, bytes / sizeof(double));
             if (rc) return rc;
-            // the tickets start at zero; every launch leaves them zero
-            NB_HIP(c, hipMemsetAsync(buf, 0, bytes, c->stream));
-            c->split_scratch = buf;
             c->split_scratch_bytes = bytes;
         }
         nb::launch_direct_split(c->p.precision == NB_PREC_FP64_FAST, c->n, c->x, c->y, c->z,
@@ -377,6 +378,7 @@ void nb_default_params(nb_params *p) {
     p->sort_bodies = 1;
     p->storage_size_param = 16;
     p->stack_size_param = 16;
+    p->quadrupole = 0;
 }
 
 const char *nb_strerror(int code) {
@@ -432,6 +434,7 @@ int nb_create(nb_ctx **out, uint32_t n, const nb_params *p) {
         return bail(NB_E_HIP);
     std::memset(c->hdr_host, 0, sizeof(TreeHdr));
     if ((rc = alloc_tree(c))) return bail(rc);
+    if (p->quadrupole && (rc = ensure_quad_buffers(c))) return bail(rc);
     for (double **s : state)
         if (hipMemsetAsync(*s, 0, (size_t)n * sizeof(double), c->stream) != hipSuccess)
             return bail(NB_E_HIP);
@@ -470,10 +473,13 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         if (rc) return rc;
         double *quad = nullptr;
         nb::MlpCell *mcell = nullptr;
-        if (c->corr) {
+        float4 *gquad = nullptr;
+        if (c->corr || p->quadrupole) {
             if ((rc = dev_alloc(c, &quad, 6ull * fresh.icap)) ||
-                (rc = dev_alloc(c, &mcell, fresh.icap))) {
+                (rc = dev_alloc(c, &mcell, fresh.icap)) ||
+                (rc = dev_alloc(c, &gquad, 2ull * fresh.icap))) {
                 dev_free(c, quad);
+                dev_free(c, mcell);
                 tree_free(c, fresh);
                 return rc;
             }
@@ -481,16 +487,22 @@ int nb_set_params(nb_ctx *c, const nb_params *p) {
         if (hipStreamSynchronize(c->stream) != hipSuccess) {
             dev_free(c, quad);
             dev_free(c, mcell);
+            dev_free(c, gquad);
             tree_free(c, fresh);
             return fail(c, NB_E_HIP, "nb_set_params: stream synchronisation failed");
         }
         tree_free(c, c->tree);
         dev_free(c, c->quad);
         dev_free(c, c->mcell);
+        dev_free(c, c->gquad);
         c->tree = fresh;
         c->quad = quad;
         c->mcell = mcell;
+        c->gquad = gquad;
         c->quad_valid = false;
+    } else if (p->quadrupole && !c->p.quadrupole) {
+        int rc = ensure_quad_buffers(c);
+        if (rc) return rc;
     }
     c->p = *p;
     c->tree_valid = false;  // opening radii depend on theta
@@ -824,6 +836,15 @@ int nb_set_correction(nb_ctx *c, nb_mlp *mlp, double tau) {
     return NB_OK;
 }
 
+int nb_quadrupole_pairs(nb_ctx *c, uint64_t *pairs) {
+    if (!c || !pairs) return NB_E_INVALID_ARG;
+    NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
+                             c->stream));
+    NB_HIP(c, hipStreamSynchronize(c->stream));
+    *pairs = c->hdr_host->quad_pairs;
+    return NB_OK;
+}
+
 int nb_correction_pairs(nb_ctx *c, uint64_t *pairs) {
     if (!c || !pairs) return NB_E_INVALID_ARG;
     NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
@@ -843,7 +864,8 @@ int quad_now(nb_ctx *c) {
     if (c->quad_valid) return NB_OK;
     int rc = ensure_quad_buffers(c);
     if (rc) return rc;
-    nb::launch_quadrupoles(c->tree, c->x, c->y, c->z, c->m, c->quad, c->mcell, c->stream);
+    nb::launch_quadrupoles(c->tree, c->x, c->y, c->z, c->m, c->quad, c->mcell,
+                           c->p.quadrupole ? c->gquad : nullptr, c->p.G, c->stream);
     NB_HIP(c, hipGetLastError());
     NB_HIP(c, hipStreamSynchronize(c->stream));
     c->quad_valid = true;

@@ -131,16 +131,17 @@ __global__ void __launch_bounds__(kF64Threads) k_direct_targets_f64(
 
 // ---------------------------------------------------------------------------
 // Split direct sums: FP32 pairs with FP64 tile accumulation (k_direct_f32c),
-// and FP64 pairs with v_rsq_f64 and a Newton step (k_direct_f64f).
+// and FP64 pairs with a corrected v_rsq_f64 (k_direct_f64f).
 //
 // grid.x: tiles of kSplitTargets targets (kSplitPerLane per lane); grid.y:
 // source splits, each a contiguous source range.  Sources are staged in LDS
 // kSplitTile at a time, converted from the FP64 state arrays on the way (no
 // separate packing pass).  With one split a block writes its targets' sums;
-// with more, each block stores its partial sums, and the block that completes
-// a tile (the last of its splits to take the tile's ticket) adds the tile's
-// partials in split order 0, 1, ..., so results do not depend on block
-// timing.  The ticket is reset by that block for the next launch.
+// with more, each block stores its partial sums and k_split_reduce adds them
+// in split order 0, 1, ..., so results do not depend on block timing.  (The
+// partials can also be added by the last block of each tile to arrive, but
+// the device-scope fence each block then needs writes back the L2: the
+// 16,384-body FP32 sum took 232 us that way against 98 us.)
 constexpr int kSplitThreads = 256;
 constexpr int kSplitPerLane = 2;  // targets per lane (each LDS read feeds 2 pairs)
 constexpr uint32_t kSplitTargets = kSplitThreads * kSplitPerLane;
@@ -158,13 +159,18 @@ __device__ inline void pair_f32(const float4 s, float px, float py, float pz, fl
     fz = fmaf(f, dz, fz);
 }
 
-// (r^2 + eps^2)^(-1/2) in FP64: the hardware estimate v_rsq_f64 and one
-// Newton-Raphson step y' = y + y (1 - v y^2) / 2, which squares the estimate's
-// relative error (bounded in the GPU tests against 1/sqrt).
+// v^(-1/2) in FP64 from the hardware estimate y = v_rsq_f64(v), whose
+// relative error is near 1e-8 (single-precision accuracy): with
+// e = 1 - v y^2 (one fused multiply-add, so e is exact to the last bits),
+// v^(-1/2) = y (1 - e)^(-1/2) = y (1 + e/2 + 3e^2/8 + 5e^3/16 + ...).  The
+// correction to second order in e leaves a truncation error near
+// 5e^3/16 ~ 1e-22, below FP64 rounding; five operations (a Newton step,
+// first order in e, would leave ~1e-14).  Checked against long-double
+// closed forms in the GPU tests.
 __device__ inline double rsqrt_f64(double v) {
     const double y = __builtin_amdgcn_rsq(v);
     const double e = fma(-(v * y), y, 1.0);
-    return fma(y * e, 0.5, y);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 __device__ inline void pair_f64f(const double4 s, double px, double py, double pz, double eps2,
@@ -182,63 +188,49 @@ __device__ inline uint32_t split_target(uint32_t q) {
     return blockIdx.x * kSplitTargets + q * kSplitThreads + threadIdx.x;
 }
 
-// End of a split direct sum for this lane's targets ti[] with sums (sx, sy,
-// sz)[]: out = scale * sum over splits in split order.  partial holds
-// 3 n gridDim.y doubles, ticket one zero word per tile.
+// End of a split direct sum for this lane's targets: with one split the
+// block writes scale * sums; with more it stores its partial sums at
+// partial[split][3][n] for k_split_reduce.
 __device__ inline void split_finish(uint32_t n, double scale, const double (&sx)[kSplitPerLane],
                                     const double (&sy)[kSplitPerLane],
                                     const double (&sz)[kSplitPerLane], double *partial,
-                                    uint32_t *ticket, double *out_x, double *out_y,
-                                    double *out_z) {
-    __shared__ uint32_t s_last;
-    const uint32_t splits = gridDim.y;
-    if (splits == 1) {
-#pragma unroll
-        for (int q = 0; q < kSplitPerLane; ++q) {
-            const uint32_t i = split_target(q);
-            if (i < n) {
-                out_x[i] = sx[q] * scale;
-                out_y[i] = sy[q] * scale;
-                out_z[i] = sz[q] * scale;
-            }
-        }
-        return;
-    }
+                                    double *out_x, double *out_y, double *out_z) {
+    const bool direct = gridDim.y == 1;
     const uint64_t base = (uint64_t)blockIdx.y * 3ull * n;
 #pragma unroll
     for (int q = 0; q < kSplitPerLane; ++q) {
         const uint32_t i = split_target(q);
-        if (i < n) {
+        if (i >= n) continue;
+        if (direct) {
+            out_x[i] = sx[q] * scale;
+            out_y[i] = sy[q] * scale;
+            out_z[i] = sz[q] * scale;
+        } else {
             partial[base + i] = sx[q];
             partial[base + n + i] = sy[q];
             partial[base + 2ull * n + i] = sz[q];
         }
     }
-    // release: this block's partials reach device scope before its ticket
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(&ticket[blockIdx.x], 1u) + 1u == splits;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();  // acquire: the other splits' partials
-#pragma unroll
-    for (int q = 0; q < kSplitPerLane; ++q) {
-        const uint32_t i = split_target(q);
-        if (i >= n) continue;
-        double ox = 0, oy = 0, oz = 0;
-        for (uint32_t sp = 0; sp < splits; ++sp) {
-            const uint64_t b = (uint64_t)sp * 3ull * n;
-            ox += __hip_atomic_load(partial + b + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            oy += __hip_atomic_load(partial + b + n + i, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-            oz += __hip_atomic_load(partial + b + 2ull * n + i, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-        }
-        out_x[i] = ox * scale;
-        out_y[i] = oy * scale;
-        out_z[i] = oz * scale;
+}
+
+// out = scale * (sum over splits 0, 1, ... in that order) of the partial sums.
+__global__ void __launch_bounds__(kSplitThreads) k_split_reduce(uint32_t n, uint32_t splits,
+                                                                double scale,
+                                                                const double *__restrict__ partial,
+                                                                double *out_x, double *out_y,
+                                                                double *out_z) {
+    const uint32_t i = blockIdx.x * kSplitThreads + threadIdx.x;
+    if (i >= n) return;
+    double ox = 0, oy = 0, oz = 0;
+    for (uint32_t sp = 0; sp < splits; ++sp) {
+        const uint64_t b = (uint64_t)sp * 3ull * n;
+        ox += partial[b + i];
+        oy += partial[b + n + i];
+        oz += partial[b + 2ull * n + i];
     }
-    if (threadIdx.x == 0) ticket[blockIdx.x] = 0;
+    out_x[i] = ox * scale;
+    out_y[i] = oy * scale;
+    out_z[i] = oz * scale;
 }
 
 // FP32 pairs on absolute FP32 positions with G*m as the source weight; each
@@ -246,8 +238,7 @@ __device__ inline void split_finish(uint32_t n, double scale, const double (&sx)
 __global__ void __launch_bounds__(kSplitThreads) k_direct_f32c(
     uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, double G, float eps2,
-    uint32_t chunk, double *partial, uint32_t *ticket, double *ax_out, double *ay_out,
-    double *az_out) {
+    uint32_t chunk, double *partial, double *ax_out, double *ay_out, double *az_out) {
     __shared__ float4 tile[kSplitTile];
     float px[kSplitPerLane], py[kSplitPerLane], pz[kSplitPerLane];
     double ax[kSplitPerLane], ay[kSplitPerLane], az[kSplitPerLane];
@@ -264,19 +255,25 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f32c(
     const uint32_t j1 = min(n, j0 + chunk);
     for (uint32_t k = j0; k < j1; k += kSplitTile) {
         const uint32_t j = k + threadIdx.x;
-        tile[threadIdx.x] = j < j1 ? make_float4((float)x[j], (float)y[j], (float)z[j],
-                                                 (float)(G * m[j]))
-                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (j < j1)
+            tile[threadIdx.x] =
+                make_float4((float)x[j], (float)y[j], (float)z[j], (float)(G * m[j]));
         __syncthreads();
         float fx[kSplitPerLane], fy[kSplitPerLane], fz[kSplitPerLane];
 #pragma unroll
         for (int q = 0; q < kSplitPerLane; ++q) fx[q] = fy[q] = fz[q] = 0.f;
-#pragma unroll 8
-        for (int jj = 0; jj < kSplitTile; ++jj) {
+        auto pairs = [&](int jj) {
             const float4 sj = tile[jj];
 #pragma unroll
             for (int q = 0; q < kSplitPerLane; ++q)
                 pair_f32(sj, px[q], py[q], pz[q], eps2, fx[q], fy[q], fz[q]);
+        };
+        const uint32_t cnt = min((uint32_t)kSplitTile, j1 - k);  // uniform
+        if (cnt == (uint32_t)kSplitTile) {
+#pragma unroll 8
+            for (int jj = 0; jj < kSplitTile; ++jj) pairs(jj);
+        } else {
+            for (uint32_t jj = 0; jj < cnt; ++jj) pairs((int)jj);
         }
 #pragma unroll
         for (int q = 0; q < kSplitPerLane; ++q) {
@@ -286,17 +283,16 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f32c(
         }
         __syncthreads();
     }
-    split_finish(n, 1.0, ax, ay, az, partial, ticket, ax_out, ay_out, az_out);
+    split_finish(n, 1.0, ax, ay, az, partial, ax_out, ay_out, az_out);
 }
 
 // FP64 pairs (the reference's formula, NaiveAlgorithm.cpp:318-351, with the
-// inverse square root from v_rsq_f64 and a Newton step, and fused
-// multiply-adds); G is applied to the sum, as in the reference (:348-351).
+// inverse square root from rsqrt_f64 and fused multiply-adds); G is applied
+// to the sum, as in the reference (:348-351).
 __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
     uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, double G, double eps2,
-    uint32_t chunk, double *partial, uint32_t *ticket, double *ax_out, double *ay_out,
-    double *az_out) {
+    uint32_t chunk, double *partial, double *ax_out, double *ay_out, double *az_out) {
     __shared__ double4 tile[kSplitTile];
     double px[kSplitPerLane], py[kSplitPerLane], pz[kSplitPerLane];
     double ax[kSplitPerLane], ay[kSplitPerLane], az[kSplitPerLane];
@@ -313,20 +309,25 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
     const uint32_t j1 = min(n, j0 + chunk);
     for (uint32_t k = j0; k < j1; k += kSplitTile) {
         const uint32_t j = k + threadIdx.x;
-        // a padding source has zero mass and a far position: it adds 0
-        tile[threadIdx.x] = j < j1 ? make_double4(x[j], y[j], z[j], m[j])
-                                   : make_double4(1e300, 0.0, 0.0, 0.0);
+        if (j < j1) tile[threadIdx.x] = make_double4(x[j], y[j], z[j], m[j]);
         __syncthreads();
-#pragma unroll 4
-        for (int jj = 0; jj < kSplitTile; ++jj) {
+        auto pairs = [&](int jj) {
             const double4 sj = tile[jj];
 #pragma unroll
             for (int q = 0; q < kSplitPerLane; ++q)
                 pair_f64f(sj, px[q], py[q], pz[q], eps2, ax[q], ay[q], az[q]);
+        };
+        // the split's last stage may be partial: only its staged sources
+        const uint32_t cnt = min((uint32_t)kSplitTile, j1 - k);  // uniform
+        if (cnt == (uint32_t)kSplitTile) {
+#pragma unroll 4
+            for (int jj = 0; jj < kSplitTile; ++jj) pairs(jj);
+        } else {
+            for (uint32_t jj = 0; jj < cnt; ++jj) pairs((int)jj);
         }
         __syncthreads();
     }
-    split_finish(n, G, ax, ay, az, partial, ticket, ax_out, ay_out, az_out);
+    split_finish(n, G, ax, ay, az, partial, ax_out, ay_out, az_out);
 }
 
 // ---------------------------------------------------------------------------
@@ -438,6 +439,7 @@ __global__ void k_group_queue_reset(TreeHdr *h, uint32_t *gq) {
     if (threadIdx.x == 0) {
         h->group_next = 0;
         h->corr_pairs = 0;
+        h->quad_pairs = 0;
     }
     if (threadIdx.x < kXcds) gq[threadIdx.x * kQueueStride] = 0;
 }
@@ -486,6 +488,7 @@ struct GroupTree {
     float e2_root;   // root edge^2
     double *ax, *ay, *az;
     uint32_t *inter;
+    const float4 *gquad;  // G*Q per internal cell (quadrupole walks)
 };
 
 // The same for a stride known only at run time (lanes per body not a power
@@ -532,10 +535,68 @@ __device__ __forceinline__ void flush_share(const float4 *p, uint32_t t, const f
 // overflows to +inf, so rsqrt gives 0 and the force 0 * offset = 0.
 __device__ inline float4 far_entry() { return make_float4(1e30f, 0.f, 0.f, 0.f); }
 
+// A cell's monopole and traceless quadrupole, G folded into e.w = G M and
+// Q = G sum m (3 s s^T - |s|^2 I) about the cell's centre of mass, passed as
+// qa = {xx, yy, zz, xy}, qb = {xz, yz, -, -}.  With R = com - body (the pair
+// offset), the potential -G M / |R| - R.Q.R / (2 |R|^5) gives
+//   a = R (G M + 5/2 R.Q.R / R^2) / R^3 - Q.R / R^5,
+// with R^2 softened by eps^2 as in the monopole.  32 VALU instructions (the
+// monopole pair takes 13); 49 flop by the direct sum's counting.
+__device__ inline void pair_quad(const float4 e, const float4 qa, const float4 qb, float px,
+                                 float py, float pz, float eps2, float &fx, float &fy, float &fz) {
+    const float dx = e.x - px, dy = e.y - py, dz = e.z - pz;
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, eps2)));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float inv2 = inv * inv;
+    const float inv3 = inv * inv2;
+    const float qx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
+    const float qy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
+    const float qz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
+    const float rqr = fmaf(dx, qx, fmaf(dy, qy, dz * qz));
+    const float b = inv3 * inv2;                                // R^-5
+    const float a = fmaf((2.5f * rqr) * inv2, b, e.w * inv3);  // GM R^-3 + 5/2 RQR R^-7
+    fx = fmaf(a, dx, fx);
+    fy = fmaf(a, dy, fy);
+    fz = fmaf(a, dz, fz);
+    fx = fmaf(-b, qx, fx);
+    fy = fmaf(-b, qy, fy);
+    fz = fmaf(-b, qz, fz);
+}
+
+// Quadrupole pairs of cell entries p[0], p[LPB], ... (t of them, cell indices
+// k[0], k[LPB], ...; the cells' G*Q records in gq[2k], gq[2k+1]).  Four
+// entries' records are loaded ahead of their arithmetic.
+template <int LPB>
+__device__ __forceinline__ void qflush_share(const float4 *p, const uint32_t *k, uint32_t stride,
+                                             uint32_t t, const float4 *__restrict__ gq, float rx,
+                                             float ry, float rz, float eps2, float &fx, float &fy,
+                                             float &fz) {
+    const uint32_t st = LPB > 0 ? (uint32_t)LPB : stride;
+    uint32_t u = 0;
+    for (; u + 4 <= t; u += 4) {
+        float4 e[4], qa[4], qb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            e[v] = p[(u + v) * st];
+            const uint32_t c = k[(u + v) * st];
+            qa[v] = gq[2ull * c];
+            qb[v] = gq[2ull * c + 1];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) pair_quad(e[v], qa[v], qb[v], rx, ry, rz, eps2, fx, fy, fz);
+    }
+    for (; u < t; ++u) {
+        const uint32_t c = k[u * st];
+        pair_quad(p[u * st], gq[2ull * c], gq[2ull * c + 1], rx, ry, rz, eps2, fx, fy, fz);
+    }
+}
+
 // LDS of one wave's walk.
 struct GroupLds {
     uint32_t *stk;
     float4 *lst;
+    float4 *qlst;    // quadrupole: accepted cells' entries
+    uint32_t *qk;    // quadrupole: their internal-cell indices
     float *thr;      // opening threshold by depth, (edge / theta)^2 = ro2_root * 4^-depth
     float4 *cl_e;    // correction: accepted cell offset from the group centre, G*M
     uint32_t *cl_k;  // correction: accepted cell's internal index
@@ -549,12 +610,13 @@ struct GroupLds {
 // and the 64 - s * lpb lanes beyond repeat share 0 and their sums are dropped.
 // NB = 2 (64 < s <= 128): a lane holds bodies lane and lane + 64 and takes
 // the whole list for both.  *ncorr_out accumulates network evaluations.
-template <bool kCorr, int NB>
+template <bool kCorr, int NB, bool kQuad>
 __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, uint32_t s,
                                              float eps2, const CorrArgs &corr,
                                              const MlpForward &w, const GroupLds &L,
                                              uint32_t &peak, bool &overflow,
-                                             uint32_t &ncorr_out) {
+                                             uint32_t &ncorr_out, uint32_t &nquad_out) {
+    static_assert(!kQuad || NB == 1, "quadrupole walks hold one body per lane");
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t *__restrict__ sidx = T.sidx;
     const double *__restrict__ x = T.x, *__restrict__ y = T.y, *__restrict__ z = T.z,
@@ -568,6 +630,9 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
     float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
     float *thr = L.thr;
     uint32_t *cl_k = L.cl_k;
+    float4 *qlst = L.qlst;
+    uint32_t *qk = L.qk;
+    const float4 *__restrict__ gquad = T.gquad;
     {
         const uint32_t ss = NB == 2 ? (uint32_t)kGroup : s;  // body slots
         const uint32_t sub = NB == 2 ? lane : lane % ss;      // which body of the group
@@ -649,6 +714,8 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
         uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
         uint32_t cn = 0;          // wave-uniform size of the correction's cell list
         uint32_t ncorr = 0;       // network evaluations of this group
+        uint32_t qn = 0;          // wave-uniform size of the quadrupole cell list
+        uint32_t nquad = 0;       // quadrupole pairs of this group
         if constexpr (kCorr) {
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
@@ -699,6 +766,41 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 acc_z[b] += (double)fz[b];
             }
             n_inter += cnt;
+        };
+
+        // Quadrupole cells qlst[base .. base + cnt), shared out over the lanes
+        // of a body as in flush.
+        auto qflush = [&](uint32_t base, uint32_t cnt) {
+            const uint32_t full = cnt / lpb;
+            const uint32_t rem = cnt - full * lpb;
+            const float4 *p = qlst + base + qs;
+            const uint32_t *pk = qk + base + qs;
+            float fx = 0.f, fy = 0.f, fz = 0.f;
+            switch (lpb) {
+            case 1: qflush_share<1>(p, pk, 1, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 2: qflush_share<2>(p, pk, 2, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 4: qflush_share<4>(p, pk, 4, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 8: qflush_share<8>(p, pk, 8, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 16: qflush_share<16>(p, pk, 16, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 32: qflush_share<32>(p, pk, 32, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            case 64: qflush_share<64>(p, pk, 64, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            default: qflush_share<0>(p, pk, lpb, full, gquad, rx[0], ry[0], rz[0], eps2, fx, fy, fz); break;
+            }
+            if (rem) {
+                // shares past the remainder take a zero-force entry with a zero
+                // quadrupole (a far entry with a real Q would give inf * 0)
+                const bool real = qs < rem;
+                const uint32_t c = real ? pk[full * lpb] : 0u;
+                const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+                const float4 qa = gquad[2ull * c], qb = gquad[2ull * c + 1];
+                pair_quad(real ? p[full * lpb] : far_entry(), real ? qa : zero, real ? qb : zero,
+                          rx[0], ry[0], rz[0], eps2, fx, fy, fz);
+            }
+            acc_x[0] += (double)fx;
+            acc_y[0] += (double)fy;
+            acc_z[0] += (double)fz;
+            n_inter += cnt;
+            nquad += cnt * s;
         };
 
         // learned correction of cells cl[base .. base+cnt) (cnt <= 16) for every body
@@ -832,6 +934,18 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 classify(r[k], depth[k], e, d2, to_list, to_open);
                 to_list = to_list && live[k];
                 to_open = to_open && live[k];
+                if constexpr (kQuad) {
+                    // accepted cells to the quadrupole list, body leaves stay
+                    const bool tq = to_list && r[k].ref >= 0;
+                    const uint64_t bq = __ballot(tq);
+                    if (tq) {
+                        const uint32_t p = qn + count_below(bq);
+                        qlst[p] = e;
+                        qk[p] = (uint32_t)r[k].ref;
+                    }
+                    qn += __popcll(bq);  // < chunk before: at most kListCap - 1 after
+                    to_list = to_list && !tq;
+                }
                 const uint64_t bl = __ballot(to_list);
                 const uint64_t bo = __ballot(to_open);
                 if (to_list) lst[ln + count_below(bl)] = e;
@@ -870,6 +984,20 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 ln = rest;
                 __syncthreads();
             }
+            if constexpr (kQuad) {
+                if (qn >= chunk) {
+                    uint32_t done = 0;
+                    for (; qn - done >= chunk; done += chunk) qflush(done, chunk);
+                    __syncthreads();
+                    const uint32_t rest = qn - done;  // < chunk <= done: no overlap
+                    if (lane < rest) {
+                        qlst[lane] = qlst[done + lane];
+                        qk[lane] = qk[done + lane];
+                    }
+                    qn = rest;
+                    __syncthreads();
+                }
+            }
             if constexpr (kCorr) {
                 if (cn >= (uint32_t)kCorrTile) {
                     uint32_t base = 0;
@@ -887,6 +1015,9 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             }
         }
         if (ln > 0) flush(0, ln);
+        if constexpr (kQuad) {
+            if (qn > 0) qflush(0, qn);
+        }
         if constexpr (kCorr) {
             if (cn > 0) corr_flush(0, cn);
         }
@@ -912,6 +1043,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             acc_z[0] = tz;
         }
         ncorr_out += ncorr;
+        nquad_out += nquad;
         // Stored also after an overflow (the caller reports it and the values
         // are then undefined): a predicate holding `overflow` would let the
         // compiler route the storing lanes past the caller's exit test, a
@@ -936,20 +1068,22 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
 
 // Forces on the bodies of one group grp = {first sorted position, count}
 // (checked by the caller): one body per lane up to 64 bodies, two beyond.
-template <bool kCorr>
+template <bool kCorr, bool kQuad = false>
 __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float eps2,
                                            const CorrArgs &corr, const MlpForward &w,
                                            const GroupLds &L, uint32_t &peak, bool &overflow,
-                                           uint32_t &ncorr_out) {
+                                           uint32_t &ncorr_out, uint32_t &nquad_out) {
     const uint32_t g0 = wave_uniform(grp.x);
     const uint32_t s = wave_uniform(grp.y);
-    if constexpr (kGroupMax > (uint32_t)kGroup) {
+    if constexpr (kGroupMax > (uint32_t)kGroup && !kQuad) {
         if (s > (uint32_t)kGroup) {
-            group_walk_n<kCorr, 2>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out);
+            group_walk_n<kCorr, 2, false>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out,
+                                          nquad_out);
             return;
         }
     }
-    group_walk_n<kCorr, 1>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out);
+    group_walk_n<kCorr, 1, kQuad>(T, g0, s, eps2, corr, w, L, peak, overflow, ncorr_out,
+                                  nquad_out);
 }
 
 // a group {first, count} that a walk may trust
@@ -957,15 +1091,17 @@ __device__ inline bool group_ok(uint2 grp, uint32_t n) {
     return grp.y != 0 && grp.y <= kGroupMax && grp.x < n && grp.y <= n - grp.x;
 }
 
-template <bool kCorr>
+template <bool kCorr, bool kQuad>
 __global__ void __launch_bounds__(kGroup) k_bh_group(
     uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
     float eps2, uint32_t icap, double *ax, double *ay, double *az, uint32_t *inter,
-    TreeHdr *h, uint32_t *gq, CorrArgs corr) {
+    TreeHdr *h, uint32_t *gq, CorrArgs corr, const float4 *__restrict__ gquad) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
+    __shared__ float4 qlst[kQuad ? kListCap : 1];
+    __shared__ uint32_t qk[kQuad ? kListCap : 1];
     __shared__ float4 cl_e[kCorr ? kCellCap : 1];
     __shared__ uint32_t cl_k[kCorr ? kCellCap : 1];
     // bcorr is cleared lane by lane (lane + 64 b): at least one wave of entries
@@ -973,16 +1109,16 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     __shared__ float4 boff[kCorr ? kGroupMax : 1];
     __shared__ float4 bcorr[kCorr ? kCorrSlots : 1];
     __shared__ float thr[kGroup];
-    const GroupLds L{stk, lst, thr, cl_e, cl_k, boff, bcorr};
+    const GroupLds L{stk, lst, qlst, qk, thr, cl_e, cl_k, boff, bcorr};
     const uint32_t lane = threadIdx.x;
     const uint32_t ngroups = min(h->group_count, n);
     // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
     const double ro_root = h->edge / theta;
     const GroupTree T{n, sidx, x, y, z, m, rec, icap, (float)(ro_root * ro_root),
-                      (float)(h->edge * h->edge), ax, ay, az, inter};
+                      (float)(h->edge * h->edge), ax, ay, az, inter, gquad};
     MlpForward w;
     if constexpr (kCorr) w.load(corr.pack, corr.master, (int)lane);
-    uint32_t peak = 0, ncorr = 0;
+    uint32_t peak = 0, ncorr = 0, nquad = 0;
     bool overflow = false;
 
     // Groups come in octree order: waves running at the same time walk
@@ -1016,11 +1152,14 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
             overflow = true;  // corrupt group table: report instead of reading out of bounds
             break;
         }
-        group_walk<kCorr>(T, grp, eps2, corr, w, L, peak, overflow, ncorr);
+        group_walk<kCorr, kQuad>(T, grp, eps2, corr, w, L, peak, overflow, ncorr, nquad);
         if (overflow) break;
     }
     if constexpr (kCorr) {
         if (lane == 0 && ncorr) atomicAdd(&h->corr_pairs, (unsigned long long)ncorr);
+    }
+    if constexpr (kQuad) {
+        if (lane == 0 && nquad) atomicAdd(&h->quad_pairs, (unsigned long long)nquad);
     }
     if (overflow) raise_flag(h, kFlagStack);
     if (lane == 0) raise_max(&h->stack_peak, peak);
@@ -1086,12 +1225,12 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
     __shared__ float thr[kGroup];
-    const GroupLds L{stk, lst, thr, nullptr, nullptr, nullptr, nullptr};
+    const GroupLds L{stk, lst, nullptr, nullptr, thr, nullptr, nullptr, nullptr, nullptr};
     const uint32_t lane = threadIdx.x;
     const uint32_t total = a.item_off[a.nq];
     const MlpForward w{};
     const CorrArgs corr{};
-    uint32_t peak = 0, ncorr = 0;
+    uint32_t peak = 0, ncorr = 0, nquad = 0;
     bool overflow = false;
     for (;;) {
         uint32_t slot = 0;
@@ -1116,13 +1255,13 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
             const GroupTree T{S.n, a.fb.sidx + S.fb_off, a.x + bo, a.y + bo, a.z + bo, a.m + bo,
                               a.fb.rec + S.node_off, S.icap, (float)(ro * ro),
                               (float)(edge * edge), a.ax + bo, a.ay + bo, a.az + bo,
-                              a.inter + bo};
+                              a.inter + bo, nullptr};
             const uint2 grp = a.fb.groups[S.fb_off + local];
             if (!group_ok(grp, S.n)) {
                 overflow = true;
                 break;
             }
-            group_walk<false>(T, grp, a.eps2, corr, w, L, peak, overflow, ncorr);
+            group_walk<false>(T, grp, a.eps2, corr, w, L, peak, overflow, ncorr, nquad);
             if (overflow) break;
         } else {
             direct_tile(a.x + bo, a.y + bo, a.z + bo, a.m + bo, S.n, local, a.G, a.eps2, a.ax + bo,
@@ -1134,19 +1273,7 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
     if (overflow && lane == 0) atomicOr(a.status, kFlagStack);
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint64_t words, uint32_t value) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
-         i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = value;
-}
-
 }  // namespace
-
-void launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s) {
-    if (words == 0) return;
-    const uint32_t blocks = (uint32_t)std::min<uint64_t>(1024, (words + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, s, p, words, value);
-}
 
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
                        const double *m, double G, double eps2, double *ax, double *ay,
@@ -1174,36 +1301,29 @@ uint32_t direct_split_count(uint32_t n) {
     return std::max<uint32_t>(1, std::min(splits, max_splits));
 }
 
-// scratch layout: partial sums [split][3][n] doubles (none with one split),
-// then one ticket word per tile, each part starting on a 256-byte line
-uint64_t direct_split_ticket_offset(uint32_t n) {
-    const uint32_t splits = direct_split_count(n);
-    const uint64_t partial = splits > 1 ? 3ull * n * splits * sizeof(double) : 0;
-    return (partial + 255) & ~255ull;
-}
-
+// scratch: the partial sums [split][3][n] doubles (none with one split)
 uint64_t direct_split_scratch_bytes(uint32_t n) {
-    const uint64_t tickets = (uint64_t)ceil_div(n, kSplitTargets) * sizeof(uint32_t);
-    return direct_split_ticket_offset(n) + ((tickets + 255) & ~255ull);
+    const uint32_t splits = direct_split_count(n);
+    return splits > 1 ? 3ull * n * splits * sizeof(double) : 0;
 }
 
 void launch_direct_split(bool fp64, uint32_t n, const double *x, const double *y,
                          const double *z, const double *m, double G, double eps2, double *ax,
-                         double *ay, double *az, void *scratch, hipStream_t s) {
+                         double *ay, double *az, double *partial, hipStream_t s) {
     if (n == 0) return;
     const uint32_t splits = direct_split_count(n);
     const uint32_t tiles = ceil_div(n, kSplitTargets);
-    double *partial = static_cast<double *>(scratch);
-    uint32_t *ticket = reinterpret_cast<uint32_t *>(static_cast<char *>(scratch) +
-                                                    direct_split_ticket_offset(n));
     const uint32_t chunk = ceil_div(ceil_div(n, splits), kSplitTile) * kSplitTile;
     const dim3 grid(tiles, splits);
     if (fp64)
         hipLaunchKernelGGL(k_direct_f64f, grid, dim3(kSplitThreads), 0, s, n, x, y, z, m, G, eps2,
-                           chunk, partial, ticket, ax, ay, az);
+                           chunk, partial, ax, ay, az);
     else
         hipLaunchKernelGGL(k_direct_f32c, grid, dim3(kSplitThreads), 0, s, n, x, y, z, m, G,
-                           (float)eps2, chunk, partial, ticket, ax, ay, az);
+                           (float)eps2, chunk, partial, ax, ay, az);
+    if (splits > 1)
+        hipLaunchKernelGGL(k_split_reduce, dim3(ceil_div(n, kSplitThreads)), dim3(kSplitThreads),
+                           0, s, n, splits, fp64 ? G : 1.0, (const double *)partial, ax, ay, az);
 }
 
 void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const double *z,
@@ -1217,26 +1337,38 @@ void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const 
 }
 
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     const double *m, double eps2, double theta, double *ax, double *ay,
-                     double *az, uint32_t *interactions, hipStream_t s) {
+                     const double *m, double eps2, double theta, const float4 *gquad, double *ax,
+                     double *ay, double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
     hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
-    hipLaunchKernelGGL(k_bh_group<false>, dim3(blocks), dim3(kGroup), 0, s, t.n,
-                       (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
-                       (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
-                       interactions, t.hdr, t.gq, CorrArgs{});
+    if (gquad)
+        hipLaunchKernelGGL((k_bh_group<false, true>), dim3(blocks), dim3(kGroup), 0, s, t.n,
+                           (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
+                           (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                           interactions, t.hdr, t.gq, CorrArgs{}, gquad);
+    else
+        hipLaunchKernelGGL((k_bh_group<false, false>), dim3(blocks), dim3(kGroup), 0, s, t.n,
+                           (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
+                           (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                           interactions, t.hdr, t.gq, CorrArgs{}, (const float4 *)nullptr);
 }
 
 void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, const double *z,
                           const double *m, double eps2, double theta, const CorrArgs &corr,
-                          double *ax, double *ay, double *az, uint32_t *interactions,
-                          hipStream_t s) {
+                          const float4 *gquad, double *ax, double *ay, double *az,
+                          uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
     hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
-    hipLaunchKernelGGL(k_bh_group<true>, dim3(blocks), dim3(kGroup), 0, s, t.n,
-                       (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
-                       (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
-                       interactions, t.hdr, t.gq, corr);
+    if (gquad)
+        hipLaunchKernelGGL((k_bh_group<true, true>), dim3(blocks), dim3(kGroup), 0, s, t.n,
+                           (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
+                           (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                           interactions, t.hdr, t.gq, corr, gquad);
+    else
+        hipLaunchKernelGGL((k_bh_group<true, false>), dim3(blocks), dim3(kGroup), 0, s, t.n,
+                           (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
+                           (const uint2 *)t.groups, theta, (float)eps2, t.icap, ax, ay, az,
+                           interactions, t.hdr, t.gq, corr, (const float4 *)nullptr);
 }
 
 void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items,

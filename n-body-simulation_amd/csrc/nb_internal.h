@@ -52,6 +52,7 @@ struct alignas(128) TreeHdr {
     uint32_t group_next;                               // traversal work-queue cursor
     uint32_t cross_count;                              // cells crossing super-block boundaries
     unsigned long long corr_pairs;                     // learned-correction evaluations
+    unsigned long long quad_pairs;                     // quadrupole cell interactions
 };
 
 // Bodies per traversal group: a group is an octree cell holding at most
@@ -255,24 +256,22 @@ void launch_direct_targets_f64(uint32_t n, const double *x, const double *y, con
                                const uint32_t *targets, double *ax, double *ay, double *az,
                                hipStream_t s);
 // Split direct sum (FP32 pairs with FP64 accumulation, or fp64: FP64 pairs
-// with v_rsq_f64 and a Newton step), one launch, deterministic.  scratch
-// holds direct_split_scratch_bytes(n) bytes (256-byte aligned) whose ticket
-// words are zero before the first launch; every launch leaves them zero.
+// with a corrected v_rsq_f64 estimate), deterministic; partial holds
+// direct_split_scratch_bytes(n) bytes (none needed with one split).
 void launch_direct_split(bool fp64, uint32_t n, const double *x, const double *y,
                          const double *z, const double *m, double G, double eps2, double *ax,
-                         double *ay, double *az, void *scratch, hipStream_t s);
+                         double *ay, double *az, double *partial, hipStream_t s);
 uint32_t direct_split_count(uint32_t n);
 uint64_t direct_split_scratch_bytes(uint32_t n);
-uint64_t direct_split_ticket_offset(uint32_t n);  // bytes before the ticket words
-// zero `words` 32-bit words with a kernel (no memset node under graph capture)
-void launch_fill_u32(uint32_t *p, uint64_t words, uint32_t value, hipStream_t s);
 void launch_bh_exact(const TreeBufs &t, const double *x, const double *y, const double *z,
                      double G, double eps2, double theta, int sort_bodies, uint32_t *stack,
                      uint32_t stack_cap, double *ax, double *ay, double *az,
                      uint32_t *interactions, hipStream_t s);
+// gquad: G*Q records of the internal cells (two float4 each) for the
+// quadrupole walk, or null for monopole cells
 void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const double *z,
-                     const double *m, double eps2, double theta, double *ax, double *ay,
-                     double *az, uint32_t *interactions, hipStream_t s);
+                     const double *m, double eps2, double theta, const float4 *gquad, double *ax,
+                     double *ay, double *az, uint32_t *interactions, hipStream_t s);
 
 // per-body kinetic and potential energy terms (reference arithmetic)
 void launch_energy_terms(uint32_t n, const double *x, const double *y, const double *z,

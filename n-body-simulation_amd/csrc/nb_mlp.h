@@ -202,8 +202,10 @@ struct CorrArgs {
     float tau2;            // correct cells with e^2 > tau2 * d_group^2
 };
 
+// quad: 6 FP64 per internal cell (needed); mcell, gquad: optional outputs
 void launch_quadrupoles(const TreeBufs &t, const double *x, const double *y, const double *z,
-                        const double *m, double *quad, MlpCell *mcell, hipStream_t s);
+                        const double *m, double *quad, MlpCell *mcell, float4 *gquad, double G,
+                        hipStream_t s);
 void launch_pair_walk(const TreeBufs &t, uint32_t nt, const uint32_t *targets, const double *x,
                       const double *y, const double *z, double theta, uint32_t *stack,
                       uint32_t stack_cap, uint64_t cap, uint32_t *pair_cell, uint32_t *pair_body,
@@ -215,7 +217,7 @@ void launch_pair_targets(const TreeBufs &t, uint64_t count, const uint32_t *pair
                          float *targets, hipStream_t s);
 void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, const double *z,
                           const double *m, double eps2, double theta, const CorrArgs &corr,
-                          double *ax, double *ay, double *az, uint32_t *interactions,
-                          hipStream_t s);
+                          const float4 *gquad, double *ax, double *ay, double *az,
+                          uint32_t *interactions, hipStream_t s);
 
 }  // namespace nb

@@ -308,13 +308,15 @@ __global__ void __launch_bounds__(kThreads) k_mlp_reduce(uint32_t ksplit,
 // ---- quadrupoles and feature records ------------------------------------------------
 // Cells of depth `level`: Q = sum over children of Q_child + m_child (3 s s^T
 // - |s|^2 I), s = child com - cell com (parallel axis); a body child has
-// Q_child = 0 and s = x - com.  Launched deepest level first.
+// Q_child = 0 and s = x - com.  Launched deepest level first.  Optional
+// outputs: the network's feature record (mcell) and the traversal's FP32
+// record G*Q = {xx, yy, zz, xy}, {xz, yz, 0, 0} (gquad, two per cell).
 __global__ void __launch_bounds__(kThreads) k_quad_level(
     int level, uint32_t icap, const TreeHdr *__restrict__ h, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const int32_t *__restrict__ ref, const InternalSums *__restrict__ sums,
     const uint8_t *__restrict__ int_depth, const uint32_t *__restrict__ int_first,
-    const uint32_t *__restrict__ sidx, double *quad, MlpCell *mcell) {
+    const uint32_t *__restrict__ sidx, double *quad, MlpCell *mcell, float4 *gquad, double G) {
     if ((uint32_t)level >= h->max_depth) return;  // no internal cell this deep
     const uint32_t I = min(h->internal_count, icap);
     const uint64_t stride = (uint64_t)gridDim.x * kThreads;
@@ -351,6 +353,12 @@ __global__ void __launch_bounds__(kThreads) k_quad_level(
             q[5] += mc * 3 * sy * sz;
         }
         for (int u = 0; u < 6; ++u) quad[6 * k + u] = q[u];
+        if (gquad) {
+            gquad[2 * k] = make_float4((float)(G * q[0]), (float)(G * q[1]), (float)(G * q[2]),
+                                       (float)(G * q[3]));
+            gquad[2 * k + 1] = make_float4((float)(G * q[4]), (float)(G * q[5]), 0.f, 0.f);
+        }
+        if (!mcell) continue;
         // geometric cell: replay the classification chain of the cell's first body
         const uint32_t b0 = sidx[int_first[k]];
         Cell c{h->minx, h->miny, h->minz, h->edge};
@@ -494,14 +502,16 @@ __global__ void __launch_bounds__(64) k_pair_targets(
 }  // namespace
 
 void launch_quadrupoles(const TreeBufs &t, const double *x, const double *y, const double *z,
-                        const double *m, double *quad, MlpCell *mcell, hipStream_t s) {
+                        const double *m, double *quad, MlpCell *mcell, float4 *gquad, double G,
+                        hipStream_t s) {
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(4096, ceil_div(t.icap, kThreads)));
     launch_max_depth(t, s);  // k_quad_level skips the levels below the deepest leaf
     for (int level = kMaxDepth - 1; level >= 0; --level)
         hipLaunchKernelGGL(k_quad_level, dim3(blocks), dim3(kThreads), 0, s, level, t.icap,
                            (const TreeHdr *)t.hdr, x, y, z, m, (const int32_t *)t.ref,
                            (const InternalSums *)t.int_sums, (const uint8_t *)t.int_depth,
-                           (const uint32_t *)t.int_first, (const uint32_t *)t.sidx, quad, mcell);
+                           (const uint32_t *)t.int_first, (const uint32_t *)t.sidx, quad, mcell,
+                           gquad, G);
 }
 
 void launch_pair_walk(const TreeBufs &t, uint32_t nt, const uint32_t *targets, const double *x,

@@ -44,7 +44,7 @@ EXPORTS = [
     "nb_mlp_default_train_opts", "nb_mlp_create", "nb_mlp_destroy", "nb_mlp_last_error",
     "nb_mlp_set_params", "nb_mlp_get_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_mlp_adam_step", "nb_mlp_train", "nb_mlp_pairs", "nb_set_correction",
-    "nb_tree_quadrupoles", "nb_correction_pairs",
+    "nb_tree_quadrupoles", "nb_correction_pairs", "nb_quadrupole_pairs",
 ]
 
 MLP_IN, MLP_OUT, MLP_PARAMS = 13, 3, 9411
@@ -60,7 +60,8 @@ class TrainOpts(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("G", ctypes.c_double), ("eps2", ctypes.c_double), ("theta", ctypes.c_double),
                 ("precision", ctypes.c_uint32), ("sort_bodies", ctypes.c_uint32),
-                ("storage_size_param", ctypes.c_uint32), ("stack_size_param", ctypes.c_uint32)]
+                ("storage_size_param", ctypes.c_uint32), ("stack_size_param", ctypes.c_uint32),
+                ("quadrupole", ctypes.c_uint32)]
 
 
 class NBodyError(RuntimeError):
@@ -134,6 +135,7 @@ def load(path=LIB_PATH):
         "nb_set_correction": (ctypes.c_int, [vp, vp, ctypes.c_double]),
         "nb_tree_quadrupoles": (ctypes.c_uint64, [vp, ctypes.c_uint64, _dp]),
         "nb_correction_pairs": (ctypes.c_int, [vp, _u64p]),
+        "nb_quadrupole_pairs": (ctypes.c_int, [vp, _u64p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -367,6 +369,12 @@ class Context:
     def correction_pairs(self):
         out = ctypes.c_uint64()
         self._check(self.lib.nb_correction_pairs(self.h, ctypes.byref(out)), "nb_correction_pairs")
+        return out.value
+
+    def quadrupole_pairs(self):
+        """Cell interactions evaluated with the quadrupole term in the last traversal."""
+        out = ctypes.c_uint64()
+        self._check(self.lib.nb_quadrupole_pairs(self.h, ctypes.byref(out)), "nb_quadrupole_pairs")
         return out.value
 
     def set_correction(self, mlp, tau=0.0):
