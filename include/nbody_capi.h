@@ -155,6 +155,25 @@ int nb_kick(nb_ctx *ctx, double dt);
  * end.  algorithm = NB_ALGO_*. */
 int nb_steps(nb_ctx *ctx, int algorithm, double dt, uint32_t count);
 
+/* Direct sum without any allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  It runs the kernels of
+ * nb_forces_direct (precision NB_PREC_FP32C or NB_PREC_FP64_FAST) on DEVICE
+ * arrays x, y, z, m (n doubles each) into DEVICE arrays ax, ay, az, scaled by
+ * G with softening eps2, enqueued on `stream` (a hipStream_t; NULL is the
+ * null stream), and returns without waiting.  The partial sums of the
+ * source splits (3 * 8 * n * splits bytes, at most 24 MiB for every n) live in
+ * a library-owned device array of NB_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; n above NB_NOALLOC_MAX_BODIES is NB_E_INVALID_ARG.  All
+ * calls on one GPU share that array: calls that overlap on one GPU race and
+ * give undefined results.  The first call on a device looks the array up
+ * (hipGetSymbolAddress), which allocates nothing. */
+#define NB_NOALLOC_SCRATCH_BYTES (24ull * 1024 * 1024)
+#define NB_NOALLOC_MAX_BODIES (1u << 23)
+int nb_forces_direct_noalloc(uint32_t n, const double *x, const double *y, const double *z,
+                             const double *m, double G, double eps2, uint32_t precision,
+                             double *ax, double *ay, double *az, void *stream);
+
 /* System energies at the current state (nBodyAlgorithm::computeEnergy,
  * nBodyAlgorithm.cpp:11-86): kinetic sum of 0.5*m*v^2, potential
  * -sum_{i<j} G m_i m_j / r without softening, per-body terms summed in body

@@ -26,6 +26,8 @@ NB_E_NO_DEVICE = 10
 PREC_FP64 = 0
 PREC_FP32C = 1
 PREC_FP64_FAST = 2
+NOALLOC_SCRATCH_BYTES = 24 * 1024 * 1024
+NOALLOC_MAX_BODIES = 1 << 23
 ALGO_NAIVE = 0
 ALGO_BARNES_HUT = 1
 
@@ -44,7 +46,7 @@ EXPORTS = [
     "nb_mlp_default_train_opts", "nb_mlp_create", "nb_mlp_destroy", "nb_mlp_last_error",
     "nb_mlp_set_params", "nb_mlp_get_params", "nb_mlp_forward", "nb_mlp_grad",
     "nb_mlp_adam_step", "nb_mlp_train", "nb_mlp_pairs", "nb_set_correction",
-    "nb_tree_quadrupoles", "nb_correction_pairs", "nb_quadrupole_pairs",
+    "nb_tree_quadrupoles", "nb_correction_pairs", "nb_forces_direct_noalloc", "nb_quadrupole_pairs",
 ]
 
 MLP_IN, MLP_OUT, MLP_PARAMS = 13, 3, 9411
@@ -104,6 +106,9 @@ def load(path=LIB_PATH):
         "nb_snapshot_wait": (ctypes.c_int, [vp, ctypes.POINTER(_dp)]),
         "nb_set_positions": (ctypes.c_int, [vp] + [_dp] * 3),
         "nb_forces_direct": (ctypes.c_int, [vp]),
+        "nb_forces_direct_noalloc": (ctypes.c_int, [ctypes.c_uint32] + [ctypes.c_void_p] * 4 +
+                                     [ctypes.c_double, ctypes.c_double, ctypes.c_uint32] +
+                                     [ctypes.c_void_p] * 4),
         "nb_forces_direct_targets": (ctypes.c_int, [vp, ctypes.c_uint32, _u32p] + [_dp] * 3),
         "nb_bh_build": (ctypes.c_int, [vp]),
         "nb_bh_forces": (ctypes.c_int, [vp]),
