@@ -61,11 +61,15 @@ ENSEMBLE = dict(nsys=512, seed=0x4E424F44590004, lo_log2=10, hi_log2=16, theta=0
 # seconds of CPU loop time the reference CPU path is timed for
 CPU_SAMPLE_S = 10.0
 
-# HBM bytes per launch of the force kernel, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE runs of this script scaled by a device copy of known size
-# (profiles/bh_disk_1m_hbm_counters.md: FETCH x 2.0, WRITE x 1.0):
-# 565,982 KiB x 2 read + 134,447 KiB written.
+# HBM bytes per launch of the force kernel, keyed by (workload, theta), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script scaled by a device
+# copy of known size (FETCH x 2.0, WRITE x 1.0):
+#  - 4M Plummer, theta 0.36: 4,369,612 KiB x 2 read + 531,218 KiB written
+#    (profiles/bh_plummer_4m_hbm_counters.md);
+#  - 1M disk, theta 0.5: 565,982 KiB x 2 read + 134,447 KiB written
+#    (profiles/bh_disk_1m_hbm_counters.md).
 TRAFFIC_BYTES = {
+    ("bh_plummer_4m", 0.36): 9.493e9,
     ("bh_disk_1m", 0.5): 1.297e9,
 }
 
