@@ -49,9 +49,12 @@ THETA_SWEEP = (0.3, 0.5, 0.7, 1.0)
 # Learned correction: training set (Plummer snapshots, accepted (cell, body)
 # pairs of a per-body walk at theta 1.0, targets from the exact cell sums) and
 # the plain Barnes-Hut opening angle it is compared with.
-MLP_TRAIN = dict(snapshots=4, n=262144, targets=1800, pairs=2_000_000, theta=1.0, epochs=40,
+MLP_TRAIN = dict(snapshots=4, n=262144, targets=1800, pairs=2_000_000, theta=0.5, epochs=40,
                  batch=4096)
-MLP_PLAIN_THETA = 0.4
+# corrected traversals tried: opening angles above the plain one's, and the
+# gate tau (cells with e / d_group > tau get the network) as fractions of theta
+MLP_THETA_STEPS = (0.0, 0.02, 0.04, 0.07, 0.1, 0.15)
+MLP_TAU_FRACS = (0.0, 0.7, 0.85)
 
 # Ensemble: 512 independent systems, sizes floor(2^U[10, 16]), alternating
 # star + disk (dt = 1 h) and Plummer (dt = 1e-3); direct sum below the
@@ -85,8 +88,9 @@ WORKLOADS = {
     "naive_disk_16k_fp32": ("disk", 16384, 0x4E424F44590000, "naive", 0.5, 1.0 / 24, nb.PREC_FP32C),
     # theta None: the largest candidate meeting the error bound (choose_theta)
     "bh_plummer_4m": ("plummer", 1 << 22, 0x4E424F44590002, "BarnesHut", None, 1e-3, nb.PREC_FP32C),
-    # theta 1.0 with the learned far-field correction on every accepted cell
-    "bh_plummer_4m_mlp": ("plummer", 1 << 22, 0x4E424F44590003, "BarnesHut", 1.0, 1e-3,
+    # the learned far-field correction against plain Barnes-Hut, both at the
+    # error bound (run_learned)
+    "bh_plummer_4m_mlp": ("plummer", 1 << 22, 0x4E424F44590003, "BarnesHut", None, 1e-3,
                           nb.PREC_FP32C),
 }
 
@@ -277,6 +281,25 @@ def choose_theta(ctx, d, dt):
     raise RuntimeError(f"no candidate opening angle meets {ERR_GATE}: {trials}")
 
 
+def energy_cost(n=1 << 20, reps=3):
+    """Time nb_energy (the --energy diagnostics, nBodyAlgorithm.cpp:11-86) on
+    the 1M-body star + disk: FP64 pair terms in the reference's order, N^2/2
+    pairs, then the host sums.  Reports wall ms per call; the kernel's own
+    time comes from rocprofv3 --kernel-trace of this mode."""
+    d = gen("disk", n, WORKLOADS["bh_disk_1m"][2])
+    with nb.Context(n) as ctx:
+        ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+        ctx.energy()  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e = ctx.energy()
+        ms = (time.perf_counter() - t0) / reps * 1e3
+    pairs = n * (n - 1) / 2
+    print(json.dumps({"energy_cost": "ms per nb_energy call (kernel, copies, host sums)",
+                      "bodies": n, "pairs": pairs, "ms": round(ms, 3), "reps": reps,
+                      "virial": e["virial"]}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,7 +323,14 @@ def main():
     ap.add_argument("--snapshot-cost", action="store_true",
                     help="only time a visualised step of the 1M-body disk workload: synchronous "
                          "reads against the asynchronous snapshot")
+    ap.add_argument("--mlp-train-theta", type=float, default=MLP_TRAIN["theta"],
+                    help="opening angle of the per-body walks that collect the training pairs")
+    ap.add_argument("--energy-cost", action="store_true",
+                    help="only time the energy diagnostics on the 1M-body disk")
     args = ap.parse_args()
+    if args.energy_cost:
+        energy_cost()
+        return
     if args.snapshot_cost:
         snapshot_cost()
         return
@@ -317,9 +347,10 @@ def main():
     kind, n, seed, algorithm, theta, dt, prec = WORKLOADS[args.workload]
     if args.theta is not None:
         theta = args.theta
-    mlp = args.workload.endswith("_mlp")
+    if args.workload.endswith("_mlp"):
+        run_learned(args)
+        return
     log(f"device: {nb.device_name()}")
-    net, train_info = (train_correction() if mlp else (None, None))
     log(f"workload {args.workload}: generating {kind} n={n} ...")
     d = gen(kind, n, seed)
     ctx = nb.Context(n, theta=theta if theta is not None else 0.5, precision=prec,
@@ -329,19 +360,13 @@ def main():
         theta, theta_search = choose_theta(ctx, d, dt)
         ctx.set_params(theta=theta)
         log(f"opening angle: {theta}")
-    if net is not None:
-        ctx.set_correction(net, 0.0)
     secs, phases, err, sample, inter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
     raw_value = n * args.steps / secs
     value, gate_met = gated(raw_value, err)
 
     # roofline of the dominant kernel: the force kernel
     accel_ms = phases["Acceleration Kernel Time"] / args.steps
-    if mlp:
-        pairs = ctx.correction_pairs()  # network evaluations of the last traversal
-        flops = MLP_FLOP_PER_PAIR * float(pairs)
-        peak, bound, kernel = BF16_PEAK_TFLOPS, "mfma", "k_bh_group<corrected>"
-    elif algorithm == "BarnesHut":
+    if algorithm == "BarnesHut":
         # quadrupole cell pairs count 49 flop (the direct sum's counting of
         # their arithmetic), monopole cells and bodies 20
         qpairs = float(ctx.quadrupole_pairs()) if args.quadrupole else 0.0
@@ -363,7 +388,7 @@ def main():
         "metric": METRIC, "value": value, "unit": "body-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * secs / args.steps, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("bf16+f32+f64" if mlp else "f32+f64") if prec == nb.PREC_FP32C else "f64",
+        "dtype": "f32+f64" if prec == nb.PREC_FP32C else "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
@@ -376,20 +401,7 @@ def main():
     }
     if theta_search is not None:
         out["config"]["theta_search"] = theta_search
-    if mlp:
-        # the same input with plain Barnes-Hut at the comparison opening angle
-        ctx.set_correction(None)
-        ctx.set_params(theta=MLP_PLAIN_THETA)
-        psecs, pph, perr, _, pinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
-        pvalue = n * args.steps / psecs
-        out["config"]["learned_correction"] = dict(
-            train_info, tau=0.0, network_evals_per_step=pairs,
-            plain_theta=MLP_PLAIN_THETA, plain_value=round(pvalue, 1),
-            plain_median_rel_err=perr, plain_mean_interactions=float(pinter.mean()),
-            speedup_vs_plain=round(raw_value / pvalue, 4))
-        log(f"plain theta={MLP_PLAIN_THETA}: {pvalue:.4g} body-steps/s, err {perr:.3e}; "
-            f"corrected: {raw_value:.4g}, err {err:.3e}")
-    if algorithm == "BarnesHut" and not mlp and not args.no_sweep and args.theta_sweep:
+    if algorithm == "BarnesHut" and not args.no_sweep and args.theta_sweep:
         sweep = []
         for th in (float(v) for v in args.theta_sweep.split(",")):
             ctx.set_params(theta=th)
@@ -408,6 +420,96 @@ def main():
     ctx.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(d, algorithm, theta, dt)
+    print(json.dumps(out), flush=True)
+
+
+def run_learned(args):
+    """Barnes-Hut with the learned far-field correction against plain
+    Barnes-Hut on the 4M Plummer sphere, both at median error <= ERR_GATE.
+    Plain: the largest gated theta (choose_theta).  Corrected: every (theta,
+    tau) of theta_plain + MLP_THETA_STEPS and tau = theta * MLP_TAU_FRACS is
+    evaluated once on the initial state (error against the same FP64 targets,
+    and the build + traversal time of that evaluation); the fastest that
+    meets ERR_GATE * THETA_MARGIN is then timed like the plain run."""
+    kind, n, seed, algorithm, _, dt, prec = WORKLOADS[args.workload]
+    log(f"device: {nb.device_name()}")
+    net, train_info = train_correction(args.mlp_train_theta)
+    log(f"workload {args.workload}: generating {kind} n={n} ...")
+    d = gen(kind, n, seed)
+    ctx = nb.Context(n, theta=0.5, precision=prec)
+    theta_p, search_p = choose_theta(ctx, d, dt)
+    ctx.set_params(theta=theta_p)
+    psecs, pph, perr, _, pinter = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
+    pvalue = n * args.steps / psecs
+    log(f"plain theta {theta_p}: {pvalue:.4g} body-steps/s, err {perr:.3e}")
+
+    # the corrected search on the initial state
+    ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    t = err_targets(n)
+    rx, ry, rz = ctx.forces_direct_targets(t)
+    den = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+    trials, best = [], None
+    for dth in MLP_THETA_STEPS:
+        th = round(theta_p + dth, 4)
+        for frac in MLP_TAU_FRACS:
+            tau = round(th * frac, 4)
+            ctx.set_params(theta=th)
+            ctx.set_correction(net, tau)
+            ctx.bh_build()
+            build_ms = ctx.phase_ms()["Octree creation"]
+            ax, ay, az = ctx.bh_forces()
+            force_ms = ctx.phase_ms()["Acceleration Kernel Time"]
+            evals = ctx.correction_pairs()
+            e = np.sqrt((ax[t] - rx) ** 2 + (ay[t] - ry) ** 2 + (az[t] - rz) ** 2) / den
+            err = float(np.median(e))
+            rec = {"theta": th, "tau": tau, "median_rel_err": err, "build_ms": round(build_ms, 3),
+                   "force_ms": round(force_ms, 3), "network_evals": int(evals)}
+            trials.append(rec)
+            log(f"corrected theta {th} tau {tau}: err {err:.3e}, force {force_ms:.2f} ms, "
+                f"{evals / n:.1f} evaluations per body")
+            if err <= ERR_GATE * THETA_MARGIN and (
+                    best is None or build_ms + force_ms < best["build_ms"] + best["force_ms"]):
+                best = rec
+    out_lc = dict(train_info, plain_theta=theta_p, plain_value=round(pvalue, 1),
+                  plain_median_rel_err=perr, plain_mean_interactions=float(pinter.mean()),
+                  plain_theta_search=search_p, corrected_trials=trials)
+    value, raw, err, theta, phases, sample = None, None, None, None, {}, 0
+    if best is not None:
+        theta = best["theta"]
+        ctx.set_params(theta=theta)
+        ctx.set_correction(net, best["tau"])
+        secs, phases, err, sample, _ = timed_run(ctx, d, algorithm, dt, args.steps, args.warmup)
+        raw = n * args.steps / secs
+        value = gated(raw, err)[0]
+        evals = ctx.correction_pairs()
+        out_lc.update(tau=best["tau"], network_evals_per_step=int(evals),
+                      speedup_vs_plain=round(raw / pvalue, 4))
+        log(f"corrected theta {theta} tau {best['tau']}: {raw:.4g} body-steps/s, err {err:.3e}")
+    else:
+        out_lc.update(speedup_vs_plain=None)
+        log("no corrected (theta, tau) met the bound")
+    ctx.close()
+    accel_ms = phases.get("Acceleration Kernel Time", 0.0) / args.steps
+    flops = MLP_FLOP_PER_PAIR * float(out_lc.get("network_evals_per_step", 0))
+    achieved = flops / (accel_ms * 1e-3) / 1e12 if accel_ms else 0.0
+    out = {
+        "metric": METRIC, "value": value, "unit": "body-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * n / raw, 4) if raw else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16+f32+f64", "data": "synthetic",
+        "config": {"workload": args.workload, "bodies": n, "distribution": kind,
+                   "algorithm": algorithm, "theta": theta, "dt": dt, "median_rel_err": err,
+                   "err_sample": sample, "gate": ERR_GATE,
+                   "gate_met": err is not None and err <= ERR_GATE,
+                   "ungated_value": None if raw is None else round(raw, 1),
+                   "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},
+                   "learned_correction": out_lc},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4),
+                     "traffic": None, "kernel": "k_bh_group<true, false>",
+                     "kernel_ms": round(accel_ms, 4), "flop_per_launch": flops},
+    }
     print(json.dumps(out), flush=True)
 
 
@@ -544,10 +646,13 @@ def timed_run(ctx, d, algorithm, dt, steps, warmup):
     return t1 - t0, phases, err, sample, inter
 
 
-def train_correction():
+def train_correction(theta=None):
     """Train the correction network on the GPU (outside the timed region):
-    pairs from MLP_TRAIN['snapshots'] Plummer snapshots, MLP_TRAIN['pairs'] in all."""
-    c = MLP_TRAIN
+    pairs from MLP_TRAIN['snapshots'] Plummer snapshots, MLP_TRAIN['pairs'] in all,
+    accepted by per-body walks at `theta` (default MLP_TRAIN['theta'])."""
+    c = dict(MLP_TRAIN)
+    if theta is not None:
+        c["theta"] = theta
     t0 = time.perf_counter()
     Xs, Ts = [], []
     per = c["pairs"] // c["snapshots"]
@@ -570,6 +675,7 @@ def train_correction():
     log(f"correction trained: {X.shape[0]} pairs ({t1 - t0:.1f} s), {c['epochs']} epochs in "
         f"{t2 - t1:.1f} s, loss {losses[0]:.3e} -> {losses[-1]:.3e} (target mean square {var:.3e})")
     return net, dict(train_pairs=int(X.shape[0]), train_snapshots=c["snapshots"],
+                     train_theta=c["theta"],
                      train_bodies=c["n"], train_epochs=c["epochs"],
                      train_seconds=round(t2 - t1, 3), train_loss=float(losses[-1]),
                      train_target_mean_square=var)

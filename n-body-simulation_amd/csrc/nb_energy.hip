@@ -5,21 +5,28 @@
 //   E_kin[j] = 0.5 * m_j * (vx^2 + vy^2 + vz^2)
 //   E_pot[j] = sum_{i<j} G * m_i * m_j / |p_j - p_i|
 // The host adds the terms in body order (nBodyAlgorithm.cpp:71-74).
-// O(N^2/2) pair work staged through LDS tiles of 64 bodies.
+// O(N^2/2) pair work staged through LDS tiles of 256 bodies; every body's sum
+// runs over i = 0, 1, ..., j - 1 in order, so the terms are bit-identical to
+// the reference's.
 #include "nb_internal.h"
 
 namespace nb {
 namespace {
 
-constexpr int kTile = 64;
+constexpr int kThreads = 256;  // bodies j per block, and sources i per LDS stage
 
-__global__ void __launch_bounds__(kTile) k_energy_terms(
+// Block b holds bodies j of tile nb - 1 - b: the triangle's longest sums
+// (the last bodies, i < j over almost all of the system) start first, and the
+// short ones fill in behind them, so the launch does not end on a few long
+// blocks.  Sources are staged kThreads at a time for all four waves.
+__global__ void __launch_bounds__(kThreads) k_energy_terms(
     uint32_t n, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ vx, const double *__restrict__ vy,
     const double *__restrict__ vz, const double *__restrict__ m, double G, double *ekin,
     double *epot) {
-    __shared__ double sx[kTile], sy[kTile], sz[kTile], sm[kTile];
-    const uint32_t j = blockIdx.x * kTile + threadIdx.x;
+    __shared__ double4 src[kThreads];
+    const uint32_t tile = gridDim.x - 1 - blockIdx.x;
+    const uint32_t j = tile * kThreads + threadIdx.x;
     const bool valid = j < n;
     const uint32_t jj = valid ? j : 0;
     const double pxj = x[jj], pyj = y[jj], pzj = z[jj], mj = m[jj];
@@ -28,23 +35,20 @@ __global__ void __launch_bounds__(kTile) k_energy_terms(
         ekin[j] = 0.5 * mj * v;
     }
     double ep = 0;
-    const uint32_t last = min(n, (blockIdx.x + 1) * kTile);  // bodies i < j of this block
-    for (uint32_t k = 0; k < last; k += kTile) {
+    const uint32_t last = min(n, (tile + 1) * kThreads);  // bodies i < j of this block
+    for (uint32_t k = 0; k < last; k += kThreads) {
         const uint32_t i = k + threadIdx.x;
-        if (i < n) {
-            sx[threadIdx.x] = x[i];
-            sy[threadIdx.x] = y[i];
-            sz[threadIdx.x] = z[i];
-            sm[threadIdx.x] = m[i];
-        }
+        if (i < n) src[threadIdx.x] = make_double4(x[i], y[i], z[i], m[i]);
         __syncthreads();
-        if (valid) {
-            for (uint32_t ii = 0; ii < (uint32_t)kTile && k + ii < j; ++ii) {
-                const double r_x = pxj - sx[ii];
-                const double r_y = pyj - sy[ii];
-                const double r_z = pzj - sz[ii];
+        if (valid && j > k) {
+            const uint32_t cnt = min((uint32_t)kThreads, j - k);  // sources i = k + ii < j
+            for (uint32_t ii = 0; ii < cnt; ++ii) {
+                const double4 si = src[ii];
+                const double r_x = pxj - si.x;
+                const double r_y = pyj - si.y;
+                const double r_z = pzj - si.z;
                 const double r = sqrt(r_x * r_x + r_y * r_y + r_z * r_z);
-                ep += G * sm[ii] * mj / r;
+                ep += G * si.w * mj / r;
             }
         }
         __syncthreads();
@@ -57,8 +61,9 @@ __global__ void __launch_bounds__(kTile) k_energy_terms(
 void launch_energy_terms(uint32_t n, const double *x, const double *y, const double *z,
                          const double *vx, const double *vy, const double *vz, const double *m,
                          double G, double *ekin, double *epot, hipStream_t s) {
-    hipLaunchKernelGGL(k_energy_terms, dim3(ceil_div(n, kTile)), dim3(kTile), 0, s, n, x, y, z, vx,
-                       vy, vz, m, G, ekin, epot);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_energy_terms, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, s, n, x, y,
+                       z, vx, vy, vz, m, G, ekin, epot);
 }
 
 }  // namespace nb

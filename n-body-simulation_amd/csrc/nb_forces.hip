@@ -222,7 +222,28 @@ __global__ void __launch_bounds__(kSplitThreads) k_split_reduce(uint32_t n, uint
     const uint32_t i = blockIdx.x * kSplitThreads + threadIdx.x;
     if (i >= n) return;
     double ox = 0, oy = 0, oz = 0;
-    for (uint32_t sp = 0; sp < splits; ++sp) {
+    // eight splits' loads in flight before their (ordered) additions: one
+    // wave per CU at 16,384 bodies, so the loads' latency is not hidden by
+    // other waves (18.8 us with one split at a time)
+    constexpr uint32_t kAhead = 8;
+    uint32_t sp = 0;
+    for (; sp + kAhead <= splits; sp += kAhead) {
+        double vx[kAhead], vy[kAhead], vz[kAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            const uint64_t b = (uint64_t)(sp + u) * 3ull * n;
+            vx[u] = partial[b + i];
+            vy[u] = partial[b + n + i];
+            vz[u] = partial[b + 2ull * n + i];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kAhead; ++u) {
+            ox += vx[u];
+            oy += vy[u];
+            oz += vz[u];
+        }
+    }
+    for (; sp < splits; ++sp) {
         const uint64_t b = (uint64_t)sp * 3ull * n;
         ox += partial[b + i];
         oy += partial[b + n + i];
