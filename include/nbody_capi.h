@@ -317,6 +317,10 @@ typedef struct nb_ensemble nb_ensemble;
 typedef struct nb_ensemble_tuning {
     uint32_t bh_threshold; /* systems with n >= bh_threshold use Barnes-Hut */
     uint32_t waves;        /* persistent wavefronts of the force launch */
+    uint32_t group_max;    /* bodies per Barnes-Hut traversal group, 1..64 (default 32) */
+    uint32_t pack_max;     /* sibling groups are packed inside cells of at most this many
+                              bodies, >= group_max (default 128) */
+    uint32_t direct_tile;  /* targets per direct-sum work item: 64 or 128 (default 64) */
 } nb_ensemble_tuning;
 
 /* p->precision must be NB_PREC_FP32C; G, eps2, theta and storage_size_param
@@ -354,9 +358,11 @@ uint64_t nb_ensemble_tree_canonical(nb_ensemble *e, uint32_t s, uint64_t cap, ui
                                     uint8_t *kind, uint32_t *body, double *mass, double *com_x,
                                     double *com_y, double *com_z, uint32_t *count);
 /* Autotuner: times `steps` loop iterations from the current state for each
- * candidate direct-sum/Barnes-Hut threshold, then each persistent grid size
- * at the best threshold; keeps the fastest and restores the state.  *best and
- * *best_step_ms (either may be NULL) receive the choice and its time. */
+ * candidate, one parameter at a time from the current best: the
+ * direct-sum/Barnes-Hut threshold, the traversal group size, the packing
+ * limit, the direct-sum tile and the persistent grid size; keeps the fastest
+ * and restores the state.  *best and *best_step_ms (either may be NULL)
+ * receive the choice and its time. */
 int nb_ensemble_autotune(nb_ensemble *e, uint32_t steps, nb_ensemble_tuning *best,
                          double *best_step_ms);
 

@@ -153,7 +153,7 @@ int apply_tuning(nb_ensemble *e) {
         EnsSystem &es = e->sys_host[s];
         es.bh = es.n >= e->tune.bh_threshold ? 1u : 0u;
         if (es.bh) bh_order.push_back(s);
-        else items[s] = nb::ceil_div(es.n, nb::kEnsTile);
+        else items[s] = nb::ceil_div(es.n, e->tune.direct_tile);
         // rough cost model: pair evaluations
         cost[s] = es.bh ? 1500.0 * es.n : (double)es.n * (double)es.n;
     }
@@ -176,14 +176,16 @@ int apply_tuning(nb_ensemble *e) {
 void enqueue_forces(nb_ensemble *e) {
     mark(e, NB_PHASE_OCTREE, true);
     nb::launch_forest_build(e->sys, e->bh_order, e->nbh, e->nsys, e->x, e->y, e->z, e->m, e->p.G,
-                            e->fb, e->items, e->status, e->stream);
+                            e->fb, nb::GroupRule{e->tune.group_max, e->tune.pack_max}, e->items,
+                            e->status, e->stream);
     mark(e, NB_PHASE_OCTREE, false);
     mark(e, NB_PHASE_ACCEL, true);
     nb::launch_ens_items(e->nsys, e->qorder, e->items, e->item_off, e->cursor, e->status,
                          e->stream);
-    nb::EnsForceArgs a{e->sys, e->qorder, e->item_off, e->nsys, e->x, e->y, e->z, e->m,
-                       e->ax, e->ay, e->az, e->fb, e->p.G, e->p.theta, (float)e->p.eps2,
-                       e->cursor, e->status, e->inter};
+    nb::EnsForceArgs a{e->sys,    e->qorder, e->item_off, e->nsys,     e->x,
+                       e->y,      e->z,      e->m,        e->ax,       e->ay,
+                       e->az,     e->fb,     e->p.G,      e->p.theta,  (float)e->p.eps2,
+                       e->cursor, e->status, e->inter,    e->tune.direct_tile / nb::kEnsTile};
     nb::launch_ens_forces(a, e->tune.waves, e->stream);
     mark(e, NB_PHASE_ACCEL, false);
 }
@@ -313,6 +315,9 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
     if (hipMemsetAsync(f.hdr, 0, nsys * sizeof(TreeHdr), e->stream) != hipSuccess) return bail(NB_E_HIP);
     e->tune.bh_threshold = 4096;
     e->tune.waves = 8192;
+    e->tune.group_max = nb::kGroupMax;
+    e->tune.pack_max = nb::kPackMax;
+    e->tune.direct_tile = nb::kEnsTile;
     if ((rc = apply_tuning(e))) return bail(rc);
     *out = e;
     return NB_OK;
@@ -455,8 +460,14 @@ int nb_ensemble_phase_ms(const nb_ensemble *e, double *ms) {
     return NB_OK;
 }
 
+bool tuning_ok(const nb_ensemble_tuning *t) {
+    return t && t->waves >= 1 && t->waves <= (1u << 20) && t->group_max >= 1 &&
+           t->group_max <= nb::kGroupLimit && t->pack_max >= t->group_max &&
+           (t->direct_tile == nb::kEnsTile || t->direct_tile == 2 * nb::kEnsTile);
+}
+
 int nb_ensemble_set_tuning(nb_ensemble *e, const nb_ensemble_tuning *t) {
-    if (!e || !t || t->waves == 0 || t->waves > (1u << 20))
+    if (!e || !tuning_ok(t))
         return efail(e, NB_E_INVALID_ARG, "nb_ensemble_set_tuning: bad tuning");
     e->tune = *t;
     return apply_tuning(e);
@@ -527,32 +538,54 @@ int nb_ensemble_autotune(nb_ensemble *e, uint32_t steps, nb_ensemble_tuning *bes
         thr.push_back(t);
         if (t * 3 / 2 <= std::min<uint32_t>(nmax, 16384)) thr.push_back(t * 3 / 2);
     }
-    nb_ensemble_tuning cur{4096, 8192}, top = cur;
+    nb_ensemble_tuning top{4096, 8192, nb::kGroupMax, nb::kPackMax, nb::kEnsTile};
     double top_ms = 1e300;
-    for (uint32_t t : thr) {
-        cur.bh_threshold = t;
-        double ms;
-        int rc = trial(cur, &ms);
-        if (rc) return rc;
-        if (ms < top_ms) {
-            top_ms = ms;
-            top = cur;
+    // one parameter at a time, each candidate from the best so far
+    auto sweep = [&](auto set, std::initializer_list<uint32_t> values) -> int {
+        const nb_ensemble_tuning base = top;
+        for (uint32_t v : values) {
+            nb_ensemble_tuning cur = base;
+            set(cur, v);
+            if (!tuning_ok(&cur)) continue;
+            double ms;
+            int rc = trial(cur, &ms);
+            if (rc) return rc;
+            if (ms < top_ms) {
+                top_ms = ms;
+                top = cur;
+            }
+        }
+        return NB_OK;
+    };
+    int rc = NB_OK;
+    {
+        const nb_ensemble_tuning base = top;
+        for (uint32_t t : thr) {
+            nb_ensemble_tuning cur = base;
+            cur.bh_threshold = t;
+            double ms;
+            if ((rc = trial(cur, &ms))) return rc;
+            if (ms < top_ms) {
+                top_ms = ms;
+                top = cur;
+            }
         }
     }
-    for (uint32_t w : {1024u, 2048u, 4096u, 16384u, 32768u}) {
-        cur = top;
-        cur.waves = w;
-        double ms;
-        int rc = trial(cur, &ms);
-        if (rc) return rc;
-        if (ms < top_ms) {
-            top_ms = ms;
-            top = cur;
-        }
-    }
+    // traversal group size, with the packing limit at 4 groups
+    if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.group_max = v; c.pack_max = 4 * v; },
+                    {16u, 24u, 48u, 64u})))
+        return rc;
+    if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.pack_max = v * c.group_max; },
+                    {1u, 2u, 8u})))
+        return rc;
+    if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.direct_tile = v; },
+                    {2u * nb::kEnsTile})))
+        return rc;
+    if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.waves = v; },
+                    {1024u, 2048u, 4096u, 16384u, 32768u})))
+        return rc;
     e->tune = top;
-    int rc = apply_tuning(e);
-    if (rc) return rc;
+    if ((rc = apply_tuning(e))) return rc;
     if ((rc = restore())) return rc;
     ENS_HIP(e, hipStreamSynchronize(e->stream));
     timing_reset(e);

@@ -443,6 +443,7 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 // atomic work queue.
 constexpr int kGroup = 64;          // lanes per wavefront (a group has up to 2 x kGroup bodies)
 static_assert(kGroupMax <= 2 * kGroup, "a lane holds at most two bodies of its group");
+static_assert(kGroupLimit <= kGroup, "the tunable groups fit one body per lane");
 constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 6 | depth)
 // Child records per lane and walk step (8 kPop cells).  More loads in flight
 // per wave cost registers: on a 1M-body disk kPop 2 and 4 made the traversal
@@ -1109,7 +1110,7 @@ __device__ __forceinline__ void group_walk(const GroupTree &T, uint2 grp, float 
 
 // a group {first, count} that a walk may trust
 __device__ inline bool group_ok(uint2 grp, uint32_t n) {
-    return grp.y != 0 && grp.y <= kGroupMax && grp.x < n && grp.y <= n - grp.x;
+    return grp.y != 0 && grp.y <= kGroupLimit && grp.x < n && grp.y <= n - grp.x;
 }
 
 template <bool kCorr, bool kQuad>
@@ -1127,7 +1128,7 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     __shared__ uint32_t cl_k[kCorr ? kCellCap : 1];
     // bcorr is cleared lane by lane (lane + 64 b): at least one wave of entries
     constexpr uint32_t kCorrSlots = kGroupMax > (uint32_t)kGroup ? kGroupMax : (uint32_t)kGroup;
-    __shared__ float4 boff[kCorr ? kGroupMax : 1];
+    __shared__ float4 boff[kCorr ? kGroupLimit : 1];
     __shared__ float4 bcorr[kCorr ? kCorrSlots : 1];
     __shared__ float thr[kGroup];
     const GroupLds L{stk, lst, qlst, qk, thr, cl_e, cl_k, boff, bcorr};
@@ -1202,22 +1203,30 @@ __global__ void __launch_bounds__(64) k_ens_items(uint32_t nq, const uint32_t *_
     (void)status;
 }
 
-// Direct sum for targets [tile*64, tile*64+64) of one system: FP32 pairs on
-// offsets from the tile's first body (FP64 subtraction, so nearby pairs keep
-// their precision; a body's own offset equals its source offset, so the self
-// pair adds exactly 0), FP64 sum of each 64-source chunk.
+// Direct sum for targets [tile*64*TPL, (tile+1)*64*TPL) of one system, TPL
+// per lane: FP32 pairs on offsets from the tile's first body (FP64
+// subtraction, so nearby pairs keep their precision; a body's own offset
+// equals its source offset, so the self pair adds exactly 0), FP64 sum of each
+// 64-source chunk.  Two targets per lane halve the LDS reads per pair.
+template <int TPL>
 __device__ inline void direct_tile(const double *__restrict__ x, const double *__restrict__ y,
                                    const double *__restrict__ z, const double *__restrict__ m,
                                    uint32_t n, uint32_t tile, double G, float eps2, double *ax,
-                                   double *ay, double *az, float4 *stage) {
+                                   double *ay, double *az, uint32_t *inter, float4 *stage) {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t t0 = tile * kEnsTile;
-    const uint32_t i = t0 + lane;
-    const bool valid = i < n;
-    const uint32_t ii = valid ? i : t0;
+    const uint32_t t0 = tile * kEnsTile * TPL;
     const double rx = x[t0], ry = y[t0], rz = z[t0];
-    const float px = (float)(x[ii] - rx), py = (float)(y[ii] - ry), pz = (float)(z[ii] - rz);
-    double acc_x = 0, acc_y = 0, acc_z = 0;
+    float px[TPL], py[TPL], pz[TPL];
+    double acc_x[TPL], acc_y[TPL], acc_z[TPL];
+#pragma unroll
+    for (int q = 0; q < TPL; ++q) {
+        const uint32_t i = t0 + lane + (uint32_t)q * kEnsTile;
+        const uint32_t ii = i < n ? i : t0;
+        px[q] = (float)(x[ii] - rx);
+        py[q] = (float)(y[ii] - ry);
+        pz[q] = (float)(z[ii] - rz);
+        acc_x[q] = acc_y[q] = acc_z[q] = 0.0;
+    }
     for (uint32_t c = 0; c < n; c += kEnsTile) {
         const uint32_t j = c + lane;
         float4 sv = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1228,16 +1237,30 @@ __device__ inline void direct_tile(const double *__restrict__ x, const double *_
         stage[lane] = sv;
         __syncthreads();
         const uint32_t cnt = min(kEnsTile, n - c);
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (uint32_t k = 0; k < cnt; ++k) pair_f32(stage[k], px, py, pz, eps2, fx, fy, fz);
-        acc_x += (double)fx;
-        acc_y += (double)fy;
-        acc_z += (double)fz;
+        float fx[TPL], fy[TPL], fz[TPL];
+#pragma unroll
+        for (int q = 0; q < TPL; ++q) fx[q] = fy[q] = fz[q] = 0.f;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 e = stage[k];
+#pragma unroll
+            for (int q = 0; q < TPL; ++q) pair_f32(e, px[q], py[q], pz[q], eps2, fx[q], fy[q], fz[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < TPL; ++q) {
+            acc_x[q] += (double)fx[q];
+            acc_y[q] += (double)fy[q];
+            acc_z[q] += (double)fz[q];
+        }
     }
-    if (valid) {
-        ax[i] = acc_x;
-        ay[i] = acc_y;
-        az[i] = acc_z;
+#pragma unroll
+    for (int q = 0; q < TPL; ++q) {
+        const uint32_t i = t0 + lane + (uint32_t)q * kEnsTile;
+        if (i < n) {
+            ax[i] = acc_x[q];
+            ay[i] = acc_y[q];
+            az[i] = acc_z[q];
+            inter[i] = n;
+        }
     }
 }
 
@@ -1284,11 +1307,12 @@ __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
             }
             group_walk<false>(T, grp, a.eps2, corr, w, L, peak, overflow, ncorr, nquad);
             if (overflow) break;
+        } else if (a.direct_tpl == 2) {
+            direct_tile<2>(a.x + bo, a.y + bo, a.z + bo, a.m + bo, S.n, local, a.G, a.eps2,
+                           a.ax + bo, a.ay + bo, a.az + bo, a.inter + bo, lst);
         } else {
-            direct_tile(a.x + bo, a.y + bo, a.z + bo, a.m + bo, S.n, local, a.G, a.eps2, a.ax + bo,
-                        a.ay + bo, a.az + bo, lst);
-            const uint32_t i = local * kEnsTile + lane;
-            if (i < S.n) a.inter[bo + i] = S.n;
+            direct_tile<1>(a.x + bo, a.y + bo, a.z + bo, a.m + bo, S.n, local, a.G, a.eps2,
+                           a.ax + bo, a.ay + bo, a.az + bo, a.inter + bo, lst);
         }
     }
     if (overflow && lane == 0) atomicOr(a.status, kFlagStack);

@@ -74,6 +74,13 @@ struct alignas(128) TreeHdr {
 //    the walk must resolve that cell to its leaves.
 constexpr uint32_t kGroupMax = 32;
 constexpr uint32_t kPackMax = 4 * kGroupMax;
+// The largest group a walk takes (one body per lane of a 64-lane wave); the
+// ensemble's autotuner chooses group_max in [1, kGroupLimit].
+constexpr uint32_t kGroupLimit = 64;
+struct GroupRule {
+    uint32_t group_max;  // bodies per traversal group
+    uint32_t pack_max;   // sibling groups are packed in cells of at most this many bodies
+};
 
 struct InternalSums {
     double x, y, z, m;  // centre-of-mass numerators and mass
@@ -244,8 +251,8 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s);
 // system's traversal-group count.
 void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
                          uint32_t nsys, const double *x, const double *y, const double *z,
-                         const double *m, double G, const ForestBufs &fb, uint32_t *items,
-                         uint32_t *status, hipStream_t s);
+                         const double *m, double G, const ForestBufs &fb, GroupRule rule,
+                         uint32_t *items, uint32_t *status, hipStream_t s);
 
 // nb_forces.hip
 void launch_direct_f64(uint32_t n, const double *x, const double *y, const double *z,
@@ -311,8 +318,9 @@ struct EnsForceArgs {
     uint32_t *cursor;  // work-queue cursor (reset by launch_ens_items)
     uint32_t *status;  // kFlag* bits
     uint32_t *inter;   // per body: accepted interactions (Barnes-Hut) or sources (direct)
+    uint32_t direct_tpl;  // direct-sum targets per lane: 1 or 2 (items of 64 or 128 targets)
 };
-constexpr uint32_t kEnsTile = 64;  // targets per direct-sum item
+constexpr uint32_t kEnsTile = 64;  // targets per direct-sum item and lane
 // item_off = exclusive prefix of items[qorder[q]]; resets the cursor and status
 void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items,
                       uint32_t *item_off, uint32_t *cursor, uint32_t *status, hipStream_t s);

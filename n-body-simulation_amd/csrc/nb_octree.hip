@@ -543,10 +543,10 @@ __device__ inline void sum_children(uint32_t pk, const double4 *__restrict__ bod
 
 // Traversal records and groups, written by the centre-of-mass kernels: a cell,
 // once summed, writes the records of its 8 child slots (their data is at
-// hand) and, if it holds more than kGroupMax bodies, marks the traversal
-// groups among its children (a child cell with at most kGroupMax bodies, or a
-// body).  The kernel that sums the root writes the root's record, and its
-// group if the root holds at most kGroupMax bodies.
+// hand) and, if it holds more than rule.group_max bodies, marks the traversal
+// groups among its children (a child cell with at most rule.group_max
+// bodies, or a body).  The kernel that sums the root writes the root's
+// record, and its group if the root holds at most rule.group_max bodies.
 struct RecOut {
     double G;
     int f64, f32;  // FP64 records (cm, edge); FP32 records and traversal groups
@@ -554,6 +554,7 @@ struct RecOut {
     double *edge;
     NodeF32 *rec;
     uint32_t *group_size, *group_slot;
+    GroupRule rule;  // group size and packing limit
 };
 
 // Record centre of a child from its term: a cell's sums, or a body's
@@ -592,26 +593,27 @@ __device__ inline void put_group(const RecOut &o, uint32_t p, uint32_t size) {
 }
 
 // Traversal groups among the children of a cell that holds more than
-// kGroupMax bodies; first: the cell's first sorted position, m[d]: child d's
-// body count (0 if empty).  Each child with at most kGroupMax bodies is a
-// group; in a cell of at most kPackMax bodies, runs of such children are
-// packed in digit order into groups of at most kGroupMax bodies instead
+// group_max bodies; first: the cell's first sorted position, m[d]: child d's
+// body count (0 if empty).  Each child with at most group_max bodies is a
+// group; in a cell of at most pack_max bodies, runs of such children are
+// packed in digit order into groups of at most group_max bodies instead
 // (siblings are consecutive in the sorted order; a child with more bodies
 // ends a run).
 __device__ inline void put_child_groups(const RecOut &o, uint32_t first, const uint32_t (&m)[8]) {
+    const uint32_t gmax = o.rule.group_max;
     uint32_t total = 0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) total += m[d];
-    const bool pack = total <= kPackMax;
+    const bool pack = total <= o.rule.pack_max;
     uint32_t pos = first, g0 = first, gs = 0;
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
         if (m[d] == 0) continue;
-        if (gs && (!pack || m[d] > kGroupMax || gs + m[d] > kGroupMax)) {
+        if (gs && (!pack || m[d] > gmax || gs + m[d] > gmax)) {
             put_group(o, g0, gs);
             gs = 0;
         }
-        if (m[d] <= kGroupMax) {
+        if (m[d] <= gmax) {
             if (gs == 0) g0 = pos;
             gs += m[d];
         }
@@ -636,10 +638,10 @@ __device__ inline void cell_outputs(const RecOut &o, uint32_t pk, uint32_t depth
         else
             put_record(o, base + d, r[d], term_com(v[d]), r[d] >= 0 ? ce : 0.0);
     }
-    if (cnt > kGroupMax) put_child_groups(o, first, c);
+    if (cnt > o.rule.group_max) put_child_groups(o, first, c);
     if (depth == 0) {
         put_record(o, 0, (int32_t)pk, term_com(s), root_edge);
-        if (cnt <= kGroupMax) put_group(o, first, cnt);
+        if (cnt <= o.rule.group_max) put_group(o, first, cnt);
     }
 }
 
@@ -889,7 +891,7 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
         const double4 g = c_s[c];
         sums[kb + c] = InternalSums{g.x, g.y, g.z, g.w};
         counts[kb + c] = n_s[c];
-        if (n_s[c] > kGroupMax) {
+        if (n_s[c] > o.rule.group_max) {
             uint32_t m[8];
 #pragma unroll
             for (int dd = 0; dd < 8; ++dd) {
@@ -931,7 +933,7 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
         }
         if (dc == 0 && (j & 7u) == 0) {  // the root (the block holds every body)
             put_record(o, 0, (int32_t)(kb + c), term_com(c_s[c]), root_edge);
-            if (pc <= kGroupMax) put_group(o, s + first_s[c], pc);
+            if (pc <= o.rule.group_max) put_group(o, s + first_s[c], pc);
         }
     }
 }
@@ -1118,7 +1120,7 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
         const double4 g = sv[id];
         sums[list[id]] = InternalSums{g.x, g.y, g.z, g.w};
         counts[list[id]] = sn[id];
-        if (sn[id] > kGroupMax) {
+        if (sn[id] > o.rule.group_max) {
             uint32_t m[8];
 #pragma unroll
             for (int dd = 0; dd < 8; ++dd) {
@@ -1138,13 +1140,13 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
                        term_com(sv[code & ~kChildHere]), ldexp(root_edge, -(int)(ldep[id] + 1u)));
         if (dd == 0 && ldep[id] == 0) {  // the root
             put_record(o, 0, (int32_t)list[id], term_com(sv[id]), root_edge);
-            if (pc <= kGroupMax) put_group(o, fst[id], pc);
+            if (pc <= o.rule.group_max) put_group(o, fst[id], pc);
         }
     }
 }
 
 // ---- traversal groups -------------------------------------------------------------
-// A group is a cell with at most kGroupMax bodies whose parent holds more, or
+// A group is a cell with at most group_max bodies whose parent holds more, or
 // a lone body whose parent holds more.  Groups partition the bodies into
 // spatially compact runs of the octree order.  Every sorted position starts
 // at most one group: mark the starts, scan, scatter -- the group list comes
@@ -1155,7 +1157,7 @@ __device__ inline void group_mark_children(uint32_t k, const uint32_t *__restric
                                            const int32_t *__restrict__ ref,
                                            const uint32_t *__restrict__ int_first,
                                            const RecOut &o) {
-    if (counts[k] <= kGroupMax) return;
+    if (counts[k] <= o.rule.group_max) return;
     uint32_t m[8];
 #pragma unroll
     for (int d = 0; d < 8; ++d) {
@@ -1385,7 +1387,7 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
     const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
     const double *__restrict__ gm, double G, ForestBufs fb,
     const uint32_t *sorted,  // may be fb.sidx itself: each slot is read, then rewritten
-    uint32_t *items, uint32_t *status) {
+    uint32_t *items, uint32_t *status, GroupRule rule) {
     __shared__ uint32_t wsum[kForestWaves + 1];
     __shared__ uint64_t kwin[kLinkBack + kForestThreads];  // topology: keys of a chunk
     __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
@@ -1485,10 +1487,10 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
         group_slot[i] = 0;
     }
     __syncthreads();
-    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot};
+    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot, rule};
     for (uint32_t k = t; k < I; k += kForestThreads)
         group_mark_children(k, counts, ref, int_first, og);
-    if (t == 0 && (n == 1 || counts[0] <= kGroupMax)) put_group(og, 0, n);  // the root
+    if (t == 0 && (n == 1 || counts[0] <= rule.group_max)) put_group(og, 0, n);  // the root
     __syncthreads();
     const uint32_t ng = wg_scan_array(group_slot, n, wsum);
     for (uint32_t i = t; i < n; i += kForestThreads) {
@@ -1509,8 +1511,8 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
 
 void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
                          uint32_t nsys, const double *x, const double *y, const double *z,
-                         const double *m, double G, const ForestBufs &fb, uint32_t *items,
-                         uint32_t *status, hipStream_t s) {
+                         const double *m, double G, const ForestBufs &fb, GroupRule rule,
+                         uint32_t *items, uint32_t *status, hipStream_t s) {
     if (nbh == 0) return;
     hipLaunchKernelGGL(k_forest_aabb, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
                        fb);
@@ -1542,7 +1544,7 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
         std::swap(v, va);
     }
     hipLaunchKernelGGL(k_forest_build, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
-                       m, G, fb, (const uint32_t *)v, items, status);
+                       m, G, fb, (const uint32_t *)v, items, status, rule);
 }
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
@@ -1602,7 +1604,9 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s) {
 
 void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) {
     const uint32_t nblocks = ceil_div(t.n, kComBlock);
-    const RecOut o{G, (int)f64, (int)f32, t.cm, t.edge, t.rec, t.group_size, t.group_slot};
+    const RecOut o{G,          (int)f64,      (int)f32,      t.cm,
+                   t.edge,     t.rec,         t.group_size,  t.group_slot,
+                   GroupRule{kGroupMax, kPackMax}};
     hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComLocalThreads), 0, s, t.n, t.icap, t.hdr,
                        (const double4 *)t.body4, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
                        (const uint32_t *)t.chain, (const uint8_t *)t.int_depth,

@@ -467,7 +467,12 @@ class Mlp:
 
 
 class EnsembleTuning(ctypes.Structure):
-    _fields_ = [("bh_threshold", ctypes.c_uint32), ("waves", ctypes.c_uint32)]
+    _fields_ = [("bh_threshold", ctypes.c_uint32), ("waves", ctypes.c_uint32),
+                ("group_max", ctypes.c_uint32), ("pack_max", ctypes.c_uint32),
+                ("direct_tile", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 ENSEMBLE_SIG = {
@@ -591,10 +596,10 @@ class Ensemble:
     def tuning(self):
         t = EnsembleTuning()
         self._check(self.lib.nb_ensemble_get_tuning(self.h, ctypes.byref(t)), "nb_ensemble_get_tuning")
-        return dict(bh_threshold=t.bh_threshold, waves=t.waves)
+        return t.as_dict()
 
-    def set_tuning(self, bh_threshold, waves):
-        t = EnsembleTuning(bh_threshold, waves)
+    def set_tuning(self, bh_threshold, waves, group_max=32, pack_max=128, direct_tile=64):
+        t = EnsembleTuning(bh_threshold, waves, group_max, pack_max, direct_tile)
         self._check(self.lib.nb_ensemble_set_tuning(self.h, ctypes.byref(t)), "nb_ensemble_set_tuning")
 
     def uses_bh(self):
@@ -607,7 +612,7 @@ class Ensemble:
         ms = ctypes.c_double()
         self._check(self.lib.nb_ensemble_autotune(self.h, steps, ctypes.byref(t), ctypes.byref(ms)),
                     "nb_ensemble_autotune")
-        return dict(bh_threshold=t.bh_threshold, waves=t.waves), ms.value
+        return t.as_dict(), ms.value
 
 
 def gen_ensemble(nsys, seed, lo_log2=10, hi_log2=16):

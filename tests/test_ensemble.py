@@ -208,6 +208,61 @@ def test_generated_ensemble_every_body_accurate(gpu, oracle, threshold):
 
 
 @pytest.mark.gpu
+def test_ensemble_tuning_knobs(gpu, oracle):
+    """The autotuner's tile and group parameters: direct-sum tiles of 64 and
+    128 targets (one or two per lane; pair offsets are taken from the tile's
+    first body, so the FP32 roundings differ) both match the FP64 direct sum
+    within FP32 pair rounding; every traversal group size and packing limit
+    gives Barnes-Hut forces as accurate as the reference's per-body traversal;
+    invalid values are rejected."""
+    sizes = [3000, 500, 9000, 130]
+    parts, st, dt = _ensemble(gpu, sizes, 310)
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    with gpu.Ensemble(sizes, theta=0.5) as e:
+        e.set_state(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], st["m"], dt)
+        got = {}
+        for tile in (64, 128):
+            e.set_tuning(bh_threshold=1 << 30, waves=2048, direct_tile=tile)  # all direct
+            e.forces()
+            got[tile] = e.accel()
+        for s, d in enumerate(parts):
+            sl = slice(off[s], off[s + 1])
+            ctx = gpu.Context(sizes[s])  # FP64 reference arithmetic
+            ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+            rx, ry, rz = ctx.forces_direct()
+            ctx.close()
+            ref = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+            for tile, (ax, ay, az) in got.items():
+                err = np.sqrt((ax[sl] - rx) ** 2 + (ay[sl] - ry) ** 2 + (az[sl] - rz) ** 2) / ref
+                assert np.median(err) < 1e-5 and err.max() < 1e-3, (tile, s, err.max())
+        for gmax, pmax in ((16, 64), (24, 24), (48, 192), (64, 512)):
+            e.set_tuning(bh_threshold=0, waves=4096, group_max=gmax, pack_max=pmax)
+            assert e.tuning["group_max"] == gmax and e.tuning["pack_max"] == pmax
+            e.forces()
+            ax, ay, az = e.accel()
+            for s, d in enumerate(parts):
+                sl = slice(off[s], off[s + 1])
+                n = sizes[s]
+                t = np.arange(0, n, max(1, n // 64), dtype=np.uint32)
+                rx, ry, rz = oracle.naive_accel_targets(d["x"], d["y"], d["z"], d["m"], t)
+                tree = oracle.Tree(n)
+                tree.build(d["x"], d["y"], d["z"], d["m"])
+                bx, by, bz, _ = tree.bh_accel(0.5)
+                tree.close()
+                rn = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
+                e_gpu = np.sqrt((ax[sl][t] - rx) ** 2 + (ay[sl][t] - ry) ** 2 +
+                                (az[sl][t] - rz) ** 2) / rn
+                e_ref = np.sqrt((bx[t] - rx) ** 2 + (by[t] - ry) ** 2 + (bz[t] - rz) ** 2) / rn
+                assert np.median(e_gpu) <= np.median(e_ref) + 2e-6, (gmax, s, np.median(e_gpu))
+        for bad in (dict(group_max=0), dict(group_max=65, pack_max=256),
+                    dict(group_max=32, pack_max=16), dict(direct_tile=96)):
+            kw = dict(bh_threshold=0, waves=4096)
+            kw.update(bad)
+            with pytest.raises(gpu.NBodyError):
+                e.set_tuning(**kw)
+
+
+@pytest.mark.gpu
 def test_ensemble_autotune_restores_state(gpu):
     sizes, st, dt = gpu.gen_ensemble(24, 7, lo_log2=10, hi_log2=13)
     with gpu.Ensemble(sizes, theta=0.5) as e:
@@ -218,6 +273,8 @@ def test_ensemble_autotune_restores_state(gpu):
         assert ms > 0
         assert best == e.tuning
         assert best["waves"] in (1024, 2048, 4096, 8192, 16384, 32768)
+        assert 1 <= best["group_max"] <= 64 and best["pack_max"] >= best["group_max"]
+        assert best["direct_tile"] in (64, 128)
         after = e.state()
         for a, b in zip(before, after):
             assert np.array_equal(a, b)

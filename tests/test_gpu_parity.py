@@ -536,6 +536,36 @@ def test_million_body_tree_properties(gpu):
     assert np.median(e) < 1e-4
 
 
+def test_plummer_4m_traversal_meets_the_error_bound(gpu):
+    # The headline configuration at full size: 4,194,304-body Plummer sphere,
+    # FP32 pairs with FP64 accumulation, at the opening angle bench.py picks
+    # (0.36) and at theta 0.3.  Every body appears once in the octree order,
+    # the node count is 1 + 8 * internal cells, and the median relative error
+    # over 16,384 sampled targets against the FP64 direct sum (reference
+    # arithmetic) is within 1e-4; a smaller theta is more accurate and costs
+    # more interactions.
+    n = 1 << 22
+    d = gpu.gen_plummer(n, 0x4E424F44590002)
+    t = np.sort(np.random.default_rng(1).choice(n, 16384, replace=False)).astype(np.uint32)
+    res = {}
+    with ctx_for(gpu, d, theta=0.36, precision=gpu.PREC_FP32C) as c:
+        rx, ry, rz = c.forces_direct_targets(t)
+        for theta in (0.36, 0.3):
+            c.set_params(theta=theta)
+            c.bh_build()
+            srt = c.tree_sorted()
+            assert np.array_equal(np.sort(srt), np.arange(n, dtype=np.uint32))
+            info = c.tree_info()
+            assert info["node_count"] == 1 + 8 * info["internal_count"]
+            ax, ay, az = c.bh_forces()
+            e = np.median(rel_err(ax[t], ay[t], az[t], rx, ry, rz))
+            res[theta] = (e, c.interactions().mean())
+            print(f"4M Plummer theta {theta}: median error {e:.3e}, "
+                  f"{res[theta][1]:.0f} interactions per body")
+    assert res[0.36][0] <= 1e-4 and res[0.3][0] < res[0.36][0]
+    assert res[0.3][1] > res[0.36][1]
+
+
 # ---- energy diagnostics ----------------------------------------------------------------
 @pytest.mark.parametrize("kind,n", [("disk", 5000), ("cube", 777)])
 def test_energy_bitexact(gpu, oracle, kind, n):
