@@ -40,10 +40,12 @@ MLP_FLOP_PER_PAIR = 18432  # 2 * (13*64 + 64*64 + 64*64 + 64*3) per network eval
 ERR_SAMPLE = 16384
 ERR_GATE = 1e-4
 # opening angles tried for the default workload, largest first; the first whose
-# median error on the initial state is at most ERR_GATE * THETA_MARGIN is used
+# median error on the initial state is at most ERR_GATE * THETA_MARGIN is
+# refined by bisection towards the failing one before it
 THETA_CANDIDATES = (1.0, 0.9, 0.8, 0.75, 0.7, 0.65, 0.6, 0.55, 0.5, 0.45, 0.42, 0.4, 0.38, 0.36,
                     0.34, 0.32, 0.3, 0.25, 0.2)
-THETA_MARGIN = 0.95
+THETA_MARGIN = 0.97
+THETA_BISECTIONS = 4  # refine between the first passing and the last failing candidate
 THETA_SWEEP = (0.3, 0.5, 0.7, 1.0)
 
 # Learned correction: training set (Plummer snapshots, accepted (cell, body)
@@ -67,11 +69,13 @@ CPU_SAMPLE_S = 10.0
 # HBM bytes per launch of the force kernel, keyed by (workload, theta), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script scaled by a device
 # copy of known size (FETCH x 2.0, WRITE x 1.0):
-#  - 4M Plummer, theta 0.36: 4,369,612 KiB x 2 read + 531,218 KiB written
+#  - 4M Plummer, theta 0.365: 4,284,166 KiB x 2 read + 531,241 KiB written,
+#    theta 0.36: 4,369,612 KiB x 2 read + 531,218 KiB written
 #    (profiles/bh_plummer_4m_hbm_counters.md);
 #  - 1M disk, theta 0.5: 565,982 KiB x 2 read + 134,447 KiB written
 #    (profiles/bh_disk_1m_hbm_counters.md).
 TRAFFIC_BYTES = {
+    ("bh_plummer_4m", 0.365): 9.318e9,
     ("bh_plummer_4m", 0.36): 9.493e9,
     ("bh_disk_1m", 0.5): 1.297e9,
 }
@@ -259,8 +263,10 @@ def err_targets(n):
 
 
 def choose_theta(ctx, d, dt):
-    """The largest candidate opening angle whose median error on the initial
-    state is at most ERR_GATE * THETA_MARGIN (one build + traversal each,
+    """The largest opening angle whose median error on the initial state is at
+    most ERR_GATE * THETA_MARGIN: the first passing candidate of
+    THETA_CANDIDATES (largest first), refined by THETA_BISECTIONS bisections
+    towards the failing candidate before it (one build + traversal each,
     against one FP64 direct sum over the sampled targets)."""
     n = d["m"].size
     ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
@@ -268,7 +274,8 @@ def choose_theta(ctx, d, dt):
     rx, ry, rz = ctx.forces_direct_targets(t)
     den = np.sqrt(rx ** 2 + ry ** 2 + rz ** 2)
     trials = []
-    for th in THETA_CANDIDATES:
+
+    def passes(th):
         ctx.set_params(theta=th)
         ctx.bh_build()
         ax, ay, az = ctx.bh_forces()
@@ -276,8 +283,20 @@ def choose_theta(ctx, d, dt):
         err = float(np.median(e))
         trials.append({"theta": th, "median_rel_err": err})
         log(f"theta {th}: initial-state median rel err {err:.3e}")
-        if err <= ERR_GATE * THETA_MARGIN:
-            return th, trials
+        return err <= ERR_GATE * THETA_MARGIN
+
+    failed = None
+    for th in THETA_CANDIDATES:
+        if passes(th):
+            lo, hi = th, failed
+            for _ in range(THETA_BISECTIONS if hi is not None else 0):
+                mid = round(0.5 * (lo + hi), 5)
+                if passes(mid):
+                    lo = mid
+                else:
+                    hi = mid
+            return lo, trials
+        failed = th
     raise RuntimeError(f"no candidate opening angle meets {ERR_GATE}: {trials}")
 
 
@@ -310,6 +329,11 @@ def main():
     ap.add_argument("--theta", type=float, default=None,
                     help="opening angle (default: the workload's; for bh_plummer_4m the largest "
                          "candidate meeting the error bound)")
+    ap.add_argument("--group-max", type=int, default=0,
+                    help="bodies per traversal group (0: the library's default)")
+    ap.add_argument("--pack-max", type=int, default=0,
+                    help="sibling groups are packed inside cells of at most this many bodies "
+                         "(0: the library's default)")
     ap.add_argument("--quadrupole", action="store_true",
                     help="Barnes-Hut cells with their quadrupole moments (FP32C traversal)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -354,7 +378,8 @@ def main():
     log(f"workload {args.workload}: generating {kind} n={n} ...")
     d = gen(kind, n, seed)
     ctx = nb.Context(n, theta=theta if theta is not None else 0.5, precision=prec,
-                     quadrupole=int(args.quadrupole))
+                     quadrupole=int(args.quadrupole), group_max=args.group_max,
+                     pack_max=args.pack_max)
     theta_search = None
     if algorithm == "BarnesHut" and theta is None:
         theta, theta_search = choose_theta(ctx, d, dt)
@@ -393,6 +418,7 @@ def main():
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
                    "cells": "quadrupole" if args.quadrupole else "monopole",
+                   "group_max": ctx.params.group_max or 32, "pack_max": ctx.params.pack_max or 128,
                    "dt": dt, "median_rel_err": err, "err_sample": sample, "gate": ERR_GATE,
                    "gate_met": gate_met, "ungated_value": round(raw_value, 1),
                    "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},

@@ -79,6 +79,13 @@ typedef struct nb_params {
      * group traversal, so a larger theta reaches the same accuracy.  The
      * FP64 modes ignore it.  Default 0. */
     uint32_t quadrupole;
+    /* No counterpart in the reference: the NB_PREC_FP32C traversal walks the
+     * octree once per group of nearby bodies, a cell of at most group_max
+     * bodies (1..64); sibling groups inside a cell of at most pack_max bodies
+     * are packed into groups of up to group_max.  0 selects the defaults
+     * (32 and 128). */
+    uint32_t group_max;
+    uint32_t pack_max;
 } nb_params;
 
 /* Fill *p with the reference defaults (G, eps2 = 1e-22, theta = 1.05,

@@ -268,7 +268,15 @@ bool params_ok(const nb_params *p) {
            (p->precision == NB_PREC_FP64 || p->precision == NB_PREC_FP32C ||
             p->precision == NB_PREC_FP64_FAST) &&
            p->storage_size_param >= 1 && p->stack_size_param >= 1 && std::isfinite(p->G) &&
-           std::isfinite(p->eps2) && p->eps2 >= 0 && std::isfinite(p->theta) && p->theta >= 0;
+           std::isfinite(p->eps2) && p->eps2 >= 0 && std::isfinite(p->theta) && p->theta >= 0 &&
+           p->group_max <= nb::kGroupLimit &&
+           (p->pack_max == 0 || p->pack_max >= (p->group_max ? p->group_max : nb::kGroupMax));
+}
+
+// the traversal group rule of the parameters (0: the defaults)
+nb::GroupRule group_rule(const nb_params &p) {
+    const uint32_t g = p.group_max ? p.group_max : nb::kGroupMax;
+    return nb::GroupRule{g, p.pack_max ? p.pack_max : std::max(nb::kPackMax, g)};
 }
 
 int build_tree(nb_ctx *c) {
@@ -289,7 +297,7 @@ int build_tree(nb_ctx *c) {
     mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
     const bool fp64 = c->p.precision != NB_PREC_FP32C;  // FP64 and FP64_FAST
-    nb::launch_com(t, c->p.G, fp64, !fp64, c->stream);
+    nb::launch_com(t, c->p.G, fp64, !fp64, group_rule(c->p), c->stream);
     c->quad_valid = false;
     if (!fp64 && (c->corr || c->p.quadrupole)) {
         nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->corr ? c->mcell : nullptr,
@@ -379,6 +387,8 @@ void nb_default_params(nb_params *p) {
     p->storage_size_param = 16;
     p->stack_size_param = 16;
     p->quadrupole = 0;
+    p->group_max = 0;
+    p->pack_max = 0;
 }
 
 const char *nb_strerror(int code) {

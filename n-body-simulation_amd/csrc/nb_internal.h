@@ -242,7 +242,7 @@ void launch_topology(const TreeBufs &t, const double *x, const double *y, const 
 // Centre of mass with the node records and traversal groups.  f64: FP64
 // records (cm, edge) for the reference-semantics traversal; f32: packed FP32
 // records and traversal groups for the group traversal.
-void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s);
+void launch_com(const TreeBufs &t, double G, bool f64, bool f32, GroupRule rule, hipStream_t s);
 // hdr->max_depth of the last build (not maintained by the build itself)
 void launch_max_depth(const TreeBufs &t, hipStream_t s);
 

@@ -1602,11 +1602,11 @@ void launch_max_depth(const TreeBufs &t, hipStream_t s) {
                        t.n, t.hdr);
 }
 
-void launch_com(const TreeBufs &t, double G, bool f64, bool f32, hipStream_t s) {
+void launch_com(const TreeBufs &t, double G, bool f64, bool f32, GroupRule rule, hipStream_t s) {
     const uint32_t nblocks = ceil_div(t.n, kComBlock);
     const RecOut o{G,          (int)f64,      (int)f32,      t.cm,
                    t.edge,     t.rec,         t.group_size,  t.group_slot,
-                   GroupRule{kGroupMax, kPackMax}};
+                   rule};
     hipLaunchKernelGGL(k_com_local, dim3(nblocks), dim3(kComLocalThreads), 0, s, t.n, t.icap, t.hdr,
                        (const double4 *)t.body4, (const uint32_t *)t.sidx, (const int32_t *)t.ref,
                        (const uint32_t *)t.chain, (const uint8_t *)t.int_depth,

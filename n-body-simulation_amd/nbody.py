@@ -63,7 +63,8 @@ class Params(ctypes.Structure):
     _fields_ = [("G", ctypes.c_double), ("eps2", ctypes.c_double), ("theta", ctypes.c_double),
                 ("precision", ctypes.c_uint32), ("sort_bodies", ctypes.c_uint32),
                 ("storage_size_param", ctypes.c_uint32), ("stack_size_param", ctypes.c_uint32),
-                ("quadrupole", ctypes.c_uint32)]
+                ("quadrupole", ctypes.c_uint32), ("group_max", ctypes.c_uint32),
+                ("pack_max", ctypes.c_uint32)]
 
 
 class NBodyError(RuntimeError):
