@@ -192,12 +192,20 @@ void enqueue_forces(nb_ensemble *e) {
 
 void enqueue_steps(nb_ensemble *e, uint32_t count) {
     const uint32_t n = (uint32_t)e->nbody;
+    // the kick ending step k - 1 and the kick-drift starting step k run as one
+    // pass (nothing reads v between them); the last step's kick writes v
     for (uint32_t k = 0; k < count; ++k) {
         mark(e, NB_PHASE_LF1, true);
-        nb::launch_ens_kick_drift(n, e->dt_sys, e->sys_of, e->ax, e->ay, e->az, e->vx, e->vy,
-                                  e->vz, e->vhx, e->vhy, e->vhz, e->x, e->y, e->z, e->stream);
+        if (k == 0)
+            nb::launch_ens_kick_drift(n, e->dt_sys, e->sys_of, e->ax, e->ay, e->az, e->vx, e->vy,
+                                      e->vz, e->vhx, e->vhy, e->vhz, e->x, e->y, e->z, e->stream);
+        else
+            nb::launch_ens_kick_kick_drift(n, e->dt_sys, e->sys_of, e->ax, e->ay, e->az, e->vhx,
+                                           e->vhy, e->vhz, e->x, e->y, e->z, e->stream);
         mark(e, NB_PHASE_LF1, false);
         enqueue_forces(e);
+    }
+    if (count > 0) {
         mark(e, NB_PHASE_LF2, true);
         nb::launch_ens_kick(n, e->dt_sys, e->sys_of, e->ax, e->ay, e->az, e->vhx, e->vhy, e->vhz,
                             e->vx, e->vy, e->vz, e->stream);

@@ -293,7 +293,16 @@ void launch_kick_drift(uint32_t n, double dt, const double *ax, const double *ay
 void launch_kick(uint32_t n, double dt, const double *ax, const double *ay, const double *az,
                  const double *vhx, const double *vhy, const double *vhz, double *vx,
                  double *vy, double *vz, hipStream_t s);
+// kick of step k and kick-drift of step k+1 in one pass; bit-identical to
+// launch_kick followed by launch_kick_drift, without storing v
+void launch_kick_kick_drift(uint32_t n, double dt, const double *ax, const double *ay,
+                            const double *az, double *vhx, double *vhy, double *vhz, double *x,
+                            double *y, double *z, hipStream_t s);
 // ensemble: body i advances by dt_sys[sys_of[i]]
+void launch_ens_kick_kick_drift(uint32_t n, const double *dt_sys, const uint16_t *sys_of,
+                                const double *ax, const double *ay, const double *az, double *vhx,
+                                double *vhy, double *vhz, double *x, double *y, double *z,
+                                hipStream_t s);
 void launch_ens_kick_drift(uint32_t n, const double *dt_sys, const uint16_t *sys_of,
                            const double *ax, const double *ay, const double *az, const double *vx,
                            const double *vy, const double *vz, double *vhx, double *vhy,
