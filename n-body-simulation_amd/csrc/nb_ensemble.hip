@@ -293,7 +293,7 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &e->items, nsys)) || (rc = ealloc(e, &e->item_off, nsys + 1ull)) ||
         (rc = ealloc(e, &e->qorder, nsys)) || (rc = ealloc(e, &e->bh_order, nsys)) ||
         (rc = ealloc(e, &e->cursor, 1)) || (rc = ealloc(e, &e->status, 1)) ||
-        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) ||
+        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.cell_off, nsys + 1ull)) ||
         (rc = ealloc(e, &f.radix_ctl, nb::radix_ctl_words((uint32_t)fo))) ||
         (rc = ealloc(e, &f.radix_ctl_sys, nb::radix_ctl_words((uint32_t)fo))))
         return bail(rc);

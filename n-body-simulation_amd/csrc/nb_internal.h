@@ -202,6 +202,9 @@ struct ForestBufs {
     uint32_t slots;
     uint16_t *fb_sys;
     uint32_t *radix_ctl, *radix_ctl_sys;
+    // per Barnes-Hut system in build order, + 1: first internal cell in the
+    // forest's cell sequence (the node-record pass)
+    uint32_t *cell_off;
 };
 
 // ---- launchers (each .hip file defines the kernels it launches) -------------

@@ -1178,6 +1178,28 @@ __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
 }
 
 // ---- 8. node records -----------------------------------------------------------
+// The FP32 traversal record of a node with centre of mass and mass c
+// (hi/lo split of the centre, G folded into the mass) and reference r.
+__device__ inline NodeF32 rec_f32(const double4 c, double G, int32_t r) {
+    NodeF32 o;
+    o.x = (float)c.x;
+    o.y = (float)c.y;
+    o.z = (float)c.z;
+    o.gm = (float)(G * c.w);
+    o.ref = r;
+    o.lx = (float)(c.x - (double)o.x);
+    o.ly = (float)(c.y - (double)o.y);
+    o.lz = (float)(c.z - (double)o.z);
+    return o;
+}
+
+// A body leaf's centre of mass as the reference forms it: numerator x m
+// divided by the mass at every visit (BarnesHutAlgorithm.cpp:351-353).
+__device__ inline double4 body_com(const double4 p) {
+    const double mb = p.w;
+    return make_double4((p.x * mb) / mb, (p.y * mb) / mb, (p.z * mb) / mb, mb);
+}
+
 __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__restrict__ body4,
                                  const int32_t *__restrict__ ref,
                                  const InternalSums *__restrict__ sums,
@@ -1200,25 +1222,13 @@ __device__ inline void record_at(uint64_t j, double root_edge, const double4 *__
         c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
         e = ldexp(root_edge, -depth);
     } else if (body) {
-        const double mb = p.w;
-        c = make_double4((p.x * mb) / mb, (p.y * mb) / mb, (p.z * mb) / mb, mb);
+        c = body_com(p);
     }
     if (write_f64) {
         cm[j] = c;
         edge[j] = e;
     }
-    if (write_f32) {
-        NodeF32 o;
-        o.x = (float)c.x;
-        o.y = (float)c.y;
-        o.z = (float)c.z;
-        o.gm = (float)(G * c.w);
-        o.ref = r;
-        o.lx = (float)(c.x - (double)o.x);
-        o.ly = (float)(c.y - (double)o.y);
-        o.lz = (float)(c.z - (double)o.z);
-        rec[j] = o;
-    }
+    if (write_f32) rec[j] = rec_f32(c, G, r);
 }
 
 // ---- ensemble: one workgroup builds one system's octree ----------------------------
@@ -1382,115 +1392,285 @@ __global__ void __launch_bounds__(kThreads) k_forest_sysrank(uint32_t slots,
                  (key[j] >> kForestLowBits);
 }
 
-__global__ void __launch_bounds__(kForestThreads) k_forest_build(
-    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order,
+// After the forest-wide sort, the build alternates stages over every body
+// slot of the forest (all systems at once, the whole GPU) with stages of one
+// workgroup per system, which hold what needs a barrier across one system:
+// the cell-numbering scan, the level-by-level centre of mass and the group
+// scan.  Every stage is the one-system build's per-element step; the trees
+// are bit-identical to a one-system build.
+
+// A slot's system and its index inside that system's slots.
+struct SlotSys {
+    uint32_t s, li;
+    EnsSystem S;
+};
+__device__ inline SlotSys slot_sys(uint32_t j, const uint16_t *__restrict__ fb_sys,
+                                   const EnsSystem *__restrict__ sys) {
+    SlotSys r;
+    r.s = fb_sys[j];
+    r.S = sys[r.s];
+    r.li = j - r.S.fb_off;
+    return r;
+}
+
+// 3. Keys again, of the bodies in sorted order (the second sort kept only the
+// high key bits), and the system-local body indices.  `sorted` may be
+// fb.sidx itself: each slot is read, then rewritten, by one thread.  Each
+// body's mass is checked once (zero mass is an error of the reference's
+// centre of mass).
+__global__ void __launch_bounds__(kThreads) k_forest_rekey(uint32_t slots,
+                                                           const uint16_t *__restrict__ fb_sys,
+                                                           const EnsSystem *__restrict__ sys,
+                                                           ForestBufs fb, const uint32_t *sorted) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= slots) return;
+    const SlotSys ss = slot_sys(j, fb_sys, sys);
+    if (!ss.S.bh || ss.li >= ss.S.n) return;
+    TreeHdr *h = fb.hdr + ss.s;
+    uint32_t q = sorted[j] - ss.S.fb_off;
+    if (q >= ss.S.n) {  // only after a failed sort, which the flag reports
+        raise_flag(h, kFlagSortWait);
+        q = 0;
+    }
+    Cell c{h->minx, h->miny, h->minz, h->edge};
+    const double4 p = fb.body4[ss.S.fb_off + q];  // one gathered line per body
+    fb.skey[j] = key_from(c, p.x, p.y, p.z);
+    fb.sidx[j] = q;
+    if (p.w == 0) raise_flag(h, kFlagZeroMass);
+}
+
+// 4. lcp of each adjacent pair of a system's sorted bodies; a run of bodies
+// sharing all 21 digits is extended and sorted by its first pair's thread.
+__global__ void __launch_bounds__(kThreads) k_forest_lcp(
+    uint32_t slots, const uint16_t *__restrict__ fb_sys, const EnsSystem *__restrict__ sys,
     const double *__restrict__ gx, const double *__restrict__ gy, const double *__restrict__ gz,
-    const double *__restrict__ gm, double G, ForestBufs fb,
-    const uint32_t *sorted,  // may be fb.sidx itself: each slot is read, then rewritten
-    uint32_t *items, uint32_t *status, GroupRule rule) {
+    ForestBufs fb) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= slots) return;
+    const SlotSys ss = slot_sys(j, fb_sys, sys);
+    if (!ss.S.bh || ss.li + 1 >= ss.S.n) return;
+    TreeHdr *h = fb.hdr + ss.s;
+    const Cell root{h->minx, h->miny, h->minz, h->edge};
+    const uint32_t fo = ss.S.fb_off, bo = ss.S.body_off;
+    lcp_at(ss.li, gx + bo, gy + bo, gz + bo, ss.S.n, root, h, fb.skey + fo, fb.sidx + fo,
+           fb.klo + fo, fb.lcp + fo);
+}
+
+// 5. chain (internal cells owned by each pair) and each system's deepest leaf,
+// 1 + max lcp: a wave inside one system reduces first, and the device-scope
+// RMW runs only when it raises the stored value.
+__global__ void __launch_bounds__(kThreads) k_forest_chain(uint32_t slots,
+                                                           const uint16_t *__restrict__ fb_sys,
+                                                           const EnsSystem *__restrict__ sys,
+                                                           ForestBufs fb) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t s = 0xffffffffu;
+    int d = -1;
+    if (j < slots) {
+        const SlotSys ss = slot_sys(j, fb_sys, sys);
+        s = ss.s;
+        if (ss.S.bh && ss.li + 1 < ss.S.n) {
+            const uint32_t fo = ss.S.fb_off;
+            chain_at(ss.li, fb.lcp + fo, fb.chain + fo);
+            d = fb.lcp[fo + ss.li];
+        }
+    }
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(s);
+    const bool one = __ballot(j < slots && s != s0) == 0;  // every lane in system s0
+    if (one) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) d = max(d, __shfl_xor(d, off, 64));
+        if ((threadIdx.x & 63u) != 0) d = -1;
+    }
+    if (d >= 0) {
+        TreeHdr *h = fb.hdr + s;
+        const uint32_t v = (uint32_t)d + 1;
+        if (v > __hip_atomic_load(&h->max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&h->max_depth, v);
+    }
+}
+
+// 6. Per system: cell numbering (exclusive scan of the chain), the internal
+// cell count, and the empty node slots the topology fills in.
+__global__ void __launch_bounds__(kForestThreads) k_forest_number(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, ForestBufs fb) {
     __shared__ uint32_t wsum[kForestWaves + 1];
-    __shared__ uint64_t kwin[kLinkBack + kForestThreads];  // topology: keys of a chunk
+    const uint32_t s = order[blockIdx.x];
+    const EnsSystem S = sys[s];
+    const uint32_t n = S.n;
+    TreeHdr *h = fb.hdr + s;
+    const uint32_t I_all = n > 1 ? wg_scan_array(fb.chain + S.fb_off, n - 1, wsum) : 0u;
+    if (threadIdx.x == 0) h->internal_count = I_all;
+    const uint32_t I = min(I_all, S.icap);
+    int32_t *ref = fb.ref + S.node_off;
+    for (uint64_t j = threadIdx.x; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
+}
+
+// 7. Topology of every body of the forest, as k_link: a block stages the keys
+// of its slots and kLinkBack before them (whatever systems they belong to;
+// a search only reads keys of its own system), every cell is local.
+__global__ void __launch_bounds__(kThreads) k_forest_link(uint32_t slots,
+                                                          const uint16_t *__restrict__ fb_sys,
+                                                          const EnsSystem *__restrict__ sys,
+                                                          ForestBufs fb) {
+    __shared__ uint64_t win[kLinkBack + kThreads];
+    const uint32_t b0 = blockIdx.x * kThreads;
+    const uint32_t w0 = b0 > kLinkBack ? b0 - kLinkBack : 0u;
+    const uint32_t w1 = min(slots, b0 + (uint32_t)kThreads);
+    for (uint32_t p = w0 + threadIdx.x; p < w1; p += kThreads) win[p - w0] = fb.skey[p];
+    __syncthreads();
+    const uint32_t j = b0 + threadIdx.x;
+    if (j >= slots) return;
+    const SlotSys ss = slot_sys(j, fb_sys, sys);
+    const EnsSystem &S = ss.S;
+    if (!S.bh || ss.li >= S.n) return;
+    const uint32_t fo = S.fb_off, io = S.int_off;
+    KeyView kv{fb.skey + fo, fb.klo + fo};
+    // window bounds relative to the system's first slot; the unsigned
+    // differences in KeyView::key place a position in or out of the window
+    // correctly also when the window starts before the system
+    kv.win = win;
+    kv.w0 = w0 - fo;
+    kv.w1 = w1 - fo;
+    link_at(ss.li, S.n, S.icap, fb.hdr + ss.s, kv, fb.sidx + fo, fb.lcp + fo, fb.chain + fo,
+            fb.ref + S.node_off, fb.int_depth + io, fb.int_parent + io, fb.int_node + io,
+            fb.body_parent + fo, fb.int_local + io, nullptr, 0u, fb.int_first + io, S.n, S.n);
+}
+
+// Cell pk's sums (sum_children's arithmetic: the 8 children in digit order
+// from 0.0) with every child's loads issued before any is used -- both
+// candidates of a child, its cell sums and count and its body record, an
+// index that does not apply reading entry 0, which every system has -- then
+// the traversal groups among its children (as group_mark_children), from the
+// counts at hand.
+__device__ __forceinline__ void sum_children_marks(uint32_t pk, const double4 *__restrict__ body4,
+                                                   const int32_t *__restrict__ ref,
+                                                   const uint32_t *__restrict__ int_first,
+                                                   InternalSums *sums, uint32_t *counts,
+                                                   TreeHdr *h, const RecOut &o) {
+    const uint64_t base = 1ull + 8ull * pk;
+    int32_t r[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) r[d] = ref[base + d];
+    const uint32_t first = int_first[pk];
+    InternalSums cs[8];
+    uint32_t cc[8];
+    double4 bp[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const uint32_t ci = r[d] >= 0 ? (uint32_t)r[d] : 0u;
+        const uint32_t bi = r[d] < kRefEmpty ? body_of_ref(r[d]) : 0u;
+        cs[d] = sums[ci];
+        cc[d] = counts[ci];
+        bp[d] = body4[bi];
+    }
+    double4 v[8];
+    uint32_t c[8];
+#pragma unroll
+    for (int d = 0; d < 8; ++d) {
+        const bool cell = r[d] >= 0, body = r[d] < kRefEmpty;
+        v[d] = cell ? make_double4(cs[d].x, cs[d].y, cs[d].z, cs[d].m)
+                    : body ? body_term(bp[d]) : make_double4(0.0, 0.0, 0.0, 0.0);
+        c[d] = cell ? cc[d] : body ? 1u : 0u;
+    }
+    InternalSums out;
+    uint32_t cnt;
+    add_children(r, v, c, out, cnt);
+    sums[pk] = out;
+    counts[pk] = cnt;
+    if (out.m == 0) raise_flag(h, kFlagZeroMass);
+    if (cnt > o.rule.group_max) put_child_groups(o, first, c);
+}
+
+// One system's centre of mass, level by level from the deepest cells (every
+// thread of the workgroup calls it): the cells are bucketed by depth into
+// lvl (16-bit cell indices in LDS, cnt/cur: kMaxDepth + 1 and kMaxDepth LDS
+// words), then each level's cells are summed, with their children's group
+// marks, between two barriers.
+constexpr uint32_t kLvlLds = 32768;  // cells whose level lists fit in LDS (64 KB)
+__device__ __forceinline__ void com_levels_lds(uint32_t I, const uint8_t *__restrict__ int_depth,
+                                               uint16_t *lvl, uint32_t *cnt, uint32_t *cur,
+                                               const double4 *__restrict__ body4,
+                                               const int32_t *__restrict__ ref,
+                                               const uint32_t *__restrict__ int_first,
+                                               InternalSums *sums, uint32_t *counts, TreeHdr *h,
+                                               const RecOut &o) {
+    const uint32_t t = threadIdx.x, nt = blockDim.x;
+    for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += nt) cnt[d] = 0;
+    __syncthreads();
+    for (uint32_t k = t; k < I; k += nt)
+        atomicAdd(&cnt[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u);
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int d = 0; d < kMaxDepth; ++d) {
+            const uint32_t c = cnt[d];
+            cnt[d] = run;
+            cur[d] = run;
+            run += c;
+        }
+        cnt[kMaxDepth] = run;
+    }
+    __syncthreads();
+    for (uint32_t k = t; k < I; k += nt)
+        lvl[atomicAdd(&cur[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u)] =
+            (uint16_t)k;
+    __syncthreads();
+    for (int d = kMaxDepth - 1; d >= 0; --d) {
+        const uint32_t lb = cnt[d], le = cnt[d + 1];
+        if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
+        for (uint32_t j = lb + t; j < le; j += nt)
+            sum_children_marks(lvl[j], body4, ref, int_first, sums, counts, h, o);
+        __syncthreads();
+    }
+}
+
+// 8. Per system: centre of mass (every cell local), the root's record, the
+// traversal groups; then the system's group count, and its status flags
+// merged into *status.
+__global__ void __launch_bounds__(kForestThreads) k_forest_com(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, double G,
+    ForestBufs fb, uint32_t *items, uint32_t *status, GroupRule rule) {
+    __shared__ uint32_t wsum[kForestWaves + 1];
     __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
     __shared__ uint32_t lvl_c[kMaxDepth];      // fill cursors
-    __shared__ int lcp_max;
+    __shared__ uint16_t lvl16[kLvlLds];        // cells by depth (systems of at most kLvlLds cells)
     const uint32_t s = order[blockIdx.x];
     const EnsSystem S = sys[s];
     const uint32_t n = S.n, t = threadIdx.x;
-    const double *x = gx + S.body_off, *y = gy + S.body_off, *z = gz + S.body_off,
-                 *m = gm + S.body_off;
     TreeHdr *h = fb.hdr + s;
     const uint32_t fo = S.fb_off;
-    uint64_t *skey = fb.skey + fo, *klo = fb.klo + fo;
-    uint32_t *sidx = fb.sidx + fo, *chain = fb.chain + fo,
-             *body_parent = fb.body_parent + fo, *group_size = fb.group_size + fo,
-             *group_slot = fb.group_slot + fo;
-    int32_t *lcp = fb.lcp + fo;
+    uint32_t *group_size = fb.group_size + fo, *group_slot = fb.group_slot + fo;
     uint2 *groups = fb.groups + fo;
-    double4 *body4 = fb.body4 + fo;
-    uint8_t *int_depth = fb.int_depth + S.int_off, *int_local = fb.int_local + S.int_off;
-    uint32_t *int_parent = fb.int_parent + S.int_off, *int_node = fb.int_node + S.int_off,
-             *lvl = fb.int_lvl + S.int_off, *counts = fb.int_count + S.int_off,
+    const double4 *body4 = fb.body4 + fo;
+    const uint8_t *int_depth = fb.int_depth + S.int_off;
+    uint32_t *lvl = fb.int_lvl + S.int_off, *counts = fb.int_count + S.int_off,
              *int_first = fb.int_first + S.int_off;
     InternalSums *sums = fb.int_sums + S.int_off;
-    int32_t *ref = fb.ref + S.node_off;
-    NodeF32 *rec = fb.rec + S.node_off;
-    const uint32_t icap = S.icap;
-
-    if (t == 0) lcp_max = -1;
-    // the root cell, from the header k_forest_aabb wrote
-    const Cell root{h->minx, h->miny, h->minz, h->edge};
-    // 2./3. the system's slots come sorted from the forest-wide radix sort:
-    // body indices in key order (ties in index order), and their keys
-    for (uint32_t i = t; i < n; i += kForestThreads) {
-        uint32_t li = sorted[fo + i] - fo;
-        if (li >= n) {  // only after a failed sort, which the flag reports
-            raise_flag(h, kFlagSortWait);
-            li = 0;
-        }
-        Cell c = root;
-        skey[i] = key_from(c, x[li], y[li], z[li]);
-        sidx[i] = li;
-    }
-    __syncthreads();
-    // 4./5. lcp, deep runs, cell numbering
-    for (uint32_t i = t; i + 1 < n; i += kForestThreads)
-        lcp_at(i, x, y, z, n, root, h, skey, sidx, klo, lcp);
-    __syncthreads();
-    {
-        int lmax = -1;  // deepest cell shared by two neighbours
-        for (uint32_t i = t; i + 1 < n; i += kForestThreads) {
-            chain_at(i, lcp, chain);
-            lmax = max(lmax, lcp[i]);
-        }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off, 64));
-        if ((t & 63u) == 0) atomicMax(&lcp_max, lmax);
-    }
-    __syncthreads();
-    if (t == 0 && lcp_max >= 0) h->max_depth = (uint32_t)lcp_max + 1;  // the deepest leaf
-    const uint32_t I_all = n > 1 ? wg_scan_array(chain, n - 1, wsum) : 0u;
-    if (t == 0) h->internal_count = I_all;
-    const uint32_t I = min(I_all, icap);
-    // 6. topology
-    for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
-    __syncthreads();
-    // a chunk of kForestThreads bodies at a time, with the chunk's keys and
-    // kLinkBack before them in LDS: the cell-start searches walk back from
-    // each body, mostly inside that window (outside it they read global memory)
-    for (uint32_t c0 = 0; c0 < n; c0 += kForestThreads) {
-        const uint32_t w0 = c0 > kLinkBack ? c0 - kLinkBack : 0u;
-        const uint32_t w1 = min(n, c0 + (uint32_t)kForestThreads);
-        __syncthreads();  // the previous chunk is done with the window
-        for (uint32_t p = w0 + t; p < w1; p += kForestThreads) kwin[p - w0] = skey[p];
-        __syncthreads();
-        KeyView kv{skey, klo};
-        kv.win = kwin;
-        kv.w0 = w0;
-        kv.w1 = w1;
-        const uint32_t i = c0 + t;
-        if (i < n)
-            link_at(i, n, icap, h, kv, sidx, lcp, chain, ref, int_depth, int_parent, int_node,
-                    body_parent, int_local, nullptr, 0u, int_first, n, n);
-    }
-    __syncthreads();
-    // 7. centre of mass, level by level: every cell is local to this workgroup
-    for (uint32_t i = t; i < n; i += kForestThreads)
-        if (m[i] == 0) raise_flag(h, kFlagZeroMass);
-    com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c,
-               [&](uint32_t k) { sum_children(k, body4, ref, sums, counts, h); });
-    // 8. records and traversal groups
-    const double root_edge = root.e;
-    for (uint64_t j = t; j < 1 + 8ull * I; j += kForestThreads)
-        record_at(j, root_edge, body4, ref, sums, int_depth, G, 0, 1, nullptr, nullptr, rec);
+    const int32_t *ref = fb.ref + S.node_off;
+    const uint32_t I = min(h->internal_count, S.icap);
+    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot, rule};
     for (uint32_t i = t; i < n; i += kForestThreads) {
         group_size[i] = 0;
         group_slot[i] = 0;
     }
-    __syncthreads();
-    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot, rule};
-    for (uint32_t k = t; k < I; k += kForestThreads)
-        group_mark_children(k, counts, ref, int_first, og);
-    if (t == 0 && (n == 1 || counts[0] <= rule.group_max)) put_group(og, 0, n);  // the root
+    __syncthreads();  // the marks below come after every clear
+    if (I <= kLvlLds) {  // group marks made with the sums
+        com_levels_lds(I, int_depth, lvl16, lvl_n, lvl_c, body4, ref, int_first, sums, counts, h,
+                       og);
+    } else {
+        com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c,
+                   [&](uint32_t k) { sum_children(k, body4, ref, sums, counts, h); });
+        for (uint32_t k = t; k < I; k += kForestThreads)
+            group_mark_children(k, counts, ref, int_first, og);
+    }
+    if (t == 0) {
+        // the root's record (the other node records: k_forest_records) and
+        // group, if the root holds at most group_max bodies
+        record_at(0, h->edge, body4, ref, sums, int_depth, G, 0, 1, nullptr, nullptr,
+                  fb.rec + S.node_off);
+        if (n == 1 || counts[0] <= rule.group_max) put_group(og, 0, n);
+    }
     __syncthreads();
     const uint32_t ng = wg_scan_array(group_slot, n, wsum);
     for (uint32_t i = t; i < n; i += kForestThreads) {
@@ -1504,6 +1684,62 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_build(
         // flags were raised with device-scope atomics: read the word the same way
         const uint32_t st = __hip_atomic_load(&h->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (st) atomicOr(status, st);
+    }
+}
+
+// 9. Exclusive scan, in system order, of the internal cell counts:
+// cell_off[k] = first cell of order[k] in the forest's cell sequence,
+// cell_off[nbh] = all cells.
+__global__ void __launch_bounds__(kForestThreads) k_forest_cell_offsets(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, uint32_t nbh,
+    ForestBufs fb, uint32_t *cell_off) {
+    __shared__ uint32_t wsum[kForestWaves + 1];
+    for (uint32_t k = threadIdx.x; k < nbh; k += kForestThreads) {
+        const uint32_t s = order[k];
+        cell_off[k] = min(fb.hdr[s].internal_count, sys[s].icap);
+    }
+    __syncthreads();
+    const uint32_t total = wg_scan_array(cell_off, nbh, wsum);
+    if (threadIdx.x == 0) cell_off[nbh] = total;
+}
+
+// 10. Node records of the child slots of every internal cell of the forest,
+// consecutive threads on consecutive slots: slot u of the sequence is child
+// u % 8 of the forest's cell u / 8.  A block takes one contiguous range of the
+// sequence and finds its first system by bisection of cell_off.
+constexpr uint32_t kRecordBlocks = 2048;
+__global__ void __launch_bounds__(kThreads) k_forest_records(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, uint32_t nbh,
+    double G, ForestBufs fb, const uint32_t *__restrict__ cell_off) {
+    __shared__ uint32_t k_first;
+    const uint64_t total = 8ull * cell_off[nbh];
+    const uint64_t per = ((total + gridDim.x - 1) / gridDim.x + kThreads - 1) / kThreads * kThreads;
+    const uint64_t u0 = (uint64_t)blockIdx.x * per;
+    const uint64_t u1 = min(total, u0 + per);
+    if (u0 >= u1) return;
+    if (threadIdx.x == 0) {  // cell_off[lo] * 8 <= u0 < cell_off[hi] * 8
+        uint32_t lo = 0, hi = nbh;
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (8ull * cell_off[mid] <= u0) lo = mid; else hi = mid;
+        }
+        k_first = lo;
+    }
+    __syncthreads();
+    uint32_t k = k_first, s = 0xffffffffu;
+    EnsSystem S{};
+    double root_edge = 0;
+    for (uint64_t u = u0 + threadIdx.x; u < u1; u += kThreads) {
+        while (8ull * cell_off[k + 1] <= u) ++k;  // systems without cells are passed over
+        if (order[k] != s) {
+            s = order[k];
+            S = sys[s];
+            root_edge = fb.hdr[s].edge;
+        }
+        const uint64_t j = 1 + (u - 8ull * cell_off[k]);  // node slot in the system
+        record_at(j, root_edge, fb.body4 + S.fb_off, fb.ref + S.node_off,
+                  fb.int_sums + S.int_off, fb.int_depth + S.int_off, G, 0, 1, nullptr, nullptr,
+                  fb.rec + S.node_off);
     }
 }
 
@@ -1543,8 +1779,20 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
         std::swap(k, ka);
         std::swap(v, va);
     }
-    hipLaunchKernelGGL(k_forest_build, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, x, y, z,
-                       m, G, fb, (const uint32_t *)v, items, status, rule);
+    const uint16_t *fs = fb.fb_sys;
+    hipLaunchKernelGGL(k_forest_rekey, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb,
+                       (const uint32_t *)v);
+    hipLaunchKernelGGL(k_forest_lcp, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, x, y, z,
+                       fb);
+    hipLaunchKernelGGL(k_forest_chain, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb);
+    hipLaunchKernelGGL(k_forest_number, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, fb);
+    hipLaunchKernelGGL(k_forest_link, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb);
+    hipLaunchKernelGGL(k_forest_com, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, G, fb,
+                       items, status, rule);
+    hipLaunchKernelGGL(k_forest_cell_offsets, dim3(1), dim3(kForestThreads), 0, s, sys, order, nbh,
+                       fb, fb.cell_off);
+    hipLaunchKernelGGL(k_forest_records, dim3(kRecordBlocks), dim3(kThreads), 0, s, sys, order,
+                       nbh, G, fb, (const uint32_t *)fb.cell_off);
 }
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,

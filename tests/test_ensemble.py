@@ -46,6 +46,50 @@ def test_ensemble_bh_forces_match_one_system(gpu):
 
 
 @pytest.mark.gpu
+def test_ensemble_deep_trees_match_one_system(gpu):
+    """Systems whose octrees need the second key word (pairs closer than
+    edge / 2^21) and deep clusters (about 31 levels), beside ordinary ones:
+    canonical trees and forces bit-identical to the one-system build."""
+    rng = np.random.default_rng(11)
+    base = rng.uniform(-1, 1, (300, 3))
+    pairs = np.concatenate([base, base + rng.normal(0, 1e-9, (300, 3))])
+    centres = rng.uniform(-1, 1, (3, 3))
+    clumps = (centres[:, None, :] + rng.uniform(-5e-10, 5e-10, (3, 700, 3))).reshape(-1, 3)
+
+    def system(p, m):
+        n = p.shape[0]
+        return dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+                    vy=np.zeros(n), vz=np.zeros(n), m=m)
+
+    parts = [system(pairs, rng.uniform(1, 2, 600)), gpu.gen_plummer(5000, 12),
+             system(clumps, rng.uniform(1e20, 1e22, 2100)), gpu.gen_star_disk(3000, 13)]
+    sizes = [p["m"].size for p in parts]
+    st = {k: np.concatenate([p[k] for p in parts]) for k in parts[0]}
+    dt = np.array([1e-3, 1e-3, 1e-3, 1.0 / 24])
+    # about 25 single-child levels per close pair: more nodes than 16 N
+    with gpu.Ensemble(sizes, theta=0.6, storage_size_param=512) as e:
+        e.set_tuning(bh_threshold=0, waves=4096)
+        e.set_state(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], st["m"], dt)
+        e.forces()
+        ax, ay, az = e.accel()
+        cans = [e.tree_canonical(s) for s in range(len(sizes))]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for s, d in enumerate(parts):
+        with gpu.Context(sizes[s], theta=0.6, precision=gpu.PREC_FP32C,
+                         storage_size_param=512) as ctx:
+            ctx.set_state(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+            ctx.bh_build()
+            one = ctx.tree_canonical()
+            ctx.bh_forces()
+            rx, ry, rz = ctx.accel()
+        for k in one:
+            assert np.array_equal(cans[s][k], one[k]), f"system {s}: {k}"
+        sl = slice(off[s], off[s + 1])
+        assert np.array_equal(ax[sl], rx) and np.array_equal(ay[sl], ry) and \
+            np.array_equal(az[sl], rz), f"system {s} (n={sizes[s]})"
+
+
+@pytest.mark.gpu
 def test_ensemble_many_systems_match_one_system(gpu):
     # 600 systems: the sort that gathers each system's slots takes 10 system
     # bits (5 radix passes, its own control words); two builds in a row reuse
