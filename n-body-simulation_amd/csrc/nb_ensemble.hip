@@ -162,6 +162,30 @@ int apply_tuning(nb_ensemble *e) {
     std::iota(qorder.begin(), qorder.end(), 0u);
     std::stable_sort(qorder.begin(), qorder.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
     e->nbh = (uint32_t)bh_order.size();
+    // the centre-of-mass launches: every block of every Barnes-Hut system,
+    // the super-blocks of systems of two or more blocks, the systems of more
+    // than one super-block
+    std::vector<uint2> blk, span, big;
+    for (uint32_t s : bh_order) {
+        const uint32_t blocks = nb::ceil_div(e->sys_host[s].n, nb::kComBlock);
+        for (uint32_t b = 0; b < blocks; ++b) blk.push_back(make_uint2(s, b));
+        if (blocks >= 2)
+            for (uint32_t b = 0; b < nb::ceil_div(blocks, nb::kComSuperBlocks); ++b)
+                span.push_back(make_uint2(s, b));
+        if (blocks > nb::kComSuperBlocks) big.push_back(make_uint2(s, 0u));
+    }
+    e->fb.nblk = (uint32_t)blk.size();
+    e->fb.nspan = (uint32_t)span.size();
+    e->fb.nbig = (uint32_t)big.size();
+    if (!blk.empty())
+        ENS_HIP(e, hipMemcpyAsync(e->fb.blk_tab, blk.data(), blk.size() * sizeof(uint2),
+                                  hipMemcpyHostToDevice, e->stream));
+    if (!span.empty())
+        ENS_HIP(e, hipMemcpyAsync(e->fb.span_tab, span.data(), span.size() * sizeof(uint2),
+                                  hipMemcpyHostToDevice, e->stream));
+    if (!big.empty())
+        ENS_HIP(e, hipMemcpyAsync(e->fb.big_tab, big.data(), big.size() * sizeof(uint2),
+                                  hipMemcpyHostToDevice, e->stream));
     ENS_HIP(e, hipMemcpyAsync(e->sys, e->sys_host.data(), S * sizeof(EnsSystem),
                               hipMemcpyHostToDevice, e->stream));
     if (e->nbh)
@@ -250,7 +274,7 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
     // storage layout
     e->sys_host.resize(nsys);
     auto up = [](uint64_t v) { return (v + nb::kEnsAlign - 1) / nb::kEnsAlign * nb::kEnsAlign; };
-    uint64_t bo = 0, fo = 0, io = 0, no = 0;
+    uint64_t bo = 0, fo = 0, io = 0, no = 0, span = 0, xo = 0, nblk = 0, nsup = 0;
     for (uint32_t s = 0; s < nsys; ++s) {
         EnsSystem &es = e->sys_host[s];
         es.body_off = (uint32_t)bo;
@@ -263,13 +287,21 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         es.int_off = (uint32_t)io;
         es.node_off = no;
         es.bh = 0;
+        es.span_off = (uint32_t)span;
+        es.cross_off = (uint32_t)xo;
+        const uint64_t blocks = nb::ceil_div(sizes[s], nb::kComBlock);
+        nblk += blocks;
+        nsup += nb::ceil_div(blocks, nb::kComSuperBlocks);
+        span += (uint64_t)nb::kMaxDepth * blocks;
+        xo += (uint64_t)nb::kMaxDepth * (nb::ceil_div(sizes[s], nb::kComSuper) + 1);
         bo += sizes[s];
         fo = up(fo + sizes[s]);
         io = up(io + es.icap);
         no = up(no + 1 + 8ull * es.icap);
     }
     // the forest-wide radix sorts take fewer than 2^30 body slots
-    if (io >= (1ull << 32) || fo >= (1ull << 30)) return bail(NB_E_INVALID_ARG);
+    if (io >= (1ull << 32) || fo >= (1ull << 30) || span >= (1ull << 32) || xo >= (1ull << 32))
+        return bail(NB_E_INVALID_ARG);
     int rc;
     double **st[] = {&e->x, &e->y, &e->z, &e->vx, &e->vy, &e->vz, &e->vhx,
                      &e->vhy, &e->vhz, &e->ax, &e->ay, &e->az, &e->m};
@@ -293,7 +325,7 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &e->items, nsys)) || (rc = ealloc(e, &e->item_off, nsys + 1ull)) ||
         (rc = ealloc(e, &e->qorder, nsys)) || (rc = ealloc(e, &e->bh_order, nsys)) ||
         (rc = ealloc(e, &e->cursor, 1)) || (rc = ealloc(e, &e->status, 1)) ||
-        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.cell_off, nsys + 1ull)) ||
+        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.spanning, span)) || (rc = ealloc(e, &f.cross, xo)) || (rc = ealloc(e, &f.blk_tab, nblk)) || (rc = ealloc(e, &f.span_tab, nsup)) || (rc = ealloc(e, &f.big_tab, nsys)) ||
         (rc = ealloc(e, &f.radix_ctl, nb::radix_ctl_words((uint32_t)fo))) ||
         (rc = ealloc(e, &f.radix_ctl_sys, nb::radix_ctl_words((uint32_t)fo))))
         return bail(rc);

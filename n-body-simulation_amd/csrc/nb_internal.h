@@ -178,6 +178,10 @@ struct EnsSystem {
     uint64_t node_off;
     uint32_t fb_off;
     uint32_t bh;     // 1: Barnes-Hut, 0: direct sum
+    // the centre-of-mass passes' tables: kMaxDepth entries per kComBlock-body
+    // block (cells crossing its end), and per kComSuper-body super-block + 1
+    // (cells crossing super-block ends)
+    uint32_t span_off, cross_off;
 };
 constexpr uint32_t kEnsAlign = 64;
 
@@ -202,9 +206,13 @@ struct ForestBufs {
     uint32_t slots;
     uint16_t *fb_sys;
     uint32_t *radix_ctl, *radix_ctl_sys;
-    // per Barnes-Hut system in build order, + 1: first internal cell in the
-    // forest's cell sequence (the node-record pass)
-    uint32_t *cell_off;
+    uint32_t *spanning, *cross;  // tables of every system (EnsSystem::span_off, cross_off)
+    // the centre-of-mass launches over the Barnes-Hut systems: (system,
+    // block) of every kComBlock-body block, (system, super-block) of the
+    // systems of two or more blocks, and the systems of more than one
+    // super-block; counts kept on the host
+    uint2 *blk_tab, *span_tab, *big_tab;
+    uint32_t nblk, nspan, nbig;
 };
 
 // ---- launchers (each .hip file defines the kernels it launches) -------------

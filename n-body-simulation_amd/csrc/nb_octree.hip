@@ -743,12 +743,14 @@ static_assert(kComBlock % kComLocalThreads == 0 && (8 * kComLdsCells) % kComLoca
 constexpr uint16_t kCodeEmpty = 0xffffu, kCodeBody = 0xfffeu, kCodeFar = 0xfffdu;
 constexpr uint16_t kCodeAt = 0x8000u;    // | sorted position - block start: a placed body
 static_assert(kComLdsCells < kCodeAt && kComBlock <= 0x7ffdu, "16-bit LDS codes");
-__global__ void __launch_bounds__(kComLocalThreads) k_com_local(
-    uint32_t n, uint32_t icap, TreeHdr *h, const double4 *__restrict__ body4,
+// Block blk of one octree (the one-system kernel below, and the ensemble's,
+// which runs the blocks of every system in one launch).
+__device__ __forceinline__ void com_local_block(
+    uint32_t blk, uint32_t n, uint32_t icap, TreeHdr *h, const double4 *__restrict__ body4,
     const uint32_t *__restrict__ sidx, const int32_t *__restrict__ ref,
     const uint32_t *__restrict__ off, const uint8_t *__restrict__ int_depth,
     const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *lvl,
-    InternalSums *sums, uint32_t *counts, RecOut o) {
+    InternalSums *sums, uint32_t *counts, const RecOut &o) {
     constexpr uint32_t nt = kComLocalThreads;
     __shared__ uint32_t cnt[kMaxDepth + 1], cur[kMaxDepth];
     __shared__ unsigned long long levels;  // bit d: a local cell of depth d
@@ -760,7 +762,7 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
     __shared__ uint16_t first_s[kComLdsCells];  // first sorted position - block start
     __shared__ uint8_t depth_s[kComLdsCells];   // depth; kMaxDepth if not local
     const uint32_t t = threadIdx.x;
-    const uint32_t s = blockIdx.x * kComBlock;
+    const uint32_t s = blk * kComBlock;
     const uint32_t e = min(n, s + kComBlock);
     const double root_edge = h->edge;
     {
@@ -938,6 +940,16 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
     }
 }
 
+__global__ void __launch_bounds__(kComLocalThreads) k_com_local(
+    uint32_t n, uint32_t icap, TreeHdr *h, const double4 *__restrict__ body4,
+    const uint32_t *__restrict__ sidx, const int32_t *__restrict__ ref,
+    const uint32_t *__restrict__ off, const uint8_t *__restrict__ int_depth,
+    const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *lvl,
+    InternalSums *sums, uint32_t *counts, RecOut o) {
+    com_local_block(blockIdx.x, n, icap, h, body4, sidx, ref, off, int_depth, int_local, int_first,
+                    lvl, sums, counts, o);
+}
+
 // Pass 2: the cells crossing block boundaries (the table
 // spanning[block * kMaxDepth + depth]), in two launches.  want = 0: one
 // workgroup per super-block of kComSuperBlocks blocks sums the cells that
@@ -952,11 +964,13 @@ __global__ void __launch_bounds__(kComLocalThreads) k_com_local(
 constexpr uint32_t kSpanCells = 448;
 constexpr uint32_t kChildEmpty = 0xffffffffu;
 constexpr uint32_t kChildHere = 0x80000000u;  // | LDS id of a child summed here
-__global__ void __launch_bounds__(kComThreads) k_com_span(
-    uint32_t nblocks, uint32_t rows, uint32_t want, TreeHdr *h, const double4 *__restrict__ body4,
-    const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning,
-    const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *cid,
-    InternalSums *sums, uint32_t *counts, uint32_t *cross, uint32_t cross_cap, RecOut o) {
+// Super-block (want 0) or cross list (want 2) sb of one octree.
+__device__ __forceinline__ void com_span_block(
+    uint32_t sb, uint32_t nblocks, uint32_t rows, uint32_t want, TreeHdr *h,
+    const double4 *__restrict__ body4, const int32_t *__restrict__ ref,
+    const uint32_t *__restrict__ spanning, const uint8_t *__restrict__ int_local,
+    const uint32_t *__restrict__ int_first, uint32_t *cid, InternalSums *sums, uint32_t *counts,
+    uint32_t *cross, uint32_t cross_cap, const RecOut &o) {
     __shared__ uint32_t ncell, cnt[kMaxDepth + 1], cur[kMaxDepth];
     __shared__ uint32_t nx, xbase, xl[kMaxDepth];  // want 0: this super-block's crossing cells
     __shared__ uint32_t list[kSpanCells];  // cell index by LDS id
@@ -968,7 +982,7 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
     __shared__ double4 sv[kSpanCells];
     __shared__ uint32_t sn[kSpanCells];
     const uint32_t t = threadIdx.x;
-    const uint32_t r0 = blockIdx.x * rows;
+    const uint32_t r0 = sb * rows;
     const uint32_t r1 = min(nblocks, r0 + rows);
     const uint64_t slot0 = (uint64_t)r0 * kMaxDepth, slot1 = (uint64_t)r1 * kMaxDepth;
     const double root_edge = h->edge;
@@ -1143,6 +1157,15 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
             if (pc <= o.rule.group_max) put_group(o, fst[id], pc);
         }
     }
+}
+
+__global__ void __launch_bounds__(kComThreads) k_com_span(
+    uint32_t nblocks, uint32_t rows, uint32_t want, TreeHdr *h, const double4 *__restrict__ body4,
+    const int32_t *__restrict__ ref, const uint32_t *__restrict__ spanning,
+    const uint8_t *__restrict__ int_local, const uint32_t *__restrict__ int_first, uint32_t *cid,
+    InternalSums *sums, uint32_t *counts, uint32_t *cross, uint32_t cross_cap, RecOut o) {
+    com_span_block(blockIdx.x, nblocks, rows, want, h, body4, ref, spanning, int_local, int_first,
+                   cid, sums, counts, cross, cross_cap, o);
 }
 
 // ---- traversal groups -------------------------------------------------------------
@@ -1491,7 +1514,8 @@ __global__ void __launch_bounds__(kThreads) k_forest_chain(uint32_t slots,
 }
 
 // 6. Per system: cell numbering (exclusive scan of the chain), the internal
-// cell count, and the empty node slots the topology fills in.
+// cell count, the empty node slots the topology fills in, and an empty table
+// of block-crossing cells.
 __global__ void __launch_bounds__(kForestThreads) k_forest_number(
     const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, ForestBufs fb) {
     __shared__ uint32_t wsum[kForestWaves + 1];
@@ -1504,11 +1528,15 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_number(
     const uint32_t I = min(I_all, S.icap);
     int32_t *ref = fb.ref + S.node_off;
     for (uint64_t j = threadIdx.x; j < 1 + 8ull * I; j += kForestThreads) ref[j] = kRefEmpty;
+    uint32_t *spanning = fb.spanning + S.span_off;
+    const uint32_t span_cap = kMaxDepth * ((n + kComBlock - 1) / kComBlock);
+    for (uint32_t j = threadIdx.x; j < span_cap; j += kForestThreads) spanning[j] = kNoCell;
 }
 
 // 7. Topology of every body of the forest, as k_link: a block stages the keys
 // of its slots and kLinkBack before them (whatever systems they belong to;
-// a search only reads keys of its own system), every cell is local.
+// a search only reads keys of its own system).  Cells are local to, or cross,
+// the system's kComBlock-body blocks as in the one-system build.
 __global__ void __launch_bounds__(kThreads) k_forest_link(uint32_t slots,
                                                           const uint16_t *__restrict__ fb_sys,
                                                           const EnsSystem *__restrict__ sys,
@@ -1534,144 +1562,71 @@ __global__ void __launch_bounds__(kThreads) k_forest_link(uint32_t slots,
     kv.w1 = w1 - fo;
     link_at(ss.li, S.n, S.icap, fb.hdr + ss.s, kv, fb.sidx + fo, fb.lcp + fo, fb.chain + fo,
             fb.ref + S.node_off, fb.int_depth + io, fb.int_parent + io, fb.int_node + io,
-            fb.body_parent + fo, fb.int_local + io, nullptr, 0u, fb.int_first + io, S.n, S.n);
+            fb.body_parent + fo, fb.int_local + io, fb.spanning + S.span_off,
+            kMaxDepth * ((S.n + kComBlock - 1) / kComBlock), fb.int_first + io, kComBlock,
+            kComSuper);
 }
 
-// Cell pk's sums (sum_children's arithmetic: the 8 children in digit order
-// from 0.0) with every child's loads issued before any is used -- both
-// candidates of a child, its cell sums and count and its body record, an
-// index that does not apply reading entry 0, which every system has -- then
-// the traversal groups among its children (as group_mark_children), from the
-// counts at hand.
-__device__ __forceinline__ void sum_children_marks(uint32_t pk, const double4 *__restrict__ body4,
-                                                   const int32_t *__restrict__ ref,
-                                                   const uint32_t *__restrict__ int_first,
-                                                   InternalSums *sums, uint32_t *counts,
-                                                   TreeHdr *h, const RecOut &o) {
-    const uint64_t base = 1ull + 8ull * pk;
-    int32_t r[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) r[d] = ref[base + d];
-    const uint32_t first = int_first[pk];
-    InternalSums cs[8];
-    uint32_t cc[8];
-    double4 bp[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        const uint32_t ci = r[d] >= 0 ? (uint32_t)r[d] : 0u;
-        const uint32_t bi = r[d] < kRefEmpty ? body_of_ref(r[d]) : 0u;
-        cs[d] = sums[ci];
-        cc[d] = counts[ci];
-        bp[d] = body4[bi];
-    }
-    double4 v[8];
-    uint32_t c[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        const bool cell = r[d] >= 0, body = r[d] < kRefEmpty;
-        v[d] = cell ? make_double4(cs[d].x, cs[d].y, cs[d].z, cs[d].m)
-                    : body ? body_term(bp[d]) : make_double4(0.0, 0.0, 0.0, 0.0);
-        c[d] = cell ? cc[d] : body ? 1u : 0u;
-    }
-    InternalSums out;
-    uint32_t cnt;
-    add_children(r, v, c, out, cnt);
-    sums[pk] = out;
-    counts[pk] = cnt;
-    if (out.m == 0) raise_flag(h, kFlagZeroMass);
-    if (cnt > o.rule.group_max) put_child_groups(o, first, c);
+// The one-system build's view of system s, for the centre-of-mass passes.
+struct ForestTree {
+    EnsSystem S;
+    TreeHdr *h;
+    RecOut o;
+};
+__device__ inline ForestTree forest_tree(const EnsSystem *__restrict__ sys, uint32_t s,
+                                         const ForestBufs &fb, double G, GroupRule rule) {
+    ForestTree f;
+    f.S = sys[s];
+    f.h = fb.hdr + s;
+    const uint32_t fo = f.S.fb_off;
+    f.o = RecOut{G, 0, 1, nullptr, nullptr, fb.rec + f.S.node_off, fb.group_size + fo,
+                 fb.group_slot + fo, rule};
+    return f;
 }
 
-// One system's centre of mass, level by level from the deepest cells (every
-// thread of the workgroup calls it): the cells are bucketed by depth into
-// lvl (16-bit cell indices in LDS, cnt/cur: kMaxDepth + 1 and kMaxDepth LDS
-// words), then each level's cells are summed, with their children's group
-// marks, between two barriers.
-constexpr uint32_t kLvlLds = 32768;  // cells whose level lists fit in LDS (64 KB)
-__device__ __forceinline__ void com_levels_lds(uint32_t I, const uint8_t *__restrict__ int_depth,
-                                               uint16_t *lvl, uint32_t *cnt, uint32_t *cur,
-                                               const double4 *__restrict__ body4,
-                                               const int32_t *__restrict__ ref,
-                                               const uint32_t *__restrict__ int_first,
-                                               InternalSums *sums, uint32_t *counts, TreeHdr *h,
-                                               const RecOut &o) {
-    const uint32_t t = threadIdx.x, nt = blockDim.x;
-    for (uint32_t d = t; d <= (uint32_t)kMaxDepth; d += nt) cnt[d] = 0;
-    __syncthreads();
-    for (uint32_t k = t; k < I; k += nt)
-        atomicAdd(&cnt[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u);
-    __syncthreads();
-    if (t == 0) {
-        uint32_t run = 0;
-        for (int d = 0; d < kMaxDepth; ++d) {
-            const uint32_t c = cnt[d];
-            cnt[d] = run;
-            cur[d] = run;
-            run += c;
-        }
-        cnt[kMaxDepth] = run;
-    }
-    __syncthreads();
-    for (uint32_t k = t; k < I; k += nt)
-        lvl[atomicAdd(&cur[min((uint32_t)int_depth[k], (uint32_t)kMaxDepth - 1)], 1u)] =
-            (uint16_t)k;
-    __syncthreads();
-    for (int d = kMaxDepth - 1; d >= 0; --d) {
-        const uint32_t lb = cnt[d], le = cnt[d + 1];
-        if (lb == le) continue;  // the same for every thread: LDS, read after a barrier
-        for (uint32_t j = lb + t; j < le; j += nt)
-            sum_children_marks(lvl[j], body4, ref, int_first, sums, counts, h, o);
-        __syncthreads();
-    }
+// 8. Centre of mass, node records and group marks, as k_com_local and
+// k_com_span, for the blocks of every system in one launch each: block
+// (system, b) of fb.blk_tab; then (system, super-block) of fb.span_tab
+// (want 0), then the systems of fb.big_tab (want 2: cells crossing
+// super-block ends).
+__global__ void __launch_bounds__(kComLocalThreads) k_forest_com_local(
+    const EnsSystem *__restrict__ sys, ForestBufs fb, double G, GroupRule rule) {
+    const uint2 e = fb.blk_tab[blockIdx.x];
+    const ForestTree f = forest_tree(sys, e.x, fb, G, rule);
+    const EnsSystem &S = f.S;
+    const uint32_t fo = S.fb_off, io = S.int_off;
+    com_local_block(e.y, S.n, S.icap, f.h, fb.body4 + fo, fb.sidx + fo, fb.ref + S.node_off,
+                    fb.chain + fo, fb.int_depth + io, fb.int_local + io, fb.int_first + io,
+                    fb.int_lvl + io, fb.int_sums + io, fb.int_count + io, f.o);
 }
 
-// 8. Per system: centre of mass (every cell local), the root's record, the
-// traversal groups; then the system's group count, and its status flags
-// merged into *status.
-__global__ void __launch_bounds__(kForestThreads) k_forest_com(
-    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, double G,
-    ForestBufs fb, uint32_t *items, uint32_t *status, GroupRule rule) {
+__global__ void __launch_bounds__(kComThreads) k_forest_com_span(
+    const EnsSystem *__restrict__ sys, const uint2 *__restrict__ tab, uint32_t want, ForestBufs fb,
+    double G, GroupRule rule) {
+    const uint2 e = tab[blockIdx.x];
+    const ForestTree f = forest_tree(sys, e.x, fb, G, rule);
+    const EnsSystem &S = f.S;
+    const uint32_t fo = S.fb_off, io = S.int_off;
+    const uint32_t nblocks = (S.n + kComBlock - 1) / kComBlock;
+    com_span_block(e.y, nblocks, want == 2 ? nblocks : kComSuperBlocks, want, f.h,
+                   fb.body4 + fo, fb.ref + S.node_off, fb.spanning + S.span_off,
+                   fb.int_local + io, fb.int_first + io, fb.int_lvl + io, fb.int_sums + io,
+                   fb.int_count + io, fb.cross + S.cross_off,
+                   kMaxDepth * ((S.n + kComSuper - 1) / kComSuper + 1), f.o);
+}
+
+// 9. Per system: the traversal groups (scan of the group marks, then the
+// list), the group count, and the system's status flags merged into *status.
+__global__ void __launch_bounds__(kForestThreads) k_forest_groups(
+    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, ForestBufs fb,
+    uint32_t *items, uint32_t *status) {
     __shared__ uint32_t wsum[kForestWaves + 1];
-    __shared__ uint32_t lvl_n[kMaxDepth + 1];  // cells per depth, then their start offsets
-    __shared__ uint32_t lvl_c[kMaxDepth];      // fill cursors
-    __shared__ uint16_t lvl16[kLvlLds];        // cells by depth (systems of at most kLvlLds cells)
     const uint32_t s = order[blockIdx.x];
     const EnsSystem S = sys[s];
-    const uint32_t n = S.n, t = threadIdx.x;
+    const uint32_t n = S.n, t = threadIdx.x, fo = S.fb_off;
     TreeHdr *h = fb.hdr + s;
-    const uint32_t fo = S.fb_off;
     uint32_t *group_size = fb.group_size + fo, *group_slot = fb.group_slot + fo;
     uint2 *groups = fb.groups + fo;
-    const double4 *body4 = fb.body4 + fo;
-    const uint8_t *int_depth = fb.int_depth + S.int_off;
-    uint32_t *lvl = fb.int_lvl + S.int_off, *counts = fb.int_count + S.int_off,
-             *int_first = fb.int_first + S.int_off;
-    InternalSums *sums = fb.int_sums + S.int_off;
-    const int32_t *ref = fb.ref + S.node_off;
-    const uint32_t I = min(h->internal_count, S.icap);
-    const RecOut og{0.0, 0, 1, nullptr, nullptr, nullptr, group_size, group_slot, rule};
-    for (uint32_t i = t; i < n; i += kForestThreads) {
-        group_size[i] = 0;
-        group_slot[i] = 0;
-    }
-    __syncthreads();  // the marks below come after every clear
-    if (I <= kLvlLds) {  // group marks made with the sums
-        com_levels_lds(I, int_depth, lvl16, lvl_n, lvl_c, body4, ref, int_first, sums, counts, h,
-                       og);
-    } else {
-        com_levels(0u, I, [](uint32_t) { return true; }, int_depth, lvl, lvl_n, lvl_c,
-                   [&](uint32_t k) { sum_children(k, body4, ref, sums, counts, h); });
-        for (uint32_t k = t; k < I; k += kForestThreads)
-            group_mark_children(k, counts, ref, int_first, og);
-    }
-    if (t == 0) {
-        // the root's record (the other node records: k_forest_records) and
-        // group, if the root holds at most group_max bodies
-        record_at(0, h->edge, body4, ref, sums, int_depth, G, 0, 1, nullptr, nullptr,
-                  fb.rec + S.node_off);
-        if (n == 1 || counts[0] <= rule.group_max) put_group(og, 0, n);
-    }
-    __syncthreads();
     const uint32_t ng = wg_scan_array(group_slot, n, wsum);
     for (uint32_t i = t; i < n; i += kForestThreads) {
         const uint32_t gs = group_size[i];
@@ -1684,62 +1639,6 @@ __global__ void __launch_bounds__(kForestThreads) k_forest_com(
         // flags were raised with device-scope atomics: read the word the same way
         const uint32_t st = __hip_atomic_load(&h->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (st) atomicOr(status, st);
-    }
-}
-
-// 9. Exclusive scan, in system order, of the internal cell counts:
-// cell_off[k] = first cell of order[k] in the forest's cell sequence,
-// cell_off[nbh] = all cells.
-__global__ void __launch_bounds__(kForestThreads) k_forest_cell_offsets(
-    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, uint32_t nbh,
-    ForestBufs fb, uint32_t *cell_off) {
-    __shared__ uint32_t wsum[kForestWaves + 1];
-    for (uint32_t k = threadIdx.x; k < nbh; k += kForestThreads) {
-        const uint32_t s = order[k];
-        cell_off[k] = min(fb.hdr[s].internal_count, sys[s].icap);
-    }
-    __syncthreads();
-    const uint32_t total = wg_scan_array(cell_off, nbh, wsum);
-    if (threadIdx.x == 0) cell_off[nbh] = total;
-}
-
-// 10. Node records of the child slots of every internal cell of the forest,
-// consecutive threads on consecutive slots: slot u of the sequence is child
-// u % 8 of the forest's cell u / 8.  A block takes one contiguous range of the
-// sequence and finds its first system by bisection of cell_off.
-constexpr uint32_t kRecordBlocks = 2048;
-__global__ void __launch_bounds__(kThreads) k_forest_records(
-    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, uint32_t nbh,
-    double G, ForestBufs fb, const uint32_t *__restrict__ cell_off) {
-    __shared__ uint32_t k_first;
-    const uint64_t total = 8ull * cell_off[nbh];
-    const uint64_t per = ((total + gridDim.x - 1) / gridDim.x + kThreads - 1) / kThreads * kThreads;
-    const uint64_t u0 = (uint64_t)blockIdx.x * per;
-    const uint64_t u1 = min(total, u0 + per);
-    if (u0 >= u1) return;
-    if (threadIdx.x == 0) {  // cell_off[lo] * 8 <= u0 < cell_off[hi] * 8
-        uint32_t lo = 0, hi = nbh;
-        while (hi - lo > 1) {
-            const uint32_t mid = lo + (hi - lo) / 2;
-            if (8ull * cell_off[mid] <= u0) lo = mid; else hi = mid;
-        }
-        k_first = lo;
-    }
-    __syncthreads();
-    uint32_t k = k_first, s = 0xffffffffu;
-    EnsSystem S{};
-    double root_edge = 0;
-    for (uint64_t u = u0 + threadIdx.x; u < u1; u += kThreads) {
-        while (8ull * cell_off[k + 1] <= u) ++k;  // systems without cells are passed over
-        if (order[k] != s) {
-            s = order[k];
-            S = sys[s];
-            root_edge = fb.hdr[s].edge;
-        }
-        const uint64_t j = 1 + (u - 8ull * cell_off[k]);  // node slot in the system
-        record_at(j, root_edge, fb.body4 + S.fb_off, fb.ref + S.node_off,
-                  fb.int_sums + S.int_off, fb.int_depth + S.int_off, G, 0, 1, nullptr, nullptr,
-                  fb.rec + S.node_off);
     }
 }
 
@@ -1787,12 +1686,17 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     hipLaunchKernelGGL(k_forest_chain, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb);
     hipLaunchKernelGGL(k_forest_number, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, fb);
     hipLaunchKernelGGL(k_forest_link, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb);
-    hipLaunchKernelGGL(k_forest_com, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, G, fb,
-                       items, status, rule);
-    hipLaunchKernelGGL(k_forest_cell_offsets, dim3(1), dim3(kForestThreads), 0, s, sys, order, nbh,
-                       fb, fb.cell_off);
-    hipLaunchKernelGGL(k_forest_records, dim3(kRecordBlocks), dim3(kThreads), 0, s, sys, order,
-                       nbh, G, fb, (const uint32_t *)fb.cell_off);
+    if (fb.nblk)
+        hipLaunchKernelGGL(k_forest_com_local, dim3(fb.nblk), dim3(kComLocalThreads), 0, s, sys, fb,
+                           G, rule);
+    if (fb.nspan)
+        hipLaunchKernelGGL(k_forest_com_span, dim3(fb.nspan), dim3(kComThreads), 0, s, sys,
+                           (const uint2 *)fb.span_tab, 0u, fb, G, rule);
+    if (fb.nbig)
+        hipLaunchKernelGGL(k_forest_com_span, dim3(fb.nbig), dim3(kComThreads), 0, s, sys,
+                           (const uint2 *)fb.big_tab, 2u, fb, G, rule);
+    hipLaunchKernelGGL(k_forest_groups, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, fb,
+                       items, status);
 }
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
