@@ -326,10 +326,12 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &e->qorder, nsys)) || (rc = ealloc(e, &e->bh_order, nsys)) ||
         (rc = ealloc(e, &e->cursor, 1)) || (rc = ealloc(e, &e->status, 1)) ||
         (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.spanning, span)) || (rc = ealloc(e, &f.cross, xo)) || (rc = ealloc(e, &f.blk_tab, nblk)) || (rc = ealloc(e, &f.span_tab, nsup)) || (rc = ealloc(e, &f.big_tab, nsys)) ||
+        (rc = ealloc(e, &f.scan_part, nb::scan_scratch_words((uint32_t)fo) + 1ull)) ||
         (rc = ealloc(e, &f.radix_ctl, nb::radix_ctl_words((uint32_t)fo))) ||
         (rc = ealloc(e, &f.radix_ctl_sys, nb::radix_ctl_words((uint32_t)fo))))
         return bail(rc);
     f.slots = (uint32_t)fo;
+    f.scan_cap = nb::scan_scratch_words((uint32_t)fo) + 1;
     if (hipHostMalloc((void **)&e->status_host, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess)
         return bail(NB_E_HIP);
     *e->status_host = 0;

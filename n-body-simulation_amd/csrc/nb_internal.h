@@ -213,6 +213,10 @@ struct ForestBufs {
     // super-block; counts kept on the host
     uint2 *blk_tab, *span_tab, *big_tab;
     uint32_t nblk, nspan, nbig;
+    // the group scan over all slots: scan_scratch_words(slots) partials and
+    // the total in the last word
+    uint32_t *scan_part;
+    uint32_t scan_cap;
 };
 
 // ---- launchers (each .hip file defines the kernels it launches) -------------

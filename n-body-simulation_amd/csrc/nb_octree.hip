@@ -1479,38 +1479,16 @@ __global__ void __launch_bounds__(kThreads) k_forest_lcp(
            fb.klo + fo, fb.lcp + fo);
 }
 
-// 5. chain (internal cells owned by each pair) and each system's deepest leaf,
-// 1 + max lcp: a wave inside one system reduces first, and the device-scope
-// RMW runs only when it raises the stored value.
+// 5. chain: the internal cells owned by each pair.  (A system's deepest leaf
+// is not needed by the ensemble; the header's max_depth stays 0.)
 __global__ void __launch_bounds__(kThreads) k_forest_chain(uint32_t slots,
                                                            const uint16_t *__restrict__ fb_sys,
                                                            const EnsSystem *__restrict__ sys,
                                                            ForestBufs fb) {
     const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    uint32_t s = 0xffffffffu;
-    int d = -1;
-    if (j < slots) {
-        const SlotSys ss = slot_sys(j, fb_sys, sys);
-        s = ss.s;
-        if (ss.S.bh && ss.li + 1 < ss.S.n) {
-            const uint32_t fo = ss.S.fb_off;
-            chain_at(ss.li, fb.lcp + fo, fb.chain + fo);
-            d = fb.lcp[fo + ss.li];
-        }
-    }
-    const uint32_t s0 = __builtin_amdgcn_readfirstlane(s);
-    const bool one = __ballot(j < slots && s != s0) == 0;  // every lane in system s0
-    if (one) {
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) d = max(d, __shfl_xor(d, off, 64));
-        if ((threadIdx.x & 63u) != 0) d = -1;
-    }
-    if (d >= 0) {
-        TreeHdr *h = fb.hdr + s;
-        const uint32_t v = (uint32_t)d + 1;
-        if (v > __hip_atomic_load(&h->max_depth, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&h->max_depth, v);
-    }
+    if (j >= slots) return;
+    const SlotSys ss = slot_sys(j, fb_sys, sys);
+    if (ss.S.bh && ss.li + 1 < ss.S.n) chain_at(ss.li, fb.lcp + ss.S.fb_off, fb.chain + ss.S.fb_off);
 }
 
 // 6. Per system: cell numbering (exclusive scan of the chain), the internal
@@ -1615,27 +1593,32 @@ __global__ void __launch_bounds__(kComThreads) k_forest_com_span(
                    kMaxDepth * ((S.n + kComSuper - 1) / kComSuper + 1), f.o);
 }
 
-// 9. Per system: the traversal groups (scan of the group marks, then the
-// list), the group count, and the system's status flags merged into *status.
-__global__ void __launch_bounds__(kForestThreads) k_forest_groups(
-    const EnsSystem *__restrict__ sys, const uint32_t *__restrict__ order, ForestBufs fb,
-    uint32_t *items, uint32_t *status) {
-    __shared__ uint32_t wsum[kForestWaves + 1];
-    const uint32_t s = order[blockIdx.x];
-    const EnsSystem S = sys[s];
-    const uint32_t n = S.n, t = threadIdx.x, fo = S.fb_off;
-    TreeHdr *h = fb.hdr + s;
-    uint32_t *group_size = fb.group_size + fo, *group_slot = fb.group_slot + fo;
-    uint2 *groups = fb.groups + fo;
-    const uint32_t ng = wg_scan_array(group_slot, n, wsum);
-    for (uint32_t i = t; i < n; i += kForestThreads) {
-        const uint32_t gs = group_size[i];
-        if (gs) groups[group_slot[i]] = make_uint2(i, gs);
-    }
-    __syncthreads();
-    if (t == 0) {
+// 9. The traversal groups of every system: after one exclusive scan of the
+// group marks over all slots of the forest (padding included, whatever it
+// holds: a system's group numbers are differences of the scan, exact modulo
+// 2^32), each group start writes its group; a system's first slot writes its
+// group count and merges its status flags into *status.
+__global__ void __launch_bounds__(kThreads) k_forest_groups(uint32_t slots,
+                                                            const uint16_t *__restrict__ fb_sys,
+                                                            const EnsSystem *__restrict__ sys,
+                                                            ForestBufs fb,
+                                                            const uint32_t *__restrict__ total,
+                                                            uint32_t *items, uint32_t *status) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= slots) return;
+    const SlotSys ss = slot_sys(j, fb_sys, sys);
+    const EnsSystem &S = ss.S;
+    if (!S.bh || ss.li >= S.n) return;
+    const uint32_t fo = S.fb_off;
+    const uint32_t base = fb.group_slot[fo];
+    const uint32_t gs = fb.group_size[j];
+    if (gs) fb.groups[fo + (fb.group_slot[j] - base)] = make_uint2(ss.li, gs);
+    if (ss.li == 0) {
+        const uint32_t end = fo + S.n < slots ? fb.group_slot[fo + S.n] : *total;
+        const uint32_t ng = end - base;
+        TreeHdr *h = fb.hdr + ss.s;
         h->group_count = ng;
-        items[s] = ng;
+        items[ss.s] = ng;
         // flags were raised with device-scope atomics: read the word the same way
         const uint32_t st = __hip_atomic_load(&h->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (st) atomicOr(status, st);
@@ -1695,8 +1678,11 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
     if (fb.nbig)
         hipLaunchKernelGGL(k_forest_com_span, dim3(fb.nbig), dim3(kComThreads), 0, s, sys,
                            (const uint2 *)fb.big_tab, 2u, fb, G, rule);
-    hipLaunchKernelGGL(k_forest_groups, dim3(nbh), dim3(kForestThreads), 0, s, sys, order, fb,
-                       items, status);
+    uint32_t *total = fb.scan_part + (fb.scan_cap - 1);
+    scan_exclusive_u32(fb.group_slot, fb.group_slot, slots, total, fb.scan_part, fb.scan_cap - 1,
+                       s);
+    hipLaunchKernelGGL(k_forest_groups, dim3(blocks), dim3(kThreads), 0, s, slots, fs, sys, fb,
+                       (const uint32_t *)total, items, status);
 }
 
 void launch_aabb(const TreeBufs &t, const double *x, const double *y, const double *z,
