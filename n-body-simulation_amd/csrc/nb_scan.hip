@@ -15,10 +15,11 @@ constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kThreads * kScanItems;
-constexpr int kRadixTile = 4096;                       // keys per radix tile
+constexpr int kRadixTile = 4096;       // keys per radix tile
+constexpr int kRadixTileBig = 8192;    // keys per tile from kRadixBigMin keys on
+constexpr uint32_t kRadixBigMin = 1u << 21;
 constexpr int kPassThreads = 512;
 constexpr int kPassWaves = kPassThreads / 64;
-constexpr int kPassItems = kRadixTile / kPassThreads;  // keys per lane
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -242,14 +243,15 @@ __device__ inline bool look_back(const uint32_t *stat, uint32_t t, uint32_t d, u
     }
 }
 
-// One stable pass over digit (key >> 9 pass) & 511 of a tile of kRadixTile
-// keys.  Wave w holds the tile's keys [w * 512, (w + 1) * 512) and ranks them
-// 64 at a time against its own digit counters, with no barrier between
+// One stable pass over digit (key >> 9 pass) & 511 of a tile of kTile
+// keys.  Wave w holds the tile's keys [w * kTile / 8, (w + 1) * kTile / 8) and
+// ranks them 64 at a time against its own digit counters, with no barrier between
 // rounds: a key's place among its digit's keys in the tile is (same digit in
 // earlier waves) + (in earlier rounds of its wave) + (in lower lanes of its
 // round), so the order is stable.  The tile is reordered in LDS by digit and
 // written out digit run by digit run, so consecutive threads write
 // consecutive addresses.
+template <int kTile>
 __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     const uint64_t *__restrict__ keys, const uint32_t *__restrict__ vals,
     uint64_t *__restrict__ okeys, uint32_t *__restrict__ ovals, uint32_t n, int pass, int passes,
@@ -259,8 +261,8 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
     __shared__ uint32_t dbase[kBins];              // first tile position of digit d
     __shared__ uint32_t gbase[kBins];              // output position of the tile's first digit-d key
     __shared__ uint32_t wsum[2][kPassWaves];
-    __shared__ uint64_t lk[kRadixTile];
-    __shared__ uint32_t lv[kRadixTile];
+    __shared__ uint64_t lk[kTile];
+    __shared__ uint32_t lv[kTile];
     __shared__ uint32_t tile_id, stuck;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
     const bool owner = tid < kBins;  // thread tid owns digit tid
@@ -287,9 +289,10 @@ __global__ void __launch_bounds__(kPassThreads) k_radix_pass(
         ctl[prev * kBins + tid] = 0;
         if (tid == 0) ctl[kCtlTiles + prev] = 0;
     }
-    const uint64_t tile = (uint64_t)t * kRadixTile;
+    const uint64_t tile = (uint64_t)t * kTile;
     const uint64_t rest = (uint64_t)n - tile;  // > 0: one block per tile
-    const uint32_t tn = rest < (uint64_t)kRadixTile ? (uint32_t)rest : (uint32_t)kRadixTile;
+    const uint32_t tn = rest < (uint64_t)kTile ? (uint32_t)rest : (uint32_t)kTile;
+    constexpr int kPassItems = kTile / kPassThreads;  // keys per lane
     const uint32_t wbase = w * (kPassItems * 64u);
     uint64_t kk[kPassItems];
     uint32_t vv[kPassItems], dg[kPassItems], rk[kPassItems];
@@ -391,6 +394,7 @@ void scan_exclusive_u32(const uint32_t *in, uint32_t *out, uint32_t n, uint32_t 
                        (const uint32_t *)scratch);
 }
 
+// tiles of a sort of n keys at the smaller tile size (the most control words)
 uint32_t radix_blocks_for(uint32_t n) { return ceil_div(n ? n : 1, kRadixTile); }
 
 uint32_t radix_ctl_words(uint32_t n) { return kCtlStatus + 2u * kBins * radix_blocks_for(n); }
@@ -405,15 +409,24 @@ SortEnd radix_sort_u64_u32(uint64_t *keys, uint32_t *vals, uint64_t *keys_alt,
         return SortEnd::kRejected;
     }
     if (n <= 1) return SortEnd::kInPlace;
-    const uint32_t nb = radix_blocks_for(n);
+    // larger tiles for large sorts: half the look-back chain and longer digit
+    // runs in the scatter, once there are enough tiles to fill the device
+    const bool big = n >= kRadixBigMin;
+    const uint32_t nb = ceil_div(n, big ? kRadixTileBig : kRadixTile);
     const int passes = std::max((bits + kDigitBits - 1) / kDigitBits, kMinPasses);
     hipLaunchKernelGGL(k_radix_count, dim3(std::min(nb, kCountBlocks)), dim3(kCountThreads), 0, s,
                        (const uint64_t *)keys, n, ctl, ctl + kCtlStatus, (uint64_t)kBins * nb);
     uint64_t *ka = keys, *kb = keys_alt;
     uint32_t *va = vals, *vb = vals_alt;
     for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_radix_pass, dim3(nb), dim3(kPassThreads), 0, s, (const uint64_t *)ka,
-                           (const uint32_t *)va, kb, vb, n, p, passes, ctl, nb, flags);
+        if (big)
+            hipLaunchKernelGGL(k_radix_pass<kRadixTileBig>, dim3(nb), dim3(kPassThreads), 0, s,
+                               (const uint64_t *)ka, (const uint32_t *)va, kb, vb, n, p, passes,
+                               ctl, nb, flags);
+        else
+            hipLaunchKernelGGL(k_radix_pass<kRadixTile>, dim3(nb), dim3(kPassThreads), 0, s,
+                               (const uint64_t *)ka, (const uint32_t *)va, kb, vb, n, p, passes,
+                               ctl, nb, flags);
         uint64_t *tk = ka; ka = kb; kb = tk;
         uint32_t *tv = va; va = vb; vb = tv;
     }

@@ -245,7 +245,8 @@ def test_tree_parity_repeated_builds(gpu, oracle):
 
 
 def test_tree_sort_more_tiles_than_resident(gpu, oracle):
-    # 733 sort tiles of 4096 keys (the last one ragged): more tiles than the
+    # 367 sort tiles of 8192 keys (the last one ragged; sorts of 2^21 keys and
+    # more take the larger tiles, one block per CU): more tiles than the
     # device holds at once, so some look-back waits name tiles whose blocks
     # start later.  Two builds, positions moved in between.
     n = 3000017
