@@ -54,7 +54,13 @@ struct nb_ctx {
     hipEvent_t snap_staged = nullptr, snap_done = nullptr;
     bool snap_begun = false;
     std::vector<hipEvent_t> events;
-    std::vector<std::pair<int, size_t>> marks;  // (phase, index of start event)
+    struct Mark {
+        int phase;
+        size_t begin, end;  // event indices (end: kOpen until closed)
+    };
+    std::vector<Mark> marks;
+    std::vector<std::pair<int, bool>> pending;  // (phase, begin?) not recorded yet
+    size_t next_event = 0;
     double phase[NB_PHASE_COUNT] = {0};
     std::vector<void *> allocations;
     std::string err;
@@ -95,41 +101,62 @@ void dev_free(nb_ctx *c, void *p) {
 
 // Phase timing with HIP events on the context stream.  Phases may nest
 // (the octree phases inside "Octree creation"); an end closes the most
-// recent open mark of its phase.  The events only time: they are read after
-// the stream has been synchronised, so they skip the system-scope fence (a
-// cache write-back at every record, about 10 us of idle GPU each).
-void mark_begin(nb_ctx *c, int phase) {
-    const size_t idx = c->marks.size() * 2;
-    while (c->events.size() < idx + 2) {
-        hipEvent_t e;
-        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
-        c->events.push_back(e);
-    }
-    c->marks.push_back({phase, idx});
-    (void)hipEventRecord(c->events[idx], c->stream);
-}
+// recent open mark of its phase.  A mark is only requested by mark_begin /
+// mark_end: flush_marks, called before each launch and at the end of a call,
+// records one event for every mark requested since the last launch, so
+// phase boundaries that coincide share one event (each record idles the GPU
+// for a few microseconds).  The events only time: they are read after the
+// stream has been synchronised, so they skip the system-scope fence (a cache
+// write-back at every record, about 10 us of idle GPU each).
+constexpr size_t kOpen = ~size_t(0);
 
-void mark_end(nb_ctx *c, int phase) {
-    for (size_t k = c->marks.size(); k-- > 0;) {
-        if (c->marks[k].first == phase) {
-            (void)hipEventRecord(c->events[c->marks[k].second + 1], c->stream);
+void mark_begin(nb_ctx *c, int phase) { c->pending.push_back({phase, true}); }
+void mark_end(nb_ctx *c, int phase) { c->pending.push_back({phase, false}); }
+
+void flush_marks(nb_ctx *c) {
+    if (c->pending.empty()) return;
+    const size_t idx = c->next_event;
+    if (c->events.size() <= idx) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+            c->pending.clear();  // timing is lost, nothing else
             return;
         }
+        c->events.push_back(e);
     }
+    (void)hipEventRecord(c->events[idx], c->stream);
+    c->next_event = idx + 1;
+    for (const auto &pm : c->pending) {
+        if (pm.second) {
+            c->marks.push_back({pm.first, idx, kOpen});
+            continue;
+        }
+        for (size_t k = c->marks.size(); k-- > 0;) {
+            if (c->marks[k].phase == pm.first && c->marks[k].end == kOpen) {
+                c->marks[k].end = idx;
+                break;
+            }
+        }
+    }
+    c->pending.clear();
 }
 
 void timing_reset(nb_ctx *c) {
     c->marks.clear();
+    c->pending.clear();
+    c->next_event = 0;
     for (double &v : c->phase) v = 0;
 }
 
 void timing_collect(nb_ctx *c) {
-    for (auto &mk : c->marks) {
+    for (const auto &mk : c->marks) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->events[mk.second], c->events[mk.second + 1]) == hipSuccess)
-            c->phase[mk.first] += ms;
+        if (mk.end != kOpen &&
+            hipEventElapsedTime(&ms, c->events[mk.begin], c->events[mk.end]) == hipSuccess)
+            c->phase[mk.phase] += ms;
     }
     c->marks.clear();
+    c->next_event = 0;
 }
 
 int status_to_code(nb_ctx *c, uint32_t st) {
@@ -150,6 +177,7 @@ int status_to_code(nb_ctx *c, uint32_t st) {
 
 // Synchronise the stream, fetch the octree header and turn flags into a code.
 int finish(nb_ctx *c, bool read_tree_status) {
+    flush_marks(c);
     if (read_tree_status)
         NB_HIP(c, hipMemcpyAsync(c->hdr_host, c->tree.hdr, sizeof(TreeHdr), hipMemcpyDeviceToHost,
                                  c->stream));
@@ -282,24 +310,30 @@ nb::GroupRule group_rule(const nb_params &p) {
 int build_tree(nb_ctx *c) {
     TreeBufs &t = c->tree;
     mark_begin(c, NB_PHASE_AABB);
+    flush_marks(c);
     nb::launch_aabb(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_AABB);
     mark_begin(c, NB_PHASE_KEYS_SORT);
     mark_begin(c, NB_PHASE_KEYS);
+    flush_marks(c);
     nb::launch_keys(t, c->x, c->y, c->z, c->m, c->stream);
     mark_end(c, NB_PHASE_KEYS);
     mark_begin(c, NB_PHASE_SORT);
+    flush_marks(c);
     nb::launch_key_sort(t, c->stream);
     mark_end(c, NB_PHASE_SORT);
     mark_end(c, NB_PHASE_KEYS_SORT);
     mark_begin(c, NB_PHASE_TOPOLOGY);
+    flush_marks(c);
     nb::launch_topology(t, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_TOPOLOGY);
     mark_begin(c, NB_PHASE_COM);
     const bool fp64 = c->p.precision != NB_PREC_FP32C;  // FP64 and FP64_FAST
+    flush_marks(c);
     nb::launch_com(t, c->p.G, fp64, !fp64, group_rule(c->p), c->stream);
     c->quad_valid = false;
     if (!fp64 && (c->corr || c->p.quadrupole)) {
+        flush_marks(c);
         nb::launch_quadrupoles(t, c->x, c->y, c->z, c->m, c->quad, c->corr ? c->mcell : nullptr,
                                c->p.quadrupole ? c->gquad : nullptr, c->p.G, c->stream);
         c->quad_valid = true;
@@ -319,6 +353,7 @@ int ensure_quad_buffers(nb_ctx *c) {
 
 int bh_forces_enqueue(nb_ctx *c) {
     mark_begin(c, NB_PHASE_ACCEL);
+    flush_marks(c);
     if (c->p.precision != NB_PREC_FP32C) {  // FP64 and FP64_FAST: the reference traversal
         const uint32_t cap = exact_stack_cap(c);
         if (!c->exact_stack || c->exact_cap != cap) {
@@ -348,6 +383,7 @@ int bh_forces_enqueue(nb_ctx *c) {
 
 int direct_enqueue(nb_ctx *c) {
     mark_begin(c, NB_PHASE_ACCEL);
+    flush_marks(c);
     if (c->p.precision == NB_PREC_FP64) {
         nb::launch_direct_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, c->ax, c->ay, c->az,
                               c->stream);
@@ -648,10 +684,12 @@ int nb_forces_direct_targets(nb_ctx *c, uint32_t nt, const uint32_t *targets, do
     timing_reset(c);
     NB_HIP(c, hipMemcpyAsync(c->targets_dev, targets, (size_t)nt * 4, hipMemcpyHostToDevice, c->stream));
     mark_begin(c, NB_PHASE_ACCEL);
+    flush_marks(c);
     nb::launch_direct_targets_f64(c->n, c->x, c->y, c->z, c->m, c->p.G, c->p.eps2, nt,
                                   c->targets_dev, c->targets_out, c->targets_out + nt,
                                   c->targets_out + 2ull * nt, c->stream);
     mark_end(c, NB_PHASE_ACCEL);
+    flush_marks(c);  // before the copies
     double *dst[] = {ax, ay, az};
     for (int k = 0; k < 3; ++k)
         NB_HIP(c, hipMemcpyAsync(dst[k], c->targets_out + (uint64_t)k * nt, (size_t)nt * 8,
@@ -683,6 +721,7 @@ int nb_kick_drift(nb_ctx *c, double dt) {
     if (!c) return NB_E_INVALID_ARG;
     timing_reset(c);
     mark_begin(c, NB_PHASE_LF1);
+    flush_marks(c);
     nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx, c->vhy,
                           c->vhz, c->x, c->y, c->z, c->stream);
     mark_end(c, NB_PHASE_LF1);
@@ -694,6 +733,7 @@ int nb_kick(nb_ctx *c, double dt) {
     if (!c) return NB_E_INVALID_ARG;
     timing_reset(c);
     mark_begin(c, NB_PHASE_LF2);
+    flush_marks(c);
     nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy, c->vz,
                     c->stream);
     mark_end(c, NB_PHASE_LF2);
@@ -709,6 +749,7 @@ int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
     // (timed as leapfrog part 1); the last step's kick writes v.
     for (uint32_t s = 0; s < count; ++s) {
         mark_begin(c, NB_PHASE_LF1);
+        flush_marks(c);
         if (s == 0)
             nb::launch_kick_drift(c->n, dt, c->ax, c->ay, c->az, c->vx, c->vy, c->vz, c->vhx,
                                   c->vhy, c->vhz, c->x, c->y, c->z, c->stream);
@@ -729,6 +770,7 @@ int nb_steps(nb_ctx *c, int algorithm, double dt, uint32_t count) {
     }
     if (count > 0) {
         mark_begin(c, NB_PHASE_LF2);
+        flush_marks(c);
         nb::launch_kick(c->n, dt, c->ax, c->ay, c->az, c->vhx, c->vhy, c->vhz, c->vx, c->vy,
                         c->vz, c->stream);
         mark_end(c, NB_PHASE_LF2);
