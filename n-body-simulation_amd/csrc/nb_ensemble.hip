@@ -1,10 +1,11 @@
 // nb_ensemble.hip -- many independent systems advanced together.
 //
 // The systems' states are concatenated in one structure-of-arrays.  One
-// loop iteration for the whole ensemble is four launches, whatever the
-// number of systems:
+// loop iteration for the whole ensemble is a fixed sequence of launches,
+// whatever the number of systems:
 //   kick-drift (every body, its system's dt)
-//   octree build of every Barnes-Hut system (one workgroup per system)
+//   octree build of every Barnes-Hut system (the one-system build's stages,
+//           each over the slots or blocks of all systems at once)
 //   forces: one persistent launch over a work queue holding the traversal
 //           groups of the Barnes-Hut systems and 64-body direct-sum tiles of
 //           the small systems, largest systems first

@@ -261,9 +261,11 @@ void launch_com(const TreeBufs &t, double G, bool f64, bool f32, GroupRule rule,
 // hdr->max_depth of the last build (not maintained by the build itself)
 void launch_max_depth(const TreeBufs &t, hipStream_t s);
 
-// Octree of every Barnes-Hut system of an ensemble, one workgroup per system
-// (order: the BH systems' indices, largest first); items[s] receives the
-// system's traversal-group count.
+// Octree of every Barnes-Hut system of an ensemble: the one-system build's
+// stages, each over the slots or blocks of all systems (order: the BH
+// systems' indices, largest first, for the per-system stages; fb's block
+// tables list their blocks); items[s] receives the system's traversal-group
+// count.
 void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t nbh,
                          uint32_t nsys, const double *x, const double *y, const double *z,
                          const double *m, double G, const ForestBufs &fb, GroupRule rule,
