@@ -403,6 +403,24 @@ def test_simulation_loop_bitexact(gpu, oracle, algo):
         np.testing.assert_array_equal(bits(got), bits(r[k]), err_msg=k)
 
 
+def test_phase_times_are_nested_and_complete(gpu):
+    """Phase timing: coinciding phase boundaries share one event.  Every phase
+    of a Barnes-Hut run has a time, the build's sub-phases add up to at most
+    the octree phase that holds them, and the keys and sort phases to at most
+    "keys and sort"."""
+    d = make(gpu, "disk", 200000)
+    with ctx_for(gpu, d, theta=0.5, precision=gpu.PREC_FP32C) as c:
+        c.bh_build()
+        c.bh_forces()
+        c.steps(gpu.ALGO_BARNES_HUT, 1.0 / 24, 3)
+        ph = c.phase_ms()
+    for k, v in ph.items():
+        assert v > 0, (k, ph)
+    sub = ph["AABB creation"] + ph["keys and sort"] + ph["topology"] + ph["Compute center of mass"]
+    assert sub <= ph["Octree creation"] * 1.001 + 1e-3, ph
+    assert ph["keys"] + ph["sort"] <= ph["keys and sort"] * 1.001 + 1e-3, ph
+
+
 @pytest.mark.parametrize("algo", ["naive", "BarnesHut"])
 def test_snapshot_async_matches_sync_reads(gpu, oracle, algo):
     # nb_snapshot_begin/wait return the state as it was at begin, although
