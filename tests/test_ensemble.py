@@ -90,6 +90,34 @@ def test_ensemble_deep_trees_match_one_system(gpu):
 
 
 @pytest.mark.gpu
+def test_ensemble_large_systems_match_one_system(gpu):
+    """Systems of more than 64 blocks of 1,024 bodies, whose cells crossing
+    super-block ends take the centre-of-mass pass over the cross list, beside
+    a small one: canonical trees and forces bit-identical to the one-system
+    build."""
+    sizes = [150000, 3000, 70000]
+    parts, st, dt = _ensemble(gpu, sizes, 700)
+    with gpu.Ensemble(sizes, theta=0.6) as e:
+        e.set_tuning(bh_threshold=0, waves=4096)
+        e.set_state(st["x"], st["y"], st["z"], st["vx"], st["vy"], st["vz"], st["m"], dt)
+        e.forces()
+        ax, ay, az = e.accel()
+        cans = [e.tree_canonical(s) for s in range(len(sizes))]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    for s, d in enumerate(parts):
+        with _single(gpu, d, 0.6) as ctx:
+            ctx.bh_build()
+            one = ctx.tree_canonical()
+            ctx.bh_forces()
+            rx, ry, rz = ctx.accel()
+        for k in one:
+            assert np.array_equal(cans[s][k], one[k]), f"system {s}: {k}"
+        sl = slice(off[s], off[s + 1])
+        assert np.array_equal(ax[sl], rx) and np.array_equal(ay[sl], ry) and \
+            np.array_equal(az[sl], rz), f"system {s} (n={sizes[s]})"
+
+
+@pytest.mark.gpu
 def test_ensemble_many_systems_match_one_system(gpu):
     # 600 systems: the sort that gathers each system's slots takes 10 system
     # bits (5 radix passes, its own control words); two builds in a row reuse
