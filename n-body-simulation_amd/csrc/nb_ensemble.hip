@@ -326,7 +326,7 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         (rc = ealloc(e, &e->items, nsys)) || (rc = ealloc(e, &e->item_off, nsys + 1ull)) ||
         (rc = ealloc(e, &e->qorder, nsys)) || (rc = ealloc(e, &e->bh_order, nsys)) ||
         (rc = ealloc(e, &e->cursor, 1)) || (rc = ealloc(e, &e->status, 1)) ||
-        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.spanning, span)) || (rc = ealloc(e, &f.cross, xo)) || (rc = ealloc(e, &f.blk_tab, nblk)) || (rc = ealloc(e, &f.span_tab, nsup)) || (rc = ealloc(e, &f.big_tab, nsys)) ||
+        (rc = ealloc(e, &e->inter, total)) || (rc = ealloc(e, &f.fb_sys, fo)) || (rc = ealloc(e, &f.spanning, span)) || (rc = ealloc(e, &f.cross, xo)) || (rc = ealloc(e, &f.blk_tab, nblk)) || (rc = ealloc(e, &f.span_tab, nsup)) || (rc = ealloc(e, &f.big_tab, nsys)) || (rc = ealloc(e, &f.sys_first, nsys)) ||
         (rc = ealloc(e, &f.scan_part, nb::scan_scratch_words((uint32_t)fo) + 1ull)) ||
         (rc = ealloc(e, &f.radix_ctl, nb::radix_ctl_words((uint32_t)fo))) ||
         (rc = ealloc(e, &f.radix_ctl_sys, nb::radix_ctl_words((uint32_t)fo))))
@@ -349,7 +349,11 @@ int nb_ensemble_create(nb_ensemble **out, uint32_t nsys, const uint32_t *sizes,
         const uint64_t end = s + 1 < nsys ? e->sys_host[s + 1].fb_off : fo;
         std::fill(fs.begin() + e->sys_host[s].fb_off, fs.begin() + end, (uint16_t)s);
     }
+    std::vector<uint32_t> first(nsys);
+    for (uint32_t s = 0; s < nsys; ++s) first[s] = e->sys_host[s].fb_off;
     if (hipMemcpy(f.fb_sys, fs.data(), fo * sizeof(uint16_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(f.sys_first, first.data(), nsys * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
         hipMemset(f.radix_ctl, 0, (size_t)nb::radix_ctl_words((uint32_t)fo) * 4) != hipSuccess ||
         hipMemset(f.radix_ctl_sys, 0, (size_t)nb::radix_ctl_words((uint32_t)fo) * 4) != hipSuccess)
         return bail(NB_E_HIP);

@@ -205,6 +205,7 @@ struct ForestBufs {
     // (by key, then by system): a buffer's rotation assumes one pass count
     uint32_t slots;
     uint16_t *fb_sys;
+    uint32_t *sys_first;  // per system: its first slot (ascending)
     uint32_t *radix_ctl, *radix_ctl_sys;
     uint32_t *spanning, *cross;  // tables of every system (EnsSystem::span_off, cross_off)
     // the centre-of-mass launches over the Barnes-Hut systems: (system,

@@ -511,36 +511,6 @@ __device__ inline double4 body_term(const double4 p) {
     return make_double4(p.x * p.w, p.y * p.w, p.z * p.w, p.w);
 }
 
-__device__ inline void sum_children(uint32_t pk, const double4 *__restrict__ body4,
-                                    const int32_t *__restrict__ ref, InternalSums *sums,
-                                    uint32_t *counts, TreeHdr *h) {
-    const uint64_t base = 1ull + 8ull * pk;
-    int32_t r[8];
-    double4 v[8];
-    uint32_t c[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) r[d] = ref[base + d];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        v[d] = make_double4(0.0, 0.0, 0.0, 0.0);
-        c[d] = 0;
-        if (r[d] >= 0) {
-            const InternalSums s = sums[r[d]];
-            v[d] = make_double4(s.x, s.y, s.z, s.m);
-            c[d] = counts[r[d]];
-        } else if (r[d] != kRefEmpty) {
-            v[d] = body_term(body4[body_of_ref(r[d])]);
-            c[d] = 1;
-        }
-    }
-    InternalSums s;
-    uint32_t cnt;
-    add_children(r, v, c, s, cnt);
-    sums[pk] = s;
-    counts[pk] = cnt;
-    if (s.m == 0) raise_flag(h, kFlagZeroMass);
-}
-
 // Traversal records and groups, written by the centre-of-mass kernels: a cell,
 // once summed, writes the records of its 8 child slots (their data is at
 // hand) and, if it holds more than rule.group_max bodies, marks the traversal
@@ -645,7 +615,8 @@ __device__ inline void cell_outputs(const RecOut &o, uint32_t pk, uint32_t depth
     }
 }
 
-// The global-memory path: sum_children, then the children's records and groups.
+// The global-memory path: the cell's 8 children summed in digit order from 0.0,
+// then the children's records and groups.
 __device__ inline void sum_children_out(uint32_t pk, uint32_t depth, uint32_t first,
                                         double root_edge, const double4 *__restrict__ body4,
                                         const int32_t *__restrict__ ref, InternalSums *sums,
@@ -851,7 +822,7 @@ __device__ __forceinline__ void com_local_block(
             double sx = 0, sy = 0, sz = 0, sm = 0;
             uint32_t cn = 0;
             uint32_t pos = first_s[c];
-            // digit order from 0.0, as sum_children
+            // digit order from 0.0, as sum_children_out
 #pragma unroll
             for (int dd = 0; dd < 8; ++dd) {
                 const uint16_t code = code_s[8 * c + dd];
@@ -1174,22 +1145,6 @@ __global__ void __launch_bounds__(kComThreads) k_com_span(
 // spatially compact runs of the octree order.  Every sorted position starts
 // at most one group: mark the starts, scan, scatter -- the group list comes
 // out in octree order, so neighbouring groups share most of their walk.
-// The traversal groups among the children of internal cell k, from global
-// memory (the ensemble build); the same marks as the centre-of-mass kernels.
-__device__ inline void group_mark_children(uint32_t k, const uint32_t *__restrict__ counts,
-                                           const int32_t *__restrict__ ref,
-                                           const uint32_t *__restrict__ int_first,
-                                           const RecOut &o) {
-    if (counts[k] <= o.rule.group_max) return;
-    uint32_t m[8];
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-        const int32_t r = ref[1ull + 8ull * k + d];
-        m[d] = r == kRefEmpty ? 0u : r >= 0 ? counts[r] : 1u;
-    }
-    put_child_groups(o, int_first[k], m);
-}
-
 __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
                                                             const uint32_t *__restrict__ group_size,
                                                             const uint32_t *__restrict__ group_slot,
@@ -1200,64 +1155,7 @@ __global__ void __launch_bounds__(kThreads) k_group_scatter(uint32_t n,
     if (s) groups[group_slot[i]] = make_uint2(i, s);
 }
 
-// ---- 8. node records -----------------------------------------------------------
-// The FP32 traversal record of a node with centre of mass and mass c
-// (hi/lo split of the centre, G folded into the mass) and reference r.
-__device__ inline NodeF32 rec_f32(const double4 c, double G, int32_t r) {
-    NodeF32 o;
-    o.x = (float)c.x;
-    o.y = (float)c.y;
-    o.z = (float)c.z;
-    o.gm = (float)(G * c.w);
-    o.ref = r;
-    o.lx = (float)(c.x - (double)o.x);
-    o.ly = (float)(c.y - (double)o.y);
-    o.lz = (float)(c.z - (double)o.z);
-    return o;
-}
-
-// A body leaf's centre of mass as the reference forms it: numerator x m
-// divided by the mass at every visit (BarnesHutAlgorithm.cpp:351-353).
-__device__ inline double4 body_com(const double4 p) {
-    const double mb = p.w;
-    return make_double4((p.x * mb) / mb, (p.y * mb) / mb, (p.z * mb) / mb, mb);
-}
-
-__device__ inline void record_at(uint64_t j, double root_edge, const double4 *__restrict__ body4,
-                                 const int32_t *__restrict__ ref,
-                                 const InternalSums *__restrict__ sums,
-                                 const uint8_t *__restrict__ int_depth, double G, int write_f64,
-                                 int write_f32, double4 *cm, double *edge, NodeF32 *rec) {
-    const int32_t r = ref[j];
-    // both candidates are loaded unconditionally (an index that does not
-    // apply reads entry 0, which every system has), then one is used: a load
-    // under a branch would be waited for as soon as it is issued
-    const bool cell = r >= 0, body = r < kRefEmpty;
-    const uint32_t ci = cell ? (uint32_t)r : 0u;
-    const InternalSums s = sums[ci];
-    const int depth = (int)int_depth[ci];
-    const double4 p = body4[body ? body_of_ref(r) : 0u];
-    double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
-    double e = 0.0;
-    if (cell) {
-        // the reference divides numerator by mass at every visit
-        // (BarnesHutAlgorithm.cpp:351-353); the quotient is the same
-        c = make_double4(s.x / s.m, s.y / s.m, s.z / s.m, s.m);
-        e = ldexp(root_edge, -depth);
-    } else if (body) {
-        c = body_com(p);
-    }
-    if (write_f64) {
-        cm[j] = c;
-        edge[j] = e;
-    }
-    if (write_f32) rec[j] = rec_f32(c, G, r);
-}
-
-// ---- ensemble: one workgroup builds one system's octree ----------------------------
-// The same per-element steps as the one-system build, separated by workgroup
-// barriers instead of kernel boundaries; scans and the radix sort run inside
-// the workgroup, and every cell is local (one centre-of-mass pass).
+// ---- ensemble: the octrees of many systems ------------------------------------------
 constexpr int kForestThreads = 1024;
 constexpr int kForestWaves = kForestThreads / 64;
 // Per-wave contiguous range [*b, *e) of m items: 16 ranges of whole 64-item
@@ -1386,12 +1284,14 @@ __global__ void __launch_bounds__(kThreads) k_forest_keys(
     const uint32_t i = j - S.fb_off;
     uint64_t k = kPadKey;
     if (S.bh && i < S.n) {
-        const TreeHdr *h = fb.hdr + s;
+        TreeHdr *h = fb.hdr + s;
         Cell c{h->minx, h->miny, h->minz, h->edge};
         const uint32_t b = S.body_off + i;
-        const double px = gx[b], py = gy[b], pz = gz[b];
+        const double px = gx[b], py = gy[b], pz = gz[b], pm = gm[b];
         k = key_from(c, px, py, pz);
-        fb.body4[j] = make_double4(px, py, pz, gm[b]);
+        fb.body4[j] = make_double4(px, py, pz, pm);
+        fb.klo[j] = k;  // the full key by slot, for k_forest_rekey
+        if (pm == 0) raise_flag(h, kFlagZeroMass);  // an error of the reference's centre of mass
     }
     key[j] = k;
     val[j] = j;
@@ -1404,15 +1304,41 @@ constexpr int kForestLowBits = 36;
 
 // 3b. After the sort by the low key bits: each slot's key becomes (its system,
 // its high key bits), for the stable sort that gathers every system's slots
-// back into its own range, in key order.
-__global__ void __launch_bounds__(kThreads) k_forest_sysrank(uint32_t slots,
+// back into its own range, in key order.  A slot's system is the last whose
+// first slot is at or before it: up to kSysrankLds systems the first slots sit
+// in LDS and each slot's system is found by bisection there, instead of a
+// gathered load of fb_sys at a scattered slot.
+constexpr uint32_t kSysrankLds = 8192;
+constexpr uint32_t kSysrankBlocks = 2048;
+__global__ void __launch_bounds__(kThreads) k_forest_sysrank(uint32_t slots, uint32_t nsys,
+                                                             const uint32_t *__restrict__ first,
                                                              const uint16_t *__restrict__ fb_sys,
                                                              uint64_t *key,
                                                              const uint32_t *__restrict__ val) {
-    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j < slots)
-        key[j] = ((uint64_t)fb_sys[val[j]] << (3 * kKeyLevels - kForestLowBits)) |
-                 (key[j] >> kForestLowBits);
+    __shared__ uint32_t fl[kSysrankLds];
+    const bool lds = nsys <= kSysrankLds;
+    if (lds) {
+        for (uint32_t k = threadIdx.x; k < nsys; k += kThreads) fl[k] = first[k];
+        __syncthreads();
+    }
+    const uint32_t stride = gridDim.x * kThreads;
+    for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < slots; j += stride) {
+        const uint32_t v = val[j];
+        uint32_t sys;
+        if (lds) {
+            // fl[0] = 0 <= v: keep fl[lo] <= v while halving the range
+            uint32_t lo = 0, len = nsys;
+            while (len > 1) {
+                const uint32_t half = len >> 1;
+                if (fl[lo + half] <= v) lo += half;
+                len -= half;
+            }
+            sys = lo;
+        } else {
+            sys = fb_sys[v];
+        }
+        key[j] = ((uint64_t)sys << (3 * kKeyLevels - kForestLowBits)) | (key[j] >> kForestLowBits);
+    }
 }
 
 // After the forest-wide sort, the build alternates stages over every body
@@ -1436,11 +1362,11 @@ __device__ inline SlotSys slot_sys(uint32_t j, const uint16_t *__restrict__ fb_s
     return r;
 }
 
-// 3. Keys again, of the bodies in sorted order (the second sort kept only the
-// high key bits), and the system-local body indices.  `sorted` may be
-// fb.sidx itself: each slot is read, then rewritten, by one thread.  Each
-// body's mass is checked once (zero mass is an error of the reference's
-// centre of mass).
+// 3. The full keys of the bodies in sorted order (the second sort kept only
+// the high key bits; k_forest_keys left every slot's full key in klo, which
+// holds nothing else until the deep runs' second key words), and the
+// system-local body indices.  `sorted` may be fb.sidx itself: each slot is
+// read, then rewritten, by one thread.
 __global__ void __launch_bounds__(kThreads) k_forest_rekey(uint32_t slots,
                                                            const uint16_t *__restrict__ fb_sys,
                                                            const EnsSystem *__restrict__ sys,
@@ -1455,11 +1381,8 @@ __global__ void __launch_bounds__(kThreads) k_forest_rekey(uint32_t slots,
         raise_flag(h, kFlagSortWait);
         q = 0;
     }
-    Cell c{h->minx, h->miny, h->minz, h->edge};
-    const double4 p = fb.body4[ss.S.fb_off + q];  // one gathered line per body
-    fb.skey[j] = key_from(c, p.x, p.y, p.z);
+    fb.skey[j] = fb.klo[ss.S.fb_off + q];  // one gathered 8-byte key per body
     fb.sidx[j] = q;
-    if (p.w == 0) raise_flag(h, kFlagZeroMass);
 }
 
 // 4. lcp of each adjacent pair of a system's sorted bodies; a run of bodies
@@ -1649,8 +1572,9 @@ void launch_forest_build(const EnsSystem *sys, const uint32_t *order, uint32_t n
         std::swap(k, ka);
         std::swap(v, va);
     }
-    hipLaunchKernelGGL(k_forest_sysrank, dim3(blocks), dim3(kThreads), 0, s, slots,
-                       (const uint16_t *)fb.fb_sys, k, (const uint32_t *)v);
+    hipLaunchKernelGGL(k_forest_sysrank, dim3(std::min(blocks, kSysrankBlocks)), dim3(kThreads), 0,
+                       s, slots, nsys, (const uint32_t *)fb.sys_first, (const uint16_t *)fb.fb_sys,
+                       k, (const uint32_t *)v);
     int sys_bits = 1;
     while (sys_bits < 16 && (1u << sys_bits) < nsys) ++sys_bits;
     // its own control words: a control buffer must always see the same pass count
