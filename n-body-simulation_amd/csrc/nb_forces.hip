@@ -338,11 +338,44 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
             for (int q = 0; q < kSplitPerLane; ++q)
                 pair_f64f(sj, px[q], py[q], pz[q], eps2, ax[q], ay[q], az[q]);
         };
+        // Four sources x kSplitPerLane targets, stage by stage across the
+        // pairs: the compiler otherwise runs each pair's chain (the estimate,
+        // its five-step correction, the force) to the end before the next
+        // pair starts.  The same operations per pair as pair_f64f, and each
+        // target's sum still runs over the sources in order: bit-identical.
+        auto pairs4 = [&](int jj) {
+            constexpr int P = 4 * kSplitPerLane;
+            double dx[P], dy[P], dz[P], r2[P], yv[P], w[P];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double4 sj = tile[jj + u];
+#pragma unroll
+                for (int q = 0; q < kSplitPerLane; ++q) {
+                    const int p = u * kSplitPerLane + q;
+                    dx[p] = sj.x - px[q];
+                    dy[p] = sj.y - py[q];
+                    dz[p] = sj.z - pz[q];
+                    w[p] = sj.w;
+                    r2[p] = fma(dx[p], dx[p], fma(dy[p], dy[p], fma(dz[p], dz[p], eps2)));
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p) yv[p] = rsqrt_f64(r2[p]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int q = 0; q < kSplitPerLane; ++q) {
+                    const int p = u * kSplitPerLane + q;
+                    const double f = (w[p] * yv[p]) * (yv[p] * yv[p]);
+                    ax[q] = fma(f, dx[p], ax[q]);
+                    ay[q] = fma(f, dy[p], ay[q]);
+                    az[q] = fma(f, dz[p], az[q]);
+                }
+        };
         // the split's last stage may be partial: only its staged sources
         const uint32_t cnt = min((uint32_t)kSplitTile, j1 - k);  // uniform
         if (cnt == (uint32_t)kSplitTile) {
-#pragma unroll 4
-            for (int jj = 0; jj < kSplitTile; ++jj) pairs(jj);
+            for (int jj = 0; jj < kSplitTile; jj += 4) pairs4(jj);
         } else {
             for (uint32_t jj = 0; jj < cnt; ++jj) pairs((int)jj);
         }
