@@ -659,12 +659,56 @@ struct GroupLds {
     float4 *bcorr;   // correction: learned correction per body
 };
 
+// Bodies of the first of two passes over each flushed list, for a group of
+// s <= 64 bodies (s: one pass), and the list entries per flush.  A pass over
+// sa bodies gives each 64 / sa lanes, so an entry costs 1 / (64 / sa) trips;
+// the split minimising the trips of both passes is taken if it saves at least
+// 5 % of the one-pass trips (a second pass has its own fold and remainder).
+// 26 bodies: 21 with 3 lanes, then 5 with 12, 0.42 trips per entry instead
+// of 0.5 with 2 lanes each (52 of 64 lanes busy).  A flush takes a multiple
+// of both passes' lanes per body (60 for 3 and 12), so that only a group's
+// last flush has remainder trips.  Computed at compile time.
+struct TwoPass {
+    uint8_t first[kGroup + 1];
+    uint8_t chunk[kGroup + 1];
+};
+constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
+constexpr TwoPass make_two_pass() {
+    TwoPass t{};
+    for (uint32_t s = 1; s <= (uint32_t)kGroup; ++s) {
+        // compare sums of 1 / lanes-per-body exactly: a / b < c / d <=> a d < c b
+        const uint32_t one = (uint32_t)kGroup / s;  // one pass: 1 / one trips per entry
+        uint32_t best = s, bn = 1, bd = one;          // best cost bn / bd
+        for (uint32_t a = 1; a < s; ++a) {
+            const uint32_t l1 = (uint32_t)kGroup / a, l2 = (uint32_t)kGroup / (s - a);
+            const uint32_t cn = l1 + l2, cd = l1 * l2;  // 1 / l1 + 1 / l2
+            if (cn * bd < bn * cd && 100 * cn * one < 88 * cd) {
+                best = a;
+                bn = cn;
+                bd = cd;
+            }
+        }
+        t.first[s] = (uint8_t)best;
+        const uint32_t l1 = (uint32_t)kGroup / best;
+        const uint32_t l2 = best < s ? (uint32_t)kGroup / (s - best) : l1;
+        const uint32_t lcm = l1 / gcd_u32(l1, l2) * l2;
+        t.chunk[s] = (uint8_t)(lcm <= (uint32_t)kGroup ? lcm * ((uint32_t)kGroup / lcm)
+                                                         : (uint32_t)kGroup);
+    }
+    return t;
+}
+__constant__ constexpr TwoPass kTwoPassTable = make_two_pass();
+
 // Forces on the s bodies of one group starting at sorted position g0 (both
 // wave-uniform, checked by the caller).  NB = 1 (s <= 64): lane l holds body
 // l % s and list share l / s; the lpb = 64 / s lanes of a body share its list,
 // and the 64 - s * lpb lanes beyond repeat share 0 and their sums are dropped.
 // NB = 2 (64 < s <= 128): a lane holds bodies lane and lane + 64 and takes
-// the whole list for both.  *ncorr_out accumulates network evaluations.
+// the whole list for both.  Monopole walks of one body per lane split the
+// group into two passes over every flushed list (kTwoPassTable): bodies
+// [0, sa) as above with sa in place of s, then bodies [sa, s), lane l
+// holding body sa + l % (s - sa) and share l / (s - sa).  *ncorr_out
+// accumulates network evaluations.
 template <bool kCorr, int NB, bool kQuad>
 __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, uint32_t s,
                                              float eps2, const CorrArgs &corr,
@@ -689,14 +733,26 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
     uint32_t *qk = L.qk;
     const float4 *__restrict__ gquad = T.gquad;
     {
-        const uint32_t ss = NB == 2 ? (uint32_t)kGroup : s;  // body slots
+        // (quadrupole walks keep one pass: their second list is shared out
+        // by the first pass's mapping only)
+        constexpr bool kTwoPass = !kQuad && NB == 1;
+        // bodies of the first pass (all of them without a second one)
+        const uint32_t sa = kTwoPass && NB == 1 ? (uint32_t)kTwoPassTable.first[s] : s;
+        const uint32_t sb = s - sa;                             // second pass: 0 or more
+        const uint32_t ss = NB == 2 ? (uint32_t)kGroup : sa;  // body slots of the first pass
         const uint32_t sub = NB == 2 ? lane : lane % ss;      // which body of the group
         const uint32_t q = NB == 2 ? 0u : lane / ss;          // which share of the list
         const uint32_t lpb = (uint32_t)kGroup / ss;           // lanes per body
         const uint32_t qs = q < lpb ? q : 0u;                 // the share this lane computes
+        // the second pass: body sa + sub2 in share q2 of 64 / sb
+        const uint32_t sub2 = sb ? lane % sb : 0u;
+        const uint32_t q2 = sb ? lane / sb : 0u;
+        const uint32_t lpb2 = sb ? (uint32_t)kGroup / sb : 1u;
+        const uint32_t qs2 = q2 < lpb2 ? q2 : 0u;
         // list entries per flush: a multiple of lpb (60 to 64), so that only
-        // the group's last flush has a remainder trip
-        const uint32_t chunk = lpb * ((uint32_t)kGroup / lpb);
+        // the group's last flush has a remainder trip (of lpb2 too with two passes)
+        const uint32_t chunk =
+            sb ? (uint32_t)kTwoPassTable.chunk[s] : lpb * ((uint32_t)kGroup / lpb);
         const bool pow2 = (ss & (ss - 1)) == 0;
         bool valid[NB];
         uint32_t bi[NB];
@@ -704,16 +760,26 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const uint32_t k = sub + (uint32_t)b * kGroup;
-            valid[b] = k < s;
+            valid[b] = k < (NB == 2 ? s : sa);
             // an unused slot repeats a body of the group (its box stays the group's)
             bi[b] = sidx[g0 + (valid[b] ? k : (b == 0 ? 0u : sub))];
             px[b] = x[bi[b]];
             py[b] = y[bi[b]];
             pz[b] = z[bi[b]];
         }
+        // the second pass's body (every lane holds one of the sb bodies)
+        uint32_t bi2 = bi[0];
+        double px2 = px[0], py2 = py[0], pz2 = pz[0];
+        if (sb) {
+            bi2 = sidx[g0 + sa + sub2];
+            px2 = x[bi2];
+            py2 = y[bi2];
+            pz2 = z[bi2];
+        }
 
         // bounding box (FP64, exact min/max), centre and half extent
-        double mnx = px[0], mny = py[0], mnz = pz[0], mxx = px[0], mxy = py[0], mxz = pz[0];
+        double mnx = fmin(px[0], px2), mny = fmin(py[0], py2), mnz = fmin(pz[0], pz2);
+        double mxx = fmax(px[0], px2), mxy = fmax(py[0], py2), mxz = fmax(pz[0], pz2);
 #pragma unroll
         for (int b = 1; b < NB; ++b) {
             mnx = fmin(mnx, px[b]);
@@ -761,10 +827,27 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             ry[b] = r3[1];
             rz[b] = r3[2];
         }
+        float rx2 = rx[0], ry2 = ry[0], rz2 = rz[0];
+        if (sb) {
+            const double mb = m[bi2];
+            const double c3[3] = {(px2 * mb) / mb, (py2 * mb) / mb, (pz2 * mb) / mb};
+            const float gh[3] = {gxh, gyh, gzh}, gl[3] = {gxl, gyl, gzl};
+            float r3[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float hi = (float)c3[c];
+                const float lo = (float)(c3[c] - (double)hi);
+                r3[c] = (hi - gh[c]) + (lo - gl[c]);
+            }
+            rx2 = r3[0];
+            ry2 = r3[1];
+            rz2 = r3[2];
+        }
 
         double acc_x[NB], acc_y[NB], acc_z[NB];
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc_x[b] = acc_y[b] = acc_z[b] = 0.0;
+        double acc2_x = 0.0, acc2_y = 0.0, acc2_z = 0.0;  // the second pass's body
         uint32_t n_inter = 0;
         uint32_t sp = 0, ln = 0;  // wave-uniform stack and list sizes
         uint32_t cn = 0;          // wave-uniform size of the correction's cell list
@@ -778,6 +861,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 if (valid[b] && q == 0) boff[k] = make_float4(rx[b], ry[b], rz[b], 0.f);
                 bcorr[lane + (uint32_t)b * kGroup] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
+            if (kTwoPass && sb && q2 == 0) boff[sa + sub2] = make_float4(rx2, ry2, rz2, 0.f);
         }
 
         // Entries lst[base .. base + cnt) (cnt <= kGroup): lane share qs takes
@@ -799,6 +883,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 switch (lpb) {
                 case 1: flush_share<1, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
                 case 2: flush_share<2, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
+                case 3: flush_share<3, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
                 case 4: flush_share<4, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
                 case 8: flush_share<8, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
                 case 16: flush_share<16, 1>(p, full, rx, ry, rz, eps2, fx, fy, fz); break;
@@ -819,6 +904,28 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 acc_x[b] += (double)fx[b];
                 acc_y[b] += (double)fy[b];
                 acc_z[b] += (double)fz[b];
+            }
+            if (kTwoPass && sb) {  // the same entries for the second pass's bodies
+                const uint32_t full2 = cnt / lpb2;
+                const uint32_t rem2 = cnt - full2 * lpb2;
+                const float4 *p2 = lst + base + qs2;
+                float gx2[1] = {0.f}, gy2[1] = {0.f}, gz2[1] = {0.f};
+                const float qx[1] = {rx2}, qy[1] = {ry2}, qz[1] = {rz2};
+                switch (lpb2) {  // the second pass's common strides (16 or 21 bodies)
+                case 3: flush_share<3, 1>(p2, full2, qx, qy, qz, eps2, gx2, gy2, gz2); break;
+                case 4: flush_share<4, 1>(p2, full2, qx, qy, qz, eps2, gx2, gy2, gz2); break;
+                default:
+                    flush_share_rt(p2, lpb2, full2, rx2, ry2, rz2, eps2, gx2[0], gy2[0], gz2[0]);
+                    break;
+                }
+                if (rem2) {
+                    const float4 e = p2[full2 * lpb2];
+                    pair_f32(qs2 < rem2 ? e : far_entry(), rx2, ry2, rz2, eps2, gx2[0], gy2[0],
+                             gz2[0]);
+                }
+                acc2_x += (double)gx2[0];
+                acc2_y += (double)gy2[0];
+                acc2_z += (double)gz2[0];
             }
             n_inter += cnt;
         };
@@ -1097,6 +1204,19 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             acc_y[0] = ty;
             acc_z[0] = tz;
         }
+        if (kTwoPass && sb) {
+            // body sa + sub2's shares sit in lanes sub2 + j sb, j < lpb2; lane sub2 adds them
+            double tx = acc2_x, ty = acc2_y, tz = acc2_z;
+            for (uint32_t j = 1; j < lpb2; ++j) {
+                const int src = (int)(sub2 + j * sb);
+                tx += __shfl(acc2_x, src, 64);
+                ty += __shfl(acc2_y, src, 64);
+                tz += __shfl(acc2_z, src, 64);
+            }
+            acc2_x = tx;
+            acc2_y = ty;
+            acc2_z = tz;
+        }
         ncorr_out += ncorr;
         nquad_out += nquad;
         // Stored also after an overflow (the caller reports it and the values
@@ -1117,6 +1237,18 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 az[bi[b]] = acc_z[b];
                 if (inter) inter[bi[b]] = n_inter;
             }
+        }
+        if (kTwoPass && sb && q2 == 0) {  // lanes 0 .. sb - 1
+            if constexpr (kCorr) {
+                const float4 c = bcorr[sa + sub2];
+                acc2_x += (double)c.x;
+                acc2_y += (double)c.y;
+                acc2_z += (double)c.z;
+            }
+            ax[bi2] = acc2_x;
+            ay[bi2] = acc2_y;
+            az[bi2] = acc2_z;
+            if (inter) inter[bi2] = n_inter;
         }
     }
 }
@@ -1146,8 +1278,16 @@ __device__ inline bool group_ok(uint2 grp, uint32_t n) {
     return grp.y != 0 && grp.y <= kGroupLimit && grp.x < n && grp.y <= n - grp.x;
 }
 
+// At most 96 VGPRs, five waves per SIMD, for the monopole and quadrupole walks
+// (their two-pass flushes would otherwise take 103 and leave four; measured:
+// 3 % faster on the 4M Plummer sphere, a forced six waves spill and lose
+// 10 %).  The learned correction's walks hold the network's weights in
+// registers and stay at one wave: the compiler's note that it cannot meet the
+// target there is expected.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
 template <bool kCorr, bool kQuad>
-__global__ void __launch_bounds__(kGroup) k_bh_group(
+__global__ void __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(5))) k_bh_group(
     uint32_t n, const uint32_t *__restrict__ sidx, const double *__restrict__ x,
     const double *__restrict__ y, const double *__restrict__ z, const double *__restrict__ m,
     const NodeF32 *__restrict__ rec, const uint2 *__restrict__ groups, double theta,
@@ -1219,6 +1359,7 @@ __global__ void __launch_bounds__(kGroup) k_bh_group(
     if (overflow) raise_flag(h, kFlagStack);
     if (lane == 0) raise_max(&h->stack_peak, peak);
 }
+#pragma clang diagnostic pop
 
 // ---- ensemble of independent systems -------------------------------------------
 __global__ void __launch_bounds__(64) k_ens_items(uint32_t nq, const uint32_t *__restrict__ qorder,
