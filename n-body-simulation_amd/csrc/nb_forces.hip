@@ -663,9 +663,10 @@ struct GroupLds {
 // s <= 64 bodies (s: one pass), and the list entries per flush.  A pass over
 // sa bodies gives each 64 / sa lanes, so an entry costs 1 / (64 / sa) trips;
 // the split minimising the trips of both passes is taken if it saves at least
-// 5 % of the one-pass trips (a second pass has its own fold and remainder).
-// 26 bodies: 21 with 3 lanes, then 5 with 12, 0.42 trips per entry instead
-// of 0.5 with 2 lanes each (52 of 64 lanes busy).  A flush takes a multiple
+// 12 % of the one-pass trips (a second pass has its own set-up, reduction and
+// registers; 10 of the 32 sizes split).  26 bodies: 5 with 12 lanes, then 21
+// with 3, 0.42 trips per entry instead of 0.5 with 2 lanes each (52 of 64
+// lanes busy).  A flush takes a multiple
 // of both passes' lanes per body (60 for 3 and 12), so that only a group's
 // last flush has remainder trips.  Computed at compile time.
 struct TwoPass {
