@@ -467,7 +467,7 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
 // always taken exactly.  Each iteration pops up to 8 cells whose children
 // must be examined, and lane l examines child (l & 7) of cell l >> 3: one
 // 32-byte record load per lane.  Accepted cells and bodies are appended to an
-// LDS list; every 64 entries the list is evaluated against the group's bodies.
+// LDS list; every 128 entries the list is evaluated against the group's bodies.
 // A group of s bodies uses lpb = floor(64/s) lanes per body, each taking
 // every lpb-th list entry, so s * lpb of the 64 lanes do useful pairs (51 for
 // s = 17, where rounding s up to a power of two would leave 34).
@@ -482,7 +482,11 @@ constexpr int kStackCap = 1024;     // cells waiting to be opened (entry = k << 
 // per wave cost registers: on a 1M-body disk kPop 2 and 4 made the traversal
 // 7 % and 17 % slower than 1 (occupancy 6 -> 4 waves per SIMD).
 constexpr int kPop = 1;
-constexpr int kListCap = kGroup * (kPop + 1);
+// List entries evaluated per flush, at most.  Each flush folds its FP32 sums
+// into the FP64 accumulators and moves the list's tail to the front: 128
+// entries halve those costs against 64.
+constexpr int kFlushSpan = 2 * kGroup;
+constexpr int kListCap = kFlushSpan + kGroup * kPop;
 constexpr int kGroupBlocks = 8192;  // persistent waves
 constexpr uint32_t kXcds = 8;       // XCDs of an MI355X; workgroups are dealt round-robin
 constexpr uint32_t kQueueStride = 32;  // one 128-byte line per group-queue counter
@@ -673,6 +677,7 @@ struct TwoPass {
     uint8_t first[kGroup + 1];
     uint8_t chunk[kGroup + 1];
 };
+static_assert(kFlushSpan <= 255, "flush spans fit TwoPass::chunk");
 constexpr uint32_t gcd_u32(uint32_t a, uint32_t b) { return b ? gcd_u32(b, a % b) : a; }
 constexpr TwoPass make_two_pass() {
     TwoPass t{};
@@ -693,8 +698,8 @@ constexpr TwoPass make_two_pass() {
         const uint32_t l1 = (uint32_t)kGroup / best;
         const uint32_t l2 = best < s ? (uint32_t)kGroup / (s - best) : l1;
         const uint32_t lcm = l1 / gcd_u32(l1, l2) * l2;
-        t.chunk[s] = (uint8_t)(lcm <= (uint32_t)kGroup ? lcm * ((uint32_t)kGroup / lcm)
-                                                         : (uint32_t)kGroup);
+        t.chunk[s] = (uint8_t)(lcm <= (uint32_t)kFlushSpan ? lcm * ((uint32_t)kFlushSpan / lcm)
+                                                             : (uint32_t)kFlushSpan);
     }
     return t;
 }
@@ -750,10 +755,10 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
         const uint32_t q2 = sb ? lane / sb : 0u;
         const uint32_t lpb2 = sb ? (uint32_t)kGroup / sb : 1u;
         const uint32_t qs2 = q2 < lpb2 ? q2 : 0u;
-        // list entries per flush: a multiple of lpb (60 to 64), so that only
+        // list entries per flush: a multiple of lpb (124 to 128), so that only
         // the group's last flush has a remainder trip (of lpb2 too with two passes)
         const uint32_t chunk =
-            sb ? (uint32_t)kTwoPassTable.chunk[s] : lpb * ((uint32_t)kGroup / lpb);
+            sb ? (uint32_t)kTwoPassTable.chunk[s] : lpb * ((uint32_t)kFlushSpan / lpb);
         const bool pow2 = (ss & (ss - 1)) == 0;
         bool valid[NB];
         uint32_t bi[NB];
@@ -865,7 +870,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             if (kTwoPass && sb && q2 == 0) boff[sa + sub2] = make_float4(rx2, ry2, rz2, 0.f);
         }
 
-        // Entries lst[base .. base + cnt) (cnt <= kGroup): lane share qs takes
+        // Entries lst[base .. base + cnt) (cnt <= kFlushSpan): lane share qs takes
         // qs, qs + lpb, ...  Every share runs cnt / lpb trips; then shares
         // qs < cnt % lpb take one more entry and the others a zero-force
         // entry (a select, so the trip count stays uniform and nothing is
@@ -1127,7 +1132,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                     }
                     cn += __popcll(bc);  // < kCorrTile before: at most kCellCap - 1 after
                 }
-                ln += __popcll(bl);  // < chunk <= kGroup before: at most kListCap - 1 after
+                ln += __popcll(bl);  // < chunk <= kFlushSpan before: at most kListCap - 1 after
                 sp += __popcll(bo);
             }
             peak = sp > peak ? sp : peak;
@@ -1143,7 +1148,7 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 for (; ln - done >= chunk; done += chunk) flush(done, chunk);
                 __syncthreads();
                 const uint32_t rest = ln - done;  // < chunk <= done: no overlap
-                if (lane < rest) lst[lane] = lst[done + lane];
+                for (uint32_t i = lane; i < rest; i += kGroup) lst[i] = lst[done + i];
                 ln = rest;
                 __syncthreads();
             }
@@ -1153,9 +1158,9 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                     for (; qn - done >= chunk; done += chunk) qflush(done, chunk);
                     __syncthreads();
                     const uint32_t rest = qn - done;  // < chunk <= done: no overlap
-                    if (lane < rest) {
-                        qlst[lane] = qlst[done + lane];
-                        qk[lane] = qk[done + lane];
+                    for (uint32_t i = lane; i < rest; i += kGroup) {
+                        qlst[i] = qlst[done + i];
+                        qk[i] = qk[done + i];
                     }
                     qn = rest;
                     __syncthreads();
