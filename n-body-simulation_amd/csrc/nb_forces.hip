@@ -656,7 +656,6 @@ struct GroupLds {
     float4 *lst;
     float4 *qlst;    // quadrupole: accepted cells' entries
     uint32_t *qk;    // quadrupole: their internal-cell indices
-    float *thr;      // opening threshold by depth, (edge / theta)^2 = ro2_root * 4^-depth
     float4 *cl_e;    // correction: accepted cell offset from the group centre, G*M
     uint32_t *cl_k;  // correction: accepted cell's internal index
     float4 *boff;    // correction: body offsets from the group centre
@@ -733,7 +732,6 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
     uint32_t *inter = T.inter;
     uint32_t *stk = L.stk;
     float4 *lst = L.lst, *cl_e = L.cl_e, *boff = L.boff, *bcorr = L.bcorr;
-    float *thr = L.thr;
     uint32_t *cl_k = L.cl_k;
     float4 *qlst = L.qlst;
     uint32_t *qk = L.qk;
@@ -1021,16 +1019,15 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
             const float ey = fmaxf(fabsf(dy) - hy, 0.f);
             const float ez = fmaxf(fabsf(dz) - hz, 0.f);
             d2 = fmaf(ex, ex, fmaf(ey, ey, ez * ez));
-            const bool far_enough = d2 > thr[depth];
+            // (edge / theta)^2 = ro2_root 4^-depth, exact (a power-of-two
+            // scaling); computed rather than read from an LDS table, which
+            // put an LDS round trip on every walk step's chain of waits
+            const bool far_enough = d2 > ldexpf(ro2_root, -2 * depth);
             const bool cell = r.ref >= 0;
             to_list = r.ref != kRefEmpty && (!cell || far_enough);
             to_open = cell && !far_enough;
         };
 
-        // this tree's thresholds by depth (depths stay below 64), the same
-        // values as ldexpf at every test, one LDS read instead of three VALU
-        thr[lane] = ldexpf(ro2_root, -2 * (int)lane);
-        __syncthreads();
         {
             float4 e;
             float d2r;
@@ -1309,8 +1306,7 @@ __global__ void __launch_bounds__(kGroup) __attribute__((amdgpu_waves_per_eu(5))
     constexpr uint32_t kCorrSlots = kGroupMax > (uint32_t)kGroup ? kGroupMax : (uint32_t)kGroup;
     __shared__ float4 boff[kCorr ? kGroupLimit : 1];
     __shared__ float4 bcorr[kCorr ? kCorrSlots : 1];
-    __shared__ float thr[kGroup];
-    const GroupLds L{stk, lst, qlst, qk, thr, cl_e, cl_k, boff, bcorr};
+    const GroupLds L{stk, lst, qlst, qk, cl_e, cl_k, boff, bcorr};
     const uint32_t lane = threadIdx.x;
     const uint32_t ngroups = min(h->group_count, n);
     // (edge/theta)^2 of the root; a child at depth d uses ro2_root * 4^-d
@@ -1448,8 +1444,7 @@ __device__ inline void direct_tile(const double *__restrict__ x, const double *_
 __global__ void __launch_bounds__(kGroup) k_ens_forces(EnsForceArgs a) {
     __shared__ uint32_t stk[kStackCap];
     __shared__ float4 lst[kListCap];
-    __shared__ float thr[kGroup];
-    const GroupLds L{stk, lst, nullptr, nullptr, thr, nullptr, nullptr, nullptr, nullptr};
+    const GroupLds L{stk, lst, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     const uint32_t lane = threadIdx.x;
     const uint32_t total = a.item_off[a.nq];
     const MlpForward w{};
