@@ -69,12 +69,12 @@ CPU_SAMPLE_S = 10.0
 # HBM bytes per launch of the force kernel, keyed by (workload, theta), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script scaled by a device
 # copy of known size (FETCH x 2.0, WRITE x 1.0):
-#  - 4M Plummer, theta 0.365: 4,012,003 KiB x 2 read + 533,247 KiB written
+#  - 4M Plummer, theta 0.365: 4,227,497 KiB x 2 read + 531,251 KiB written
 #    (profiles/bh_plummer_4m_hbm_counters.md);
-#  - 1M disk, theta 0.5: 547,460 KiB x 2 read + 136,369 KiB written
+#  - 1M disk, theta 0.5: 548,369 KiB x 2 read + 134,486 KiB written
 #    (profiles/bh_disk_1m_hbm_counters.md).
 TRAFFIC_BYTES = {
-    ("bh_plummer_4m", 0.365): 8.763e9,
+    ("bh_plummer_4m", 0.365): 9.202e9,
     ("bh_disk_1m", 0.5): 1.261e9,
 }
 
