@@ -69,13 +69,13 @@ CPU_SAMPLE_S = 10.0
 # HBM bytes per launch of the force kernel, keyed by (workload, theta), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script scaled by a device
 # copy of known size (FETCH x 2.0, WRITE x 1.0):
-#  - 4M Plummer, theta 0.365: 4,227,497 KiB x 2 read + 531,251 KiB written
+#  - 4M Plummer, theta 0.37375: 4,928,224 KiB x 2 read + 528,254 KiB written
 #    (profiles/bh_plummer_4m_hbm_counters.md);
-#  - 1M disk, theta 0.5: 548,369 KiB x 2 read + 134,486 KiB written
+#  - 1M disk, theta 0.5: 749,624 KiB x 2 read + 133,460 KiB written
 #    (profiles/bh_disk_1m_hbm_counters.md).
 TRAFFIC_BYTES = {
-    ("bh_plummer_4m", 0.365): 9.202e9,
-    ("bh_disk_1m", 0.5): 1.261e9,
+    ("bh_plummer_4m", 0.37375): 10.634e9,
+    ("bh_disk_1m", 0.5): 1.672e9,
 }
 
 WORKLOADS = {
@@ -416,7 +416,7 @@ def main():
         "config": {"workload": args.workload, "bodies": n, "distribution": kind,
                    "algorithm": algorithm, "theta": theta if algorithm == "BarnesHut" else None,
                    "cells": "quadrupole" if args.quadrupole else "monopole",
-                   "group_max": ctx.params.group_max or 32, "pack_max": ctx.params.pack_max or 128,
+                   "group_max": ctx.params.group_max or 64, "pack_max": ctx.params.pack_max or 256,
                    "dt": dt, "median_rel_err": err, "err_sample": sample, "gate": ERR_GATE,
                    "gate_met": gate_met, "ungated_value": round(raw_value, 1),
                    "phase_ms_per_step": {k: round(v / args.steps, 4) for k, v in phases.items()},

@@ -83,7 +83,7 @@ typedef struct nb_params {
      * octree once per group of nearby bodies, a cell of at most group_max
      * bodies (1..64); sibling groups inside a cell of at most pack_max bodies
      * are packed into groups of up to group_max.  0 selects the defaults
-     * (32 and 128). */
+     * (64 and 256). */
     uint32_t group_max;
     uint32_t pack_max;
 } nb_params;
@@ -324,9 +324,9 @@ typedef struct nb_ensemble nb_ensemble;
 typedef struct nb_ensemble_tuning {
     uint32_t bh_threshold; /* systems with n >= bh_threshold use Barnes-Hut */
     uint32_t waves;        /* persistent wavefronts of the force launch */
-    uint32_t group_max;    /* bodies per Barnes-Hut traversal group, 1..64 (default 32) */
+    uint32_t group_max;    /* bodies per Barnes-Hut traversal group, 1..64 (default 64) */
     uint32_t pack_max;     /* sibling groups are packed inside cells of at most this many
-                              bodies, >= group_max (default 128) */
+                              bodies, >= group_max (default 256) */
     uint32_t direct_tile;  /* targets per direct-sum work item: 64 or 128 (default 64) */
 } nb_ensemble_tuning;
 

@@ -620,7 +620,7 @@ int nb_ensemble_autotune(nb_ensemble *e, uint32_t steps, nb_ensemble_tuning *bes
     }
     // traversal group size, with the packing limit at 4 groups
     if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.group_max = v; c.pack_max = 4 * v; },
-                    {16u, 24u, 48u, 64u})))
+                    {16u, 24u, 32u, 48u})))
         return rc;
     if ((rc = sweep([](nb_ensemble_tuning &c, uint32_t v) { c.pack_max = v * c.group_max; },
                     {1u, 2u, 8u})))

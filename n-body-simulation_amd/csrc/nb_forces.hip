@@ -667,7 +667,7 @@ struct GroupLds {
 // sa bodies gives each 64 / sa lanes, so an entry costs 1 / (64 / sa) trips;
 // the split minimising the trips of both passes is taken if it saves at least
 // 12 % of the one-pass trips (a second pass has its own set-up, reduction and
-// registers; 10 of the 32 sizes split).  26 bodies: 5 with 12 lanes, then 21
+// registers; 31 of the 64 sizes split).  26 bodies: 5 with 12 lanes, then 21
 // with 3, 0.42 trips per entry instead of 0.5 with 2 lanes each (52 of 64
 // lanes busy).  A flush takes a multiple
 // of both passes' lanes per body (60 for 3 and 12), so that only a group's

@@ -60,19 +60,24 @@ struct alignas(128) TreeHdr {
 // parent), and the sibling groups of a cell with at most kPackMax bodies are
 // packed, in digit order, into groups of at most kGroupMax bodies (fewer
 // walks for somewhat larger boxes).  A 64-lane wave walks one group; the
-// walk also supports up to 128 bodies (two per lane).  Measured traversal
-// times (same box, against 64 / 128):
-//  - one group per cell, no packing: 1M-body disk +16 %, 4M-body Plummer
-//    sphere +6 %;
-//  - 64 / 96 and 64 / 192: slower on both; 96 / 192 and 128 / 256: +26 % and
-//    +22 % on the disk;
-//  - 32 / 128 (this): disk -0.4 %, Plummer -2.2 %, with 8 % fewer interactions;
-//  - against 32 / 128: 24 / 96 disk +23 %, Plummer +9 %; 40 / 160 disk +13 %,
-//    Plummer +16 %; 32 / 96 disk +4 %, Plummer -0.2 %;
-//  - packing without a limit: the Plummer traversal ten times slower, because
-//    siblings around a dense non-group sibling make a box that contains it and
-//    the walk must resolve that cell to its leaves.
-constexpr uint32_t kGroupMax = 32;
+// walk also supports up to 128 bodies (two per lane).
+// Measured with the two-pass flush (groups whose size leaves lanes idle
+// evaluate each list in two passes), same box, against 32 / 128:
+//  - 64 / 256 (this): 4M-body Plummer sphere at its gated opening angle
+//    325.3 M against 304.6 M body-steps/s (theta 0.374 against 0.365: the
+//    larger boxes are stricter, so the error bound allows a larger theta),
+//    traversal 11.66 against 12.53 ms; 1M-body disk 0.977 against 1.022 ms;
+//  - 48 / 192: Plummer 316.8 M, disk 0.940 ms (the disk's best); 64 / 128:
+//    315.8 M, 0.966 ms; 40 / 160: 313.4 M, 0.967 ms; 64 / 384 and 64 / 512:
+//    Plummer 326.6 M and 326.5 M (the limit stays at 4 groups, away from the
+//    unbounded case below).
+// Before the two-pass flush, groups of 33 to 64 bodies left up to half the
+// lanes idle and 32 / 128 was the fastest (64 / 128 and 96 / 192 slower on
+// both inputs; one group per cell, no packing: disk +16 %, Plummer +6 %).
+// Packing without a limit made the Plummer traversal ten times slower:
+// siblings around a dense non-group sibling make a box that contains it, and
+// the walk must resolve that cell to its leaves.
+constexpr uint32_t kGroupMax = 64;
 constexpr uint32_t kPackMax = 4 * kGroupMax;
 // The largest group a walk takes (one body per lane of a 64-lane wave); the
 // ensemble's autotuner chooses group_max in [1, kGroupLimit].

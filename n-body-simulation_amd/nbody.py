@@ -599,7 +599,7 @@ class Ensemble:
         self._check(self.lib.nb_ensemble_get_tuning(self.h, ctypes.byref(t)), "nb_ensemble_get_tuning")
         return t.as_dict()
 
-    def set_tuning(self, bh_threshold, waves, group_max=32, pack_max=128, direct_tile=64):
+    def set_tuning(self, bh_threshold, waves, group_max=64, pack_max=256, direct_tile=64):
         t = EnsembleTuning(bh_threshold, waves, group_max, pack_max, direct_tile)
         self._check(self.lib.nb_ensemble_set_tuning(self.h, ctypes.byref(t)), "nb_ensemble_set_tuning")
 
