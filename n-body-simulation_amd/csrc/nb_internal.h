@@ -59,8 +59,9 @@ struct alignas(128) TreeHdr {
 // kGroupMax bodies whose parent holds more (or a lone body under such a
 // parent), and the sibling groups of a cell with at most kPackMax bodies are
 // packed, in digit order, into groups of at most kGroupMax bodies (fewer
-// walks for somewhat larger boxes).  A 64-lane wave walks one group; the
-// walk also supports up to 128 bodies (two per lane).
+// walks for somewhat larger boxes).  A 64-lane wave walks one group (the
+// walk's template also has a two-bodies-per-lane form for groups of up to
+// 128, compiled only if kGroupMax exceeds 64).
 // Measured with the two-pass flush (groups whose size leaves lanes idle
 // evaluate each list in two passes), same box, against 32 / 128:
 //  - 64 / 256 (this): 4M-body Plummer sphere at its gated opening angle
