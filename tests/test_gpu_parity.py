@@ -608,3 +608,27 @@ def test_bh_forces_repeatable(gpu):
         for u, v in zip(a1, a2):
             np.testing.assert_array_equal(bits(u), bits(v))
         assert np.all(np.isfinite(a1[0])) and np.abs(a1[0]).max() > 0
+
+
+@pytest.mark.parametrize("group_max", [1, 5, 11, 13, 17, 22, 23, 26, 33, 40, 49, 53, 64])
+def test_bh_group_passes_cover_every_pair(gpu, oracle, group_max):
+    # theta = 0 opens every cell, so each group's list is every body of the
+    # system and the traversal is an FP32 direct sum.  The group sizes span
+    # one pass (1, 5, 64) and the two-pass splits of the flush (11 -> 3 + 8,
+    # 13 -> 4 + 9, 17 -> 1 + 16, 22 -> 1 + 21, 40 -> 8 + 32, 53 -> 21 + 32, ...),
+    # so a lane mapping that drops or repeats a share of the list shows as a
+    # wrong interaction count or a force error far above FP32 rounding.
+    n = 3000
+    d = make(gpu, "cluster", n, seed=3)
+    with ctx_for(gpu, d, theta=0.0, precision=gpu.PREC_FP32C, group_max=group_max,
+                 pack_max=4 * group_max) as c:
+        c.bh_build()
+        ax, ay, az = c.bh_forces()
+        inter = c.interactions()
+    np.testing.assert_array_equal(inter, np.full(n, n, np.uint32))
+    rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"])
+    e = rel_err(ax, ay, az, rx, ry, rz)
+    # FP32 pairs summed in FP32 over at most 128 entries, folded into FP64; a
+    # dropped or repeated share of the list would put most bodies near 1e-1
+    assert np.median(e) < 1e-5, np.median(e)
+    assert np.quantile(e, 0.999) < 1e-3, np.quantile(e, 0.999)
