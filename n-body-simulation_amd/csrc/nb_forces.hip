@@ -915,7 +915,8 @@ __device__ __forceinline__ void group_walk_n(const GroupTree &T, uint32_t g0, ui
                 const float4 *p2 = lst + base + qs2;
                 float gx2[1] = {0.f}, gy2[1] = {0.f}, gz2[1] = {0.f};
                 const float qx[1] = {rx2}, qy[1] = {ry2}, qz[1] = {rz2};
-                switch (lpb2) {  // the second pass's common strides (16 or 21 bodies)
+                switch (lpb2) {  // the second pass's common strides (32, 21 or 16 bodies)
+                case 2: flush_share<2, 1>(p2, full2, qx, qy, qz, eps2, gx2, gy2, gz2); break;
                 case 3: flush_share<3, 1>(p2, full2, qx, qy, qz, eps2, gx2, gy2, gz2); break;
                 case 4: flush_share<4, 1>(p2, full2, qx, qy, qz, eps2, gx2, gy2, gz2); break;
                 default:
