@@ -494,14 +494,6 @@ static_assert(kXcds * kQueueStride <= TreeBufs::kGroupQueueWords, "queue counter
 
 // Every traversal starts with fresh queue cursors (a kernel, not a memset,
 // so the sequence can be captured in a graph).
-__global__ void k_group_queue_reset(TreeHdr *h, uint32_t *gq) {
-    if (threadIdx.x == 0) {
-        h->group_next = 0;
-        h->corr_pairs = 0;
-        h->quad_pairs = 0;
-    }
-    if (threadIdx.x < kXcds) gq[threadIdx.x * kQueueStride] = 0;
-}
 
 // The device-scope maximum with a read first: the read-modify-write (which
 // serialises at one memory channel when every wave does it) only when it
@@ -1282,6 +1274,69 @@ __device__ inline bool group_ok(uint2 grp, uint32_t n) {
     return grp.y != 0 && grp.y <= kGroupLimit && grp.x < n && grp.y <= n - grp.x;
 }
 
+// Before a traversal: reset the group queue, and order the last kTailGroups
+// groups of each XCD's chunk by decreasing estimated cost, its size times the
+// list length the previous traversal recorded for its first body.  The walks
+// that end the launch are then the shortest instead of whichever groups come
+// last in octree order (on a 1M-body disk the slowest waves ended 200 us after
+// the queue ran dry, a fifth of the launch).  Only the order in which groups
+// start changes: each group's forces are the same.  Without recorded lengths
+// (all zero) the order stays.  One workgroup per chunk.
+constexpr uint32_t kTailGroups = 2048;
+constexpr int kTailThreads = 1024;
+__global__ void __launch_bounds__(kTailThreads) k_group_queue_prepare(
+    TreeHdr *h, uint32_t *gq, uint32_t n, uint2 *groups, const uint32_t *__restrict__ sidx,
+    const uint32_t *__restrict__ inter) {
+    __shared__ unsigned long long key[kTailGroups];  // ~cost << 32 | position
+    __shared__ uint2 grp[kTailGroups];
+    const uint32_t t = threadIdx.x, c = blockIdx.x;  // c: chunk (and XCD) index
+    if (c == 0 && t == 0) {
+        h->group_next = 0;
+        h->corr_pairs = 0;
+        h->quad_pairs = 0;
+    }
+    if (t == 0) gq[c * kQueueStride] = 0;
+    if (!inter) return;
+    const uint32_t ngroups = min(h->group_count, n);  // the traversal's chunk bounds
+    const uint32_t lo = (uint32_t)((uint64_t)ngroups * c / kXcds);
+    const uint32_t hi = (uint32_t)((uint64_t)ngroups * (c + 1) / kXcds);
+    const uint32_t m = min(hi - lo, kTailGroups);
+    if (m < 2) return;  // the same for the whole workgroup
+    const uint32_t base = hi - m;
+    for (uint32_t i = t; i < kTailGroups; i += kTailThreads) {
+        unsigned long long k = ~0ull;  // past the tail: sorts last
+        if (i < m) {
+            const uint2 g = groups[base + i];
+            grp[i] = g;
+            uint32_t cost = 0;  // a group the walk would reject keeps cost 0
+            if (group_ok(g, n)) {
+                const uint64_t v = (uint64_t)g.y * inter[sidx[g.x]];
+                cost = v < 0xffffffffull ? (uint32_t)v : 0xffffffffu;
+            }
+            k = ((unsigned long long)~cost << 32) | i;
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    // bitonic sort, ascending: decreasing cost, ties in octree order
+    for (uint32_t k = 2; k <= kTailGroups; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = t; i < kTailGroups; i += kTailThreads) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = key[i], b = key[ixj];
+                    if ((a > b) == ((i & k) == 0)) {
+                        key[i] = b;
+                        key[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = t; i < m; i += kTailThreads) groups[base + i] = grp[(uint32_t)key[i]];
+}
+
 // At most 96 VGPRs, five waves per SIMD, for the monopole and quadrupole walks
 // (their two-pass flushes would otherwise take 103 and leave four; measured:
 // 3 % faster on the 4M Plummer sphere, a forced six waves spill and lose
@@ -1561,7 +1616,8 @@ void launch_bh_group(const TreeBufs &t, const double *x, const double *y, const 
                      const double *m, double eps2, double theta, const float4 *gquad, double *ax,
                      double *ay, double *az, uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
-    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
+    hipLaunchKernelGGL(k_group_queue_prepare, dim3(kXcds), dim3(kTailThreads), 0, s, t.hdr, t.gq,
+                       t.n, t.groups, (const uint32_t *)t.sidx, (const uint32_t *)interactions);
     if (gquad)
         hipLaunchKernelGGL((k_bh_group<false, true>), dim3(blocks), dim3(kGroup), 0, s, t.n,
                            (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
@@ -1579,7 +1635,8 @@ void launch_bh_group_corr(const TreeBufs &t, const double *x, const double *y, c
                           const float4 *gquad, double *ax, double *ay, double *az,
                           uint32_t *interactions, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>(kGroupBlocks, t.n);
-    hipLaunchKernelGGL(k_group_queue_reset, dim3(1), dim3(64), 0, s, t.hdr, t.gq);
+    hipLaunchKernelGGL(k_group_queue_prepare, dim3(kXcds), dim3(kTailThreads), 0, s, t.hdr, t.gq,
+                       t.n, t.groups, (const uint32_t *)t.sidx, (const uint32_t *)interactions);
     if (gquad)
         hipLaunchKernelGGL((k_bh_group<true, true>), dim3(blocks), dim3(kGroup), 0, s, t.n,
                            (const uint32_t *)t.sidx, x, y, z, m, (const NodeF32 *)t.rec,
