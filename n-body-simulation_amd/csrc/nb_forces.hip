@@ -1274,15 +1274,17 @@ __device__ inline bool group_ok(uint2 grp, uint32_t n) {
     return grp.y != 0 && grp.y <= kGroupLimit && grp.x < n && grp.y <= n - grp.x;
 }
 
-// Before a traversal: reset the group queue, and order the last kTailGroups
-// groups of each XCD's chunk by decreasing estimated cost, its size times the
-// list length the previous traversal recorded for its first body.  The walks
+// Before a traversal: reset the group queue, and order the last groups of
+// each XCD's chunk (kTailGroups, at most two thirds of the chunk: sorting all
+// of it scatters the walks that share L2 lines) by decreasing estimated cost,
+// a group's size times the list length the previous traversal recorded for
+// its first body.  The walks
 // that end the launch are then the shortest instead of whichever groups come
 // last in octree order (on a 1M-body disk the slowest waves ended 200 us after
 // the queue ran dry, a fifth of the launch).  Only the order in which groups
 // start changes: each group's forces are the same.  Without recorded lengths
 // (all zero) the order stays.  One workgroup per chunk.
-constexpr uint32_t kTailGroups = 2048;
+constexpr uint32_t kTailGroups = 4096;
 constexpr int kTailThreads = 1024;
 __global__ void __launch_bounds__(kTailThreads) k_group_queue_prepare(
     TreeHdr *h, uint32_t *gq, uint32_t n, uint2 *groups, const uint32_t *__restrict__ sidx,
@@ -1300,7 +1302,7 @@ __global__ void __launch_bounds__(kTailThreads) k_group_queue_prepare(
     const uint32_t ngroups = min(h->group_count, n);  // the traversal's chunk bounds
     const uint32_t lo = (uint32_t)((uint64_t)ngroups * c / kXcds);
     const uint32_t hi = (uint32_t)((uint64_t)ngroups * (c + 1) / kXcds);
-    const uint32_t m = min(hi - lo, kTailGroups);
+    const uint32_t m = min(2 * (hi - lo) / 3, kTailGroups);
     if (m < 2) return;  // the same for the whole workgroup
     const uint32_t base = hi - m;
     for (uint32_t i = t; i < kTailGroups; i += kTailThreads) {
