@@ -1278,19 +1278,29 @@ __device__ inline bool group_ok(uint2 grp, uint32_t n) {
 // each XCD's chunk (kTailGroups, at most two thirds of the chunk: sorting all
 // of it scatters the walks that share L2 lines) by decreasing estimated cost,
 // a group's size times the list length the previous traversal recorded for
-// its first body.  The walks
-// that end the launch are then the shortest instead of whichever groups come
-// last in octree order (on a 1M-body disk the slowest waves ended 200 us after
-// the queue ran dry, a fifth of the launch).  Only the order in which groups
-// start changes: each group's forces are the same.  Without recorded lengths
-// (all zero) the order stays.  One workgroup per chunk.
+// its first body.  The walks that end the launch are then the shortest
+// instead of whichever groups come last in octree order (on a 1M-body disk
+// the slowest waves ended 200 us after the queue ran dry, a fifth of the
+// launch).  A counting sort by cost class (8 classes per factor of two, 256
+// in all) in LDS; within a class the order is that of the LDS atomics.  Only
+// the order in which groups start changes: each group's forces are the same.
+// Without recorded lengths (all zero) the order stays.  One workgroup per
+// chunk.
 constexpr uint32_t kTailGroups = 4096;
 constexpr int kTailThreads = 1024;
+constexpr uint32_t kCostClasses = 256;
+__device__ inline uint32_t cost_class(uint32_t v) {  // decreasing in v; 0 -> last class
+    if (v == 0) return kCostClasses - 1;
+    const uint32_t e = 31u - (uint32_t)__clz(v);  // floor(log2 v)
+    const uint32_t f = e >= 3 ? (v >> (e - 3)) & 7u : (v << (3 - e)) & 7u;
+    return kCostClasses - 2 - min(8 * e + f, kCostClasses - 2);
+}
 __global__ void __launch_bounds__(kTailThreads) k_group_queue_prepare(
     TreeHdr *h, uint32_t *gq, uint32_t n, uint2 *groups, const uint32_t *__restrict__ sidx,
     const uint32_t *__restrict__ inter) {
-    __shared__ unsigned long long key[kTailGroups];  // ~cost << 32 | position
     __shared__ uint2 grp[kTailGroups];
+    __shared__ uint8_t cls[kTailGroups];
+    __shared__ uint32_t hist[kCostClasses], fill[kCostClasses];
     const uint32_t t = threadIdx.x, c = blockIdx.x;  // c: chunk (and XCD) index
     if (c == 0 && t == 0) {
         h->group_next = 0;
@@ -1305,38 +1315,35 @@ __global__ void __launch_bounds__(kTailThreads) k_group_queue_prepare(
     const uint32_t m = min(2 * (hi - lo) / 3, kTailGroups);
     if (m < 2) return;  // the same for the whole workgroup
     const uint32_t base = hi - m;
-    for (uint32_t i = t; i < kTailGroups; i += kTailThreads) {
-        unsigned long long k = ~0ull;  // past the tail: sorts last
-        if (i < m) {
-            const uint2 g = groups[base + i];
-            grp[i] = g;
-            uint32_t cost = 0;  // a group the walk would reject keeps cost 0
-            if (group_ok(g, n)) {
-                const uint64_t v = (uint64_t)g.y * inter[sidx[g.x]];
-                cost = v < 0xffffffffull ? (uint32_t)v : 0xffffffffu;
-            }
-            k = ((unsigned long long)~cost << 32) | i;
+    for (uint32_t b = t; b < kCostClasses; b += kTailThreads) hist[b] = fill[b] = 0;
+    __syncthreads();
+    for (uint32_t i = t; i < m; i += kTailThreads) {
+        const uint2 g = groups[base + i];
+        grp[i] = g;
+        uint32_t cost = 0;  // a group the walk would reject keeps cost 0
+        if (group_ok(g, n)) {
+            const uint64_t v = (uint64_t)g.y * inter[sidx[g.x]];
+            cost = v < 0xffffffffull ? (uint32_t)v : 0xffffffffu;
         }
-        key[i] = k;
+        const uint32_t k = cost_class(cost);
+        cls[i] = (uint8_t)k;
+        atomicAdd(&hist[k], 1u);
     }
     __syncthreads();
-    // bitonic sort, ascending: decreasing cost, ties in octree order
-    for (uint32_t k = 2; k <= kTailGroups; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = t; i < kTailGroups; i += kTailThreads) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long a = key[i], b = key[ixj];
-                    if ((a > b) == ((i & k) == 0)) {
-                        key[i] = b;
-                        key[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
+    if (hist[kCostClasses - 1] == m) return;  // no recorded lengths: keep the order
+    if (t == 0) {  // exclusive prefix of the classes
+        uint32_t run = 0;
+        for (uint32_t b = 0; b < kCostClasses; ++b) {
+            const uint32_t v = hist[b];
+            hist[b] = run;
+            run += v;
         }
     }
-    for (uint32_t i = t; i < m; i += kTailThreads) groups[base + i] = grp[(uint32_t)key[i]];
+    __syncthreads();
+    for (uint32_t i = t; i < m; i += kTailThreads) {
+        const uint32_t k = cls[i];
+        groups[base + hist[k] + atomicAdd(&fill[k], 1u)] = grp[i];
+    }
 }
 
 // At most 96 VGPRs, five waves per SIMD, for the monopole and quadrupole walks
