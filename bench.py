@@ -69,13 +69,13 @@ CPU_SAMPLE_S = 10.0
 # HBM bytes per launch of the force kernel, keyed by (workload, theta), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE runs of this script scaled by a device
 # copy of known size (FETCH x 2.0, WRITE x 1.0):
-#  - 4M Plummer, theta 0.37375: 4,928,224 KiB x 2 read + 528,254 KiB written
+#  - 4M Plummer, theta 0.37375: 5,775,771 KiB x 2 read + 528,162 KiB written
 #    (profiles/bh_plummer_4m_hbm_counters.md);
-#  - 1M disk, theta 0.5: 749,624 KiB x 2 read + 133,460 KiB written
+#  - 1M disk, theta 0.5: 932,786 KiB x 2 read + 133,202 KiB written
 #    (profiles/bh_disk_1m_hbm_counters.md).
 TRAFFIC_BYTES = {
-    ("bh_plummer_4m", 0.37375): 10.634e9,
-    ("bh_disk_1m", 0.5): 1.672e9,
+    ("bh_plummer_4m", 0.37375): 12.370e9,
+    ("bh_disk_1m", 0.5): 2.047e9,
 }
 
 WORKLOADS = {
