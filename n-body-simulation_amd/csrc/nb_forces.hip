@@ -389,6 +389,12 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 // walk with its own stack (here in global memory, interleaved across bodies
 // so that lanes at the same stack depth touch adjacent words); children are
 // pushed in digit order 5,7,4,6,1,3,0,2 as in BarnesHutAlgorithm.cpp:370-385.
+// The top of that stack, the children of the last opened cell, stays in
+// registers: the last pushed child is visited at once, the other seven are
+// taken from registers until a child is opened, and only then do the ones
+// left go to memory, below the new children.  The visit order, and so every
+// sum, is the reference's; a visit that pops a register saves a load that
+// waited on memory.
 constexpr int kExactThreads = 256;
 
 __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
@@ -406,19 +412,20 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
     const uint64_t stride = n;
     double acc_x = 0, acc_y = 0, acc_z = 0;
     uint32_t cnt = 0;
-    uint32_t top = 1;
-    stk[0] = 0;  // root
+    uint32_t top = 0;   // entries of the memory stack
+    uint32_t rb[7];     // the last opened cell's children still to visit, in push order
+    uint32_t rc = 0;    // how many; rb[rc - 1] is visited next
+    uint32_t node = 0;  // the root first
     // every node is visited at most once per body in a correct octree
     uint64_t budget = 1ull + 8ull * (uint64_t)icap;
-    while (top > 0) {
+    for (;;) {
         if (budget-- == 0) {
             raise_flag(h, kFlagStack);
             break;
         }
-        --top;
-        const uint32_t node = stk[(uint64_t)top * stride];
         const double4 c = cm[node];
         const int32_t r = ref[node];
+        bool opened = false;
         if (c.w != 0 && r != self_ref) {
             const double d_x = c.x - pos_x_i;
             const double d_y = c.y - pos_y_i;
@@ -434,19 +441,40 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
                 acc_z += c.w * (d_z * denominator);
                 ++cnt;
             } else {
-                if (top + 8 > cap) {
+                // the reference's stack would hold top + rc entries and 8 more
+                if (top + rc + 8 > cap) {
                     raise_flag(h, kFlagStack);
                     break;
                 }
+#pragma unroll
+                for (uint32_t q = 0; q < 7; ++q)  // the unvisited children go below
+                    if (q < rc) stk[(uint64_t)(top + q) * stride] = rb[q];
+                top += rc;
                 const uint32_t base = 1u + 8u * (uint32_t)r;
-                stk[(uint64_t)(top++) * stride] = base + 5;
-                stk[(uint64_t)(top++) * stride] = base + 7;
-                stk[(uint64_t)(top++) * stride] = base + 4;
-                stk[(uint64_t)(top++) * stride] = base + 6;
-                stk[(uint64_t)(top++) * stride] = base + 1;
-                stk[(uint64_t)(top++) * stride] = base + 3;
-                stk[(uint64_t)(top++) * stride] = base + 0;
-                stk[(uint64_t)(top++) * stride] = base + 2;
+                rb[0] = base + 5;
+                rb[1] = base + 7;
+                rb[2] = base + 4;
+                rb[3] = base + 6;
+                rb[4] = base + 1;
+                rb[5] = base + 3;
+                rb[6] = base + 0;
+                rc = 7;
+                node = base + 2;
+                opened = true;
+            }
+        }
+        if (!opened) {
+            if (rc > 0) {
+                --rc;
+                uint32_t v = rb[0];
+#pragma unroll
+                for (uint32_t q = 1; q < 7; ++q)
+                    if (q == rc) v = rb[q];
+                node = v;
+            } else if (top > 0) {
+                node = stk[(uint64_t)(--top) * stride];
+            } else {
+                break;
             }
         }
     }
