@@ -337,8 +337,13 @@ def test_tree_deep_runs(gpu, oracle):
 
 
 # ---- Barnes-Hut forces -------------------------------------------------------------------
-@pytest.mark.parametrize("kind,n,theta", [("cube", 2000, 0.5), ("cluster", 4000, 1.05),
-                                          ("disk", 20000, 0.5), ("plummer", 20000, 0.7)])
+# theta = 0 opens every cell, so the register batches of unvisited children
+# spill to the memory stack on every level; n = 1 and 2 walk a root leaf and
+# a root with two body leaves
+@pytest.mark.parametrize("kind,n,theta", [("cube", 1, 0.5), ("cube", 2, 0.5), ("cube", 2000, 0.5),
+                                          ("cube", 300, 0.0), ("cluster", 3000, 0.0),
+                                          ("cluster", 4000, 1.05), ("disk", 20000, 0.5),
+                                          ("plummer", 20000, 0.7)])
 def test_bh_exact_bitexact(gpu, oracle, kind, n, theta):
     d = make(gpu, kind, n, seed=7)
     with ctx_for(gpu, d, theta=theta) as c:
@@ -351,6 +356,29 @@ def test_bh_exact_bitexact(gpu, oracle, kind, n, theta):
     np.testing.assert_array_equal(inter, rint)
     for a, r in ((ax, rx), (ay, ry), (az, rz)):
         np.testing.assert_array_equal(bits(a), bits(r))
+
+
+def test_bh_exact_deep_runs(gpu, oracle):
+    # pairs closer than edge/2^21: each pair hangs under ~25 single-child
+    # cells, so the walk opens long chains whose pushed batches are all empty
+    rng = np.random.default_rng(5)
+    n = 600
+    base = rng.uniform(-1, 1, (n // 2, 3))
+    off = rng.normal(0, 1e-9, (n // 2, 3))
+    p = np.concatenate([base, base + off])
+    d = dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+             vy=np.zeros(n), vz=np.zeros(n), m=rng.uniform(1, 2, n))
+    for theta in (0.5, 0.0):
+        with ctx_for(gpu, d, storage_size_param=512, theta=theta) as c:
+            c.bh_build()
+            ax, ay, az = c.bh_forces()
+            inter = c.interactions()
+        t = oracle.Tree(n, 512 * n)
+        t.build(d["x"], d["y"], d["z"], d["m"])
+        rx, ry, rz, rint = t.bh_accel(theta)
+        np.testing.assert_array_equal(inter, rint)
+        for a, r in ((ax, rx), (ay, ry), (az, rz)):
+            np.testing.assert_array_equal(bits(a), bits(r))
 
 
 @pytest.mark.parametrize("kind,n,theta", [("disk", 30000, 0.5), ("plummer", 30000, 0.5),
