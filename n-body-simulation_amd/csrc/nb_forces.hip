@@ -415,7 +415,10 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 // batch down one and the bottom one to memory.  The visit order, and so every
 // sum, is the reference's; a visit that pops a register saves a load that
 // waited on memory (1M-body disk at theta 0.5: 27.8 ms with the whole stack
-// in memory, 19.3 with one batch, 17.5 with two, 16.9 with six).
+// in memory, 19.3 with one batch, 17.5 with two, 16.9 with six; then with
+// the opening test decided from an estimate and a node's three loads issued
+// together, 15.8 with six, 15.2 with four at 98 VGPRs, 15.9 with three and
+// 16.1 with two, although three and two leave room for five and six waves).
 constexpr int kExactThreads = 256;
 
 __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
@@ -433,7 +436,13 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
     const uint64_t stride = n;
     double acc_x = 0, acc_y = 0, acc_z = 0;
     uint32_t cnt = 0;
-    constexpr int K = 6;  // register batches (104 VGPRs)
+    // The opening test edge * (1.0 / sqrt(d2)) < theta is decided from
+    // edge * rsqrt_f64(d2), within 1e-14 of it, wherever that estimate is
+    // farther than 2^-30 (relative) from theta; only the rest pay the IEEE
+    // square root and division.  The decisions, and so the visit order and
+    // every sum, are the reference's.
+    const double theta_lo = theta * (1.0 - 0x1p-30), theta_hi = theta * (1.0 + 0x1p-30);
+    constexpr int K = 4;  // register batches (98 VGPRs)
     uint32_t top = 0;     // entries of the memory stack
     uint32_t rb[K][7];    // rb[0]: the last opened cell's unvisited children, in push order
     uint32_t rc[K];       // their counts; rb[k][rc[k] - 1] is the top of batch k
@@ -448,14 +457,29 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
         }
         const double4 c = cm[node];
         const int32_t r = ref[node];
+        const double e = edge[node];  // every slot below mcap has one, read or not
+        // all three loads in flight together: left to itself the compiler
+        // issues the centre of mass and the edge only after the tests on the
+        // mass and the reference, a second memory round trip per visit
+        asm volatile("" ::"v"(c.x), "v"(c.y), "v"(c.z), "v"(c.w), "v"(r), "v"(e));
         bool opened = false;
         if (c.w != 0 && r != self_ref) {
             const double d_x = c.x - pos_x_i;
             const double d_y = c.y - pos_y_i;
             const double d_z = c.z - pos_z_i;
-            const double d = 1.0 / sqrt(d_x * d_x + d_y * d_y + d_z * d_z);
-            const double currentTheta = edge[node] * d;
-            if ((currentTheta < theta) || r < 0) {
+            bool accept = r < 0;
+            if (!accept) {
+                const double d2 = d_x * d_x + d_y * d_y + d_z * d_z;
+                const double est = e * rsqrt_f64(d2);
+                if (est < theta_lo) {
+                    accept = true;
+                } else if (!(est > theta_hi)) {  // near theta, or not finite
+                    const double d = 1.0 / sqrt(d2);
+                    const double currentTheta = e * d;
+                    accept = currentTheta < theta;
+                }
+            }
+            if (accept) {
                 double denominator = (d_x * d_x) + (d_y * d_y) + (d_z * d_z) + eps2;
                 denominator = denominator * denominator * denominator;
                 denominator = 1.0 / sqrt(denominator);
