@@ -406,19 +406,17 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 
 // ---------------------------------------------------------------------------
 // Reference-semantics Barnes-Hut traversal (FP64).  Per body a depth-first
-// walk with its own stack (here in global memory, interleaved across bodies
-// so that lanes at the same stack depth touch adjacent words); children are
-// pushed in digit order 5,7,4,6,1,3,0,2 as in BarnesHutAlgorithm.cpp:370-385.
-// The top of that stack stays in registers, as the unvisited children of the
-// last K opened cells (batches of up to 7: the last pushed child is visited
-// at once).  Opening a cell while the top batch holds children moves every
-// batch down one and the bottom one to memory.  The visit order, and so every
-// sum, is the reference's; a visit that pops a register saves a load that
-// waited on memory (1M-body disk at theta 0.5: 27.8 ms with the whole stack
-// in memory, 19.3 with one batch, 17.5 with two, 16.9 with six; then with
-// the opening test decided from an estimate and a node's three loads issued
-// together, 15.8 with six, 15.2 with four at 98 VGPRs, 15.9 with three and
-// 16.1 with two, although three and two leave room for five and six waves).
+// walk with its own stack (the older part in global memory, interleaved
+// across bodies so that lanes at the same stack depth touch adjacent words);
+// children are pushed in digit order 5,7,4,6,1,3,0,2 as in
+// BarnesHutAlgorithm.cpp:370-385, and the last pushed is visited at once.
+// The stack is kept as batches, one per opened cell (below).  The visit
+// order, and so every sum, is the reference's.  1M-body disk at theta 0.5:
+// 27.8 ms with one stack entry per child in memory; 16.9 ms with the top six
+// batches as seven registers each; 15.2 ms with four such batches, the
+// opening test decided from an estimate and a node's three loads issued
+// together; with a batch as one word, 14.4, 13.3, 12.5, 12.5 and 12.7 ms
+// with 2, 4, 8, 12 and 16 batches in registers.
 constexpr int kExactThreads = 256;
 
 __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
@@ -442,12 +440,19 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
     // square root and division.  The decisions, and so the visit order and
     // every sum, are the reference's.
     const double theta_lo = theta * (1.0 - 0x1p-30), theta_hi = theta * (1.0 + 0x1p-30);
-    constexpr int K = 4;  // register batches (98 VGPRs)
-    uint32_t top = 0;     // entries of the memory stack
-    uint32_t rb[K][7];    // rb[0]: the last opened cell's unvisited children, in push order
-    uint32_t rc[K];       // their counts; rb[k][rc[k] - 1] is the top of batch k
+    // The stack holds batches: the unvisited children of one opened cell r,
+    // pushed in digit order 5,7,4,6,1,3,0 and popped from the end, as one
+    // word (r << 3) | count (cell indices stay below 2^26).  The top kBatches
+    // batches live in registers, bw[0] on top, with no empty batch above a
+    // full one; older batches go to the memory stack one word each.  Popping
+    // a child is a shift of a constant; a memory pop returns a whole batch.
+    constexpr int kBatches = 8;  // 66 VGPRs
+    constexpr uint32_t kPopDigit = 0x0316475u;  // nibble c-1: the child popped at count c
+    uint32_t bw[kBatches];
 #pragma unroll
-    for (int k = 0; k < K; ++k) rc[k] = 0;
+    for (int k = 0; k < kBatches; ++k) bw[k] = 0;
+    uint32_t mtop = 0;      // batches in memory
+    uint32_t mentries = 0;  // children they hold
     uint32_t node = 0;  // the root first
     uint64_t budget = 1ull + 8ull * (uint64_t)icap;
     for (;;) {
@@ -490,54 +495,39 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
             } else {
                 uint32_t held = 0;
 #pragma unroll
-                for (int k = 0; k < K; ++k) held += rc[k];
-                if (top + held + 8 > cap) {
+                for (int k = 0; k < kBatches; ++k) held += bw[k] & 7u;
+                if (mentries + held + 8 > cap) {
                     raise_flag(h, kFlagStack);
                     break;
                 }
-                if (rc[0]) {
+                if (bw[0]) {
                     // the bottom batch goes to memory, the others down one
-#pragma unroll
-                    for (uint32_t q = 0; q < 7; ++q)
-                        if (q < rc[K - 1]) stk[(uint64_t)(top + q) * stride] = rb[K - 1][q];
-                    top += rc[K - 1];
-#pragma unroll
-                    for (int k = K - 1; k > 0; --k) {
-#pragma unroll
-                        for (int q = 0; q < 7; ++q) rb[k][q] = rb[k - 1][q];
-                        rc[k] = rc[k - 1];
+                    if (bw[kBatches - 1]) {
+                        stk[(uint64_t)(mtop++) * stride] = bw[kBatches - 1];
+                        mentries += bw[kBatches - 1] & 7u;
                     }
+#pragma unroll
+                    for (int k = kBatches - 1; k > 0; --k) bw[k] = bw[k - 1];
                 }
-                const uint32_t base = 1u + 8u * (uint32_t)r;
-                rb[0][0] = base + 5;
-                rb[0][1] = base + 7;
-                rb[0][2] = base + 4;
-                rb[0][3] = base + 6;
-                rb[0][4] = base + 1;
-                rb[0][5] = base + 3;
-                rb[0][6] = base + 0;
-                rc[0] = 7;
-                node = base + 2;
+                bw[0] = ((uint32_t)r << 3) | 7u;
+                node = 8u * (uint32_t)r + 3u;  // child 2, pushed last
                 opened = true;
             }
         }
         if (!opened) {
-            bool got = false;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                if (!got && rc[k] > 0) {
-                    const uint32_t t = --rc[k];
-                    uint32_t v = rb[k][0];
-#pragma unroll
-                    for (uint32_t q = 1; q < 7; ++q)
-                        if (q == t) v = rb[k][q];
-                    node = v;
-                    got = true;
-                }
+            if (bw[0] == 0) {  // every register batch is empty
+                if (mtop == 0) break;
+                bw[0] = stk[(uint64_t)(--mtop) * stride];
+                mentries -= bw[0] & 7u;
             }
-            if (!got) {
-                if (top == 0) break;
-                node = stk[(uint64_t)(--top) * stride];
+            const uint32_t c = bw[0] & 7u;
+            node = (bw[0] & ~7u) + 1u + ((kPopDigit >> (4u * (c - 1u))) & 7u);
+            if (c == 1) {  // batch 0 is done: the others up one
+#pragma unroll
+                for (int k = 0; k < kBatches - 1; ++k) bw[k] = bw[k + 1];
+                bw[kBatches - 1] = 0;
+            } else {
+                bw[0] -= 1u;
             }
         }
     }
