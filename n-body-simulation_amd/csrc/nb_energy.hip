@@ -47,8 +47,7 @@ __global__ void __launch_bounds__(kThreads) k_energy_terms(
                 const double r_x = pxj - si.x;
                 const double r_y = pyj - si.y;
                 const double r_z = pzj - si.z;
-                const double r = sqrt(r_x * r_x + r_y * r_y + r_z * r_z);
-                ep += G * si.w * mj / r;
+                ep += div_sqrt_ieee(G * si.w * mj, r_x * r_x + r_y * r_y + r_z * r_z);
             }
         }
         __syncthreads();

@@ -24,39 +24,6 @@ namespace {
 __device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status, f); }
 
 // ---------------------------------------------------------------------------
-// 1.0 / sqrt(v) with both operations correctly rounded, as the reference
-// evaluates its inverse square root.  For v in [2^-512, 2^512] this runs the
-// same operations as the compiler's IEEE square root (a v_rsq_f64 estimate,
-// one Goldschmidt step for the root and its half-reciprocal, two residual
-// corrections) and IEEE division (v_rcp_f64, two Newton steps, one residual
-// correction), without the parts that act only outside that range: the
-// root's scaling below 2^-767 and its zero and infinity cases, and the
-// division's operand scaling and special-value fix-up (with numerator 1 and a
-// root within 2^+-256 neither applies).  The result is the same bits; the
-// GPU tests check it against the oracle's 1.0 / sqrt(v) on every pair of
-// the direct sum and the traversal.  Outside the range: 1.0 / sqrt(v).
-__device__ inline double inv_sqrt_ieee(double v) {
-    if (!(v >= 0x1p-512 && v <= 0x1p512)) return 1.0 / sqrt(v);
-    const double g = __builtin_amdgcn_rsq(v);
-    double s = v * g;
-    double hh = g * 0.5;
-    const double r = fma(-hh, s, 0.5);
-    s = fma(s, r, s);
-    hh = fma(hh, r, hh);
-    double d = fma(-s, s, v);
-    s = fma(d, hh, s);
-    d = fma(-s, s, v);
-    s = fma(d, hh, s);  // sqrt(v), correctly rounded
-    double y = __builtin_amdgcn_rcp(s);
-    double e = fma(-s, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-s, y, 1.0);
-    y = fma(y, e, y);
-    e = fma(-s, y, 1.0);  // the quotient estimate 1.0 * y is y
-    return fma(e, y, y);
-}
-
-// ---------------------------------------------------------------------------
 // FP64 direct sum, reference order.  The sum over j = 0..n-1 of a target must
 // run in that order to be bit-exact, but only its three additions per pair
 // are sequential; the pair terms m_j (r_ij den_ij), with their IEEE division

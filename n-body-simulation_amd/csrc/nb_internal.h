@@ -359,6 +359,63 @@ void launch_ens_items(uint32_t nq, const uint32_t *qorder, const uint32_t *items
                       uint32_t *item_off, uint32_t *cursor, uint32_t *status, hipStream_t s);
 void launch_ens_forces(const EnsForceArgs &a, uint32_t waves, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Correctly rounded FP64 square root and division for the reference-order
+// kernels, within ranges where the compiler's IEEE sequences need none of
+// their range handling.  The compiler's square root takes a v_rsq_f64
+// estimate, one Goldschmidt step for the root and its half-reciprocal and two
+// residual corrections, scaling inputs below 2^-767 and passing zero and
+// infinity through; its division takes v_rcp_f64 of the divisor, two Newton
+// steps and one residual correction of the quotient, with operand scaling
+// (v_div_scale_f64) and a special-value fix-up (v_div_fixup_f64) that change
+// nothing while dividend, divisor and quotient are normal and their exponents
+// differ by less than 768.  The cores below run the same operations without
+// that handling, so inside the guarded ranges they give the same bits; the
+// GPU tests compare every pair of the direct sum, the traversal and the
+// energy with the oracle's IEEE operations.  Outside: the plain operators.
+__device__ inline double sqrt_core(double v) {  // v in [2^-767, 2^1023)
+    const double g = __builtin_amdgcn_rsq(v);
+    double s = v * g;
+    double h = g * 0.5;
+    const double r = fma(-h, s, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    double d = fma(-s, s, v);
+    s = fma(d, h, s);
+    d = fma(-s, s, v);
+    return fma(d, h, s);
+}
+__device__ inline double div_core(double a, double b) {  // |a|, |b| in [2^-300, 2^300]
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-b, y, 1.0);
+    y = fma(y, e, y);
+    const double q = a * y;
+    e = fma(-b, q, a);
+    return fma(e, y, q);
+}
+// 1.0 / sqrt(v), both operations correctly rounded (the reference's
+// inverse square root); the core for v in [2^-512, 2^512]
+__device__ inline double inv_sqrt_ieee(double v) {
+    if (!(v >= 0x1p-512 && v <= 0x1p512)) return 1.0 / sqrt(v);
+    const double s = sqrt_core(v);
+    double y = __builtin_amdgcn_rcp(s);
+    double e = fma(-s, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-s, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-s, y, 1.0);  // the quotient 1.0 * y is y
+    return fma(e, y, y);
+}
+// a / sqrt(v), both operations correctly rounded; the cores for v in
+// [2^-600, 2^600] and |a| in [2^-300, 2^300]
+__device__ inline double div_sqrt_ieee(double a, double v) {
+    const double fa = fabs(a);
+    if (!(v >= 0x1p-600 && v <= 0x1p600 && fa >= 0x1p-300 && fa <= 0x1p300)) return a / sqrt(v);
+    return div_core(a, sqrt_core(v));
+}
+
 inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Host: canonical depth-first dump of an octree (children in digit order),
