@@ -27,14 +27,15 @@ __device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status,
 // FP64 direct sum, reference order.  The sum over j = 0..n-1 of a target must
 // run in that order to be bit-exact, but only its three additions per pair
 // are sequential; the pair terms m_j (r_ij den_ij), with their IEEE division
-// and square root, are not.  A workgroup owns 64 targets (the lanes of wave
-// 0).  Sources are staged in LDS kF64Stage at a time; for each chunk of
-// kF64Chunk sources all sixteen waves compute the terms into one of two LDS
-// tiles while wave 0 adds the previous tile in source order, eight terms'
-// reads in flight ahead of their additions.  (16,384 bodies: 1.26 ms with 8
-// waves and one read per addition, 1.09 ms with 16 waves, 0.75 ms with the
-// reads ahead, 0.72 ms with 48-source chunks instead of 32, 0.66 ms with
-// inv_sqrt_ieee.)
+// and square root, are not.  A workgroup owns 64 targets (one per lane).
+// Sources are staged in LDS kF64Stage at a time; for each chunk of kF64Chunk
+// sources all sixteen waves compute the terms into one of two LDS tiles while
+// waves 0, 1 and 2 add the x, y and z components of the previous tile in
+// source order, eight terms' reads in flight ahead of their additions.
+// (16,384 bodies: 1.26 ms with 8 waves and one read per addition by wave 0,
+// 1.09 ms with 16 waves, 0.75 ms with the reads ahead, 0.72 ms with
+// 48-source chunks instead of 32, 0.66 ms with inv_sqrt_ieee, 0.58 ms with
+// the three components added by three waves instead of one.)
 constexpr int kDirectF64Tile = 64;                  // targets per workgroup
 constexpr int kF64Threads = 1024;                   // 16 waves computing pair terms
 constexpr int kF64Chunk = 48;                       // sources per term tile
@@ -42,44 +43,34 @@ constexpr int kF64PerWave = kF64Chunk / (kF64Threads / 64);
 constexpr int kF64Stage = 240;                      // sources per LDS stage
 static_assert(kF64Stage % kF64Chunk == 0, "whole chunks per stage");
 
-// acc (valid in wave 0) = sum over j in order of m_j (r_ij den_ij) for the
-// lane's target at (px, py, pz)
-__device__ inline void direct_f64_body(uint32_t n, const double *__restrict__ x,
-                                       const double *__restrict__ y,
-                                       const double *__restrict__ z,
-                                       const double *__restrict__ m, double eps2, double px,
-                                       double py, double pz, double *acc) {
+// Returns, in waves 0, 1 and 2, component 0, 1 or 2 (x, y, z) of the sum
+// over j in order of m_j (r_ij den_ij) for the lane's target at (px, py, pz):
+// the three sums are independent, so three waves on three SIMDs add them.
+__device__ inline double direct_f64_body(uint32_t n, const double *__restrict__ x,
+                                         const double *__restrict__ y,
+                                         const double *__restrict__ z,
+                                         const double *__restrict__ m, double eps2, double px,
+                                         double py, double pz) {
     __shared__ double4 src[kF64Stage];
     __shared__ double tx[2][kF64Chunk][kDirectF64Tile], ty[2][kF64Chunk][kDirectF64Tile],
         tz[2][kF64Chunk][kDirectF64Tile];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    double acc_x = 0, acc_y = 0, acc_z = 0;
-    uint32_t pending = 0;  // sources in the tile wave 0 still has to add
+    double (*const own)[kF64Chunk][kDirectF64Tile] = w == 0 ? tx : (w == 1 ? ty : tz);
+    double acc = 0;
+    uint32_t pending = 0;  // sources in the tile waves 0-2 still have to add
     int buf = 0;
-    // wave 0: add tile `b` (cnt sources) in source order
+    // waves 0-2: add their component of tile `b` (cnt sources) in source order
     auto add_tile = [&](int b, uint32_t cnt) {
         uint32_t jj = 0;
         // eight terms' LDS reads in flight before their (ordered) additions
         for (; jj + 8 <= cnt; jj += 8) {
-            double vx[8], vy[8], vz[8];
+            double v[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                vx[u] = tx[b][jj + u][lane];
-                vy[u] = ty[b][jj + u][lane];
-                vz[u] = tz[b][jj + u][lane];
-            }
+            for (int u = 0; u < 8; ++u) v[u] = own[b][jj + u][lane];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                acc_x += vx[u];
-                acc_y += vy[u];
-                acc_z += vz[u];
-            }
+            for (int u = 0; u < 8; ++u) acc += v[u];
         }
-        for (; jj < cnt; ++jj) {
-            acc_x += tx[b][jj][lane];
-            acc_y += ty[b][jj][lane];
-            acc_z += tz[b][jj][lane];
-        }
+        for (; jj < cnt; ++jj) acc += own[b][jj][lane];
     };
     for (uint32_t k0 = 0; k0 < n; k0 += kF64Stage) {
         const uint32_t staged = min((uint32_t)kF64Stage, n - k0);
@@ -105,17 +96,15 @@ __device__ inline void direct_f64_body(uint32_t n, const double *__restrict__ x,
                     tz[buf][jj][lane] = sj.w * (r_z * denominator);
                 }
             }
-            // wave 0 adds the previous tile (written before the last barrier)
-            if (w == 0 && pending) add_tile(buf ^ 1, pending);
+            // waves 0-2 add the previous tile (written before the last barrier)
+            if (w < 3 && pending) add_tile(buf ^ 1, pending);
             __syncthreads();
             pending = cnt;
             buf ^= 1;
         }
     }
-    if (w == 0 && pending) add_tile(buf ^ 1, pending);
-    acc[0] = acc_x;
-    acc[1] = acc_y;
-    acc[2] = acc_z;
+    if (w < 3 && pending) add_tile(buf ^ 1, pending);
+    return acc;
 }
 
 __global__ void __launch_bounds__(kF64Threads) k_direct_f64(
@@ -125,13 +114,9 @@ __global__ void __launch_bounds__(kF64Threads) k_direct_f64(
     const uint32_t i = blockIdx.x * kDirectF64Tile + (threadIdx.x & 63u);
     const bool valid = i < n;
     const uint32_t ii = valid ? i : 0;
-    double acc[3];
-    direct_f64_body(n, x, y, z, m, eps2, x[ii], y[ii], z[ii], acc);
-    if (valid && threadIdx.x < 64u) {
-        ax[i] = acc[0] * G;
-        ay[i] = acc[1] * G;
-        az[i] = acc[2] * G;
-    }
+    const double acc = direct_f64_body(n, x, y, z, m, eps2, x[ii], y[ii], z[ii]);
+    const uint32_t w = threadIdx.x >> 6;
+    if (valid && w < 3) (w == 0 ? ax : (w == 1 ? ay : az))[i] = acc * G;
 }
 
 __global__ void __launch_bounds__(kF64Threads) k_direct_targets_f64(
@@ -141,13 +126,9 @@ __global__ void __launch_bounds__(kF64Threads) k_direct_targets_f64(
     const uint32_t t = blockIdx.x * kDirectF64Tile + (threadIdx.x & 63u);
     const bool valid = t < nt;
     const uint32_t i = valid ? targets[t] : 0;
-    double acc[3];
-    direct_f64_body(n, x, y, z, m, eps2, x[i], y[i], z[i], acc);
-    if (valid && threadIdx.x < 64u) {
-        ax[t] = acc[0] * G;
-        ay[t] = acc[1] * G;
-        az[t] = acc[2] * G;
-    }
+    const double acc = direct_f64_body(n, x, y, z, m, eps2, x[i], y[i], z[i]);
+    const uint32_t w = threadIdx.x >> 6;
+    if (valid && w < 3) (w == 0 ? ax : (w == 1 ? ay : az))[t] = acc * G;
 }
 
 // ---------------------------------------------------------------------------
