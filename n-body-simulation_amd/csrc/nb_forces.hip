@@ -398,7 +398,8 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 // batches as seven registers each; 15.2 ms with four such batches, the
 // opening test decided from an estimate and a node's three loads issued
 // together; with a batch as one word, 14.4, 13.3, 12.5, 12.5 and 12.7 ms
-// with 2, 4, 8, 12 and 16 batches in registers.
+// with 2, 4, 8, 12 and 16 batches in registers.  Loading the record of the
+// stack top one visit ahead was slower: 13.8 ms (85 VGPRs, five waves).
 constexpr int kExactThreads = 256;
 
 __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
