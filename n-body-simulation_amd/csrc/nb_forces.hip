@@ -23,6 +23,8 @@ namespace {
 
 __device__ inline void raise_flag(TreeHdr *h, uint32_t f) { atomicOr(&h->status, f); }
 
+constexpr uint32_t kXcds = 8;  // XCDs of an MI355X; workgroups are dealt round-robin
+
 // ---------------------------------------------------------------------------
 // FP64 direct sum, reference order.  The sum over j = 0..n-1 of a target must
 // run in that order to be bit-exact, but only its three additions per pair
@@ -400,6 +402,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 // together; with a batch as one word, 14.4, 13.3, 12.5, 12.5 and 12.7 ms
 // with 2, 4, 8, 12 and 16 batches in registers.  Loading the record of the
 // stack top one visit ahead was slower: 13.8 ms (85 VGPRs, five waves).
+// Workgroups dealt to the XCDs in contiguous runs of bodies: 10.7 ms.
 constexpr int kExactThreads = 256;
 
 __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
@@ -408,7 +411,12 @@ __global__ void __launch_bounds__(kExactThreads) k_bh_exact(
     const double *__restrict__ edge, const int32_t *__restrict__ ref, double G, double eps2,
     double theta, uint32_t *stack, uint32_t cap, uint32_t icap, double *ax, double *ay,
     double *az, uint32_t *inter, TreeHdr *h) {
-    const uint32_t id = blockIdx.x * kExactThreads + threadIdx.x;
+    // Workgroup b runs on XCD b mod 8: give each XCD a contiguous run of the
+    // octree order, so that its L2 holds the records of one region of the tree.
+    const uint32_t nblk = gridDim.x, xcd = blockIdx.x % kXcds;
+    const uint32_t per = nblk / kXcds, extra = nblk % kXcds;
+    const uint32_t lb = xcd * per + min(xcd, extra) + blockIdx.x / kXcds;
+    const uint32_t id = lb * kExactThreads + threadIdx.x;
     if (id >= n) return;
     const uint32_t i = order ? order[id] : id;
     const int32_t self_ref = ref_of_body(i);
@@ -552,7 +560,6 @@ constexpr int kPop = 1;
 constexpr int kFlushSpan = 2 * kGroup;
 constexpr int kListCap = kFlushSpan + kGroup * kPop;
 constexpr int kGroupBlocks = 8192;  // persistent waves
-constexpr uint32_t kXcds = 8;       // XCDs of an MI355X; workgroups are dealt round-robin
 constexpr uint32_t kQueueStride = 32;  // one 128-byte line per group-queue counter
 static_assert(kXcds * kQueueStride <= TreeBufs::kGroupQueueWords, "queue counters fit TreeBufs::gq");
 
