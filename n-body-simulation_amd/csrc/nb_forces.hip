@@ -44,6 +44,8 @@ constexpr int kF64Chunk = 48;                       // sources per term tile
 constexpr int kF64PerWave = kF64Chunk / (kF64Threads / 64);
 constexpr int kF64Stage = 240;                      // sources per LDS stage
 static_assert(kF64Stage % kF64Chunk == 0, "whole chunks per stage");
+static_assert(kF64Threads >= 3 * 64 && kF64Chunk % (kF64Threads / 64) == 0,
+              "three adder waves; whole terms per wave");
 
 // Returns, in waves 0, 1 and 2, component 0, 1 or 2 (x, y, z) of the sum
 // over j in order of m_j (r_ij den_ij) for the lane's target at (px, py, pz):
