@@ -624,6 +624,36 @@ def test_energy_bitexact(gpu, oracle, kind, n):
         assert np.float64(e[k]).view(np.int64) == np.float64(r[k]).view(np.int64), k
 
 
+# The FP64 kernels take 1/sqrt(v) and a/sqrt(v) through the IEEE sequences
+# without their range handling only for v (and a) inside fixed ranges
+# (den^3 in [2^-512, 2^512] for the forces, v in [2^-600, 2^600] and |a| in
+# [2^-300, 2^300] for the energy) and call the plain operators outside.  Two
+# bodies per system, so that each pair term decides a result's bits on its
+# own: separations on both sides of every bound, against the oracle.
+@pytest.mark.parametrize("sep,m1", [(1e-40, 1.0), (1e-20, 1.0), (1.0, 1.0), (1e20, 1.0),
+                                    (1e40, 1.0), (1e-100, 1.0), (1e100, 1.0),
+                                    (1.0, 1e-60), (3e-10, 1e-70)])
+def test_ieee_sequences_both_sides_of_their_range(gpu, oracle, sep, m1):
+    d = dict(x=np.array([0.0, sep]), y=np.array([0.0, 0.5 * sep]), z=np.zeros(2),
+             vx=np.array([1e-3, -2e-3]), vy=np.zeros(2), vz=np.zeros(2),
+             m=np.array([m1, 2.5]))
+    eps2 = 1e-90  # self terms stay finite (0 * 1/sqrt(eps2^3)); far below every d^2 here
+    if sep < 1e-50:  # den^3 would underflow: forces only through the energy
+        eps2 = None
+    if eps2 is not None:
+        with ctx_for(gpu, d, eps2=eps2) as c:
+            ax, ay, az = c.forces_direct()
+        rx, ry, rz = oracle.naive_accel(d["x"], d["y"], d["z"], d["m"], eps2=eps2)
+        for a, r in ((ax, rx), (ay, ry), (az, rz)):
+            assert np.all(np.isfinite(r))
+            np.testing.assert_array_equal(bits(a), bits(r))
+    with ctx_for(gpu, d) as c:
+        e = c.energy()
+    r = oracle.energy(d["x"], d["y"], d["z"], d["vx"], d["vy"], d["vz"], d["m"])
+    for k in ("kinetic", "potential", "total"):
+        assert np.float64(e[k]).view(np.int64) == np.float64(r[k]).view(np.int64), k
+
+
 def test_bh_forces_repeatable(gpu):
     # the traversal must not depend on state left by an earlier traversal of
     # the same octree (work-queue cursors, accumulators)
