@@ -35,19 +35,42 @@ __global__ void __launch_bounds__(kThreads) k_energy_terms(
         ekin[j] = 0.5 * mj * v;
     }
     double ep = 0;
+    // The numerator G m_i m_j is (G m_i) m_j, so G m_i is staged with the
+    // source.  When G m_i and m_j both lie in [2^-150, 2^150] (the whole tile
+    // and this lane), the numerator lies in div_core's range and only the
+    // squared distance needs checking.
+    const bool mj_ok = fabs(mj) >= 0x1p-150 && fabs(mj) <= 0x1p150;
     const uint32_t last = min(n, (tile + 1) * kThreads);  // bodies i < j of this block
     for (uint32_t k = 0; k < last; k += kThreads) {
         const uint32_t i = k + threadIdx.x;
-        if (i < n) src[threadIdx.x] = make_double4(x[i], y[i], z[i], m[i]);
-        __syncthreads();
+        bool gm_ok = true;
+        if (i < n) {
+            const double gm = G * m[i];
+            gm_ok = fabs(gm) >= 0x1p-150 && fabs(gm) <= 0x1p150;
+            src[threadIdx.x] = make_double4(x[i], y[i], z[i], gm);
+        }
+        const bool tile_ok = __syncthreads_and(gm_ok);
         if (valid && j > k) {
             const uint32_t cnt = min((uint32_t)kThreads, j - k);  // sources i = k + ii < j
-            for (uint32_t ii = 0; ii < cnt; ++ii) {
-                const double4 si = src[ii];
-                const double r_x = pxj - si.x;
-                const double r_y = pyj - si.y;
-                const double r_z = pzj - si.z;
-                ep += div_sqrt_ieee(G * si.w * mj, r_x * r_x + r_y * r_y + r_z * r_z);
+            if (tile_ok && mj_ok) {
+                for (uint32_t ii = 0; ii < cnt; ++ii) {
+                    const double4 si = src[ii];
+                    const double r_x = pxj - si.x;
+                    const double r_y = pyj - si.y;
+                    const double r_z = pzj - si.z;
+                    const double a = si.w * mj;
+                    const double v = r_x * r_x + r_y * r_y + r_z * r_z;
+                    ep += (v >= 0x1p-600 && v <= 0x1p600) ? div_core(a, sqrt_core(v))
+                                                          : a / sqrt(v);
+                }
+            } else {
+                for (uint32_t ii = 0; ii < cnt; ++ii) {
+                    const double4 si = src[ii];
+                    const double r_x = pxj - si.x;
+                    const double r_y = pyj - si.y;
+                    const double r_z = pzj - si.z;
+                    ep += div_sqrt_ieee(si.w * mj, r_x * r_x + r_y * r_y + r_z * r_z);
+                }
             }
         }
         __syncthreads();
