@@ -153,6 +153,11 @@ constexpr int kSplitPerLane = 2;  // targets per lane (each LDS read feeds 2 pai
 constexpr uint32_t kSplitTargets = kSplitThreads * kSplitPerLane;
 constexpr int kSplitTile = 256;   // sources per LDS stage
 constexpr uint32_t kSplitBlocks = 2048;  // blocks a launch aims for (2 per SIMD)
+// sources per interleaved FP64 step (x kSplitPerLane targets): 124 VGPRs, four
+// waves per SIMD; 2 (72 VGPRs, seven waves) and 8 (240) measured within 2 %,
+// five waves forced (spilling) 15 % slower
+constexpr int kF64fSources = 4;
+static_assert(kSplitTile % kF64fSources == 0, "whole steps per stage");
 
 __device__ inline void pair_f32(const float4 s, float px, float py, float pz, float eps2,
                                 float &fx, float &fy, float &fz) {
@@ -344,16 +349,16 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
             for (int q = 0; q < kSplitPerLane; ++q)
                 pair_f64f(sj, px[q], py[q], pz[q], eps2, ax[q], ay[q], az[q]);
         };
-        // Four sources x kSplitPerLane targets, stage by stage across the
+        // kF64fSources sources x kSplitPerLane targets, stage by stage across the
         // pairs: the compiler otherwise runs each pair's chain (the estimate,
         // its five-step correction, the force) to the end before the next
         // pair starts.  The same operations per pair as pair_f64f, and each
         // target's sum still runs over the sources in order: bit-identical.
-        auto pairs4 = [&](int jj) {
-            constexpr int P = 4 * kSplitPerLane;
+        auto pairs_s = [&](int jj) {
+            constexpr int P = kF64fSources * kSplitPerLane;
             double dx[P], dy[P], dz[P], r2[P], yv[P], w[P];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kF64fSources; ++u) {
                 const double4 sj = tile[jj + u];
 #pragma unroll
                 for (int q = 0; q < kSplitPerLane; ++q) {
@@ -368,7 +373,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
 #pragma unroll
             for (int p = 0; p < P; ++p) yv[p] = rsqrt_f64(r2[p]);
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+            for (int u = 0; u < kF64fSources; ++u)
 #pragma unroll
                 for (int q = 0; q < kSplitPerLane; ++q) {
                     const int p = u * kSplitPerLane + q;
@@ -381,7 +386,7 @@ __global__ void __launch_bounds__(kSplitThreads) k_direct_f64f(
         // the split's last stage may be partial: only its staged sources
         const uint32_t cnt = min((uint32_t)kSplitTile, j1 - k);  // uniform
         if (cnt == (uint32_t)kSplitTile) {
-            for (int jj = 0; jj < kSplitTile; jj += 4) pairs4(jj);
+            for (int jj = 0; jj < kSplitTile; jj += kF64fSources) pairs_s(jj);
         } else {
             for (uint32_t jj = 0; jj < cnt; ++jj) pairs((int)jj);
         }
