@@ -72,10 +72,13 @@ CPU_SAMPLE_S = 10.0
 #  - 4M Plummer, theta 0.37375: 5,775,771 KiB x 2 read + 528,162 KiB written
 #    (profiles/bh_plummer_4m_hbm_counters.md);
 #  - 1M disk, theta 0.5: 932,786 KiB x 2 read + 133,202 KiB written
-#    (profiles/bh_disk_1m_hbm_counters.md).
+#    (profiles/bh_disk_1m_hbm_counters.md);
+#  - 1M disk, FP64 per-body traversal (k_bh_exact): 1,106,567 KiB x 2 read +
+#    172,409 KiB written (profiles/bh_disk_1m_exact_kernel_stats.md).
 TRAFFIC_BYTES = {
     ("bh_plummer_4m", 0.37375): 12.370e9,
     ("bh_disk_1m", 0.5): 2.047e9,
+    ("bh_disk_1m_exact", 0.5): 2.443e9,
 }
 
 WORKLOADS = {
